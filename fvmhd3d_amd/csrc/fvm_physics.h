@@ -1,0 +1,357 @@
+// Device-side physics of the Voronoi-face pipeline, templated on the
+// arithmetic type T (double for the FP64 path, float for the FP32 face path).
+//
+// Every expression keeps the reference's evaluation order so that, built with
+// -ffp-contract=off, the FP64 instantiation reproduces the reference's double
+// arithmetic operation for operation.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fvm {
+
+enum { NF = 10 };
+enum { DENS = 0, VELX, VELY, VELZ, ETHM, BX, BY, BZ, PSI, ENTR };
+enum { MASS = 0, MOMX, MOMY, MOMZ, ENER, WBX, WBY, WBZ, MPSI, MENTR };
+enum { NO_BOUNDARY = 0, REFLECTING = 1, OUTFLOW = 2, DIOD = 3, INFLOW = 4 };
+enum { LIM_SHIPPED = 0, LIM_MINMOD = 1, LIM_SMOOTH = 2 };
+
+constexpr double REF_HUGE = 1e30;  // Boundary.h:5-7
+constexpr double REF_PI = 3.14159265358979323846;
+
+template <typename T> __device__ __forceinline__ T sqr(T x) { return x * x; }
+// std::min / std::max argument conventions
+template <typename T> __device__ __forceinline__ T smin(T a, T b) { return (b < a) ? b : a; }
+template <typename T> __device__ __forceinline__ T smax(T a, T b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
+
+// Problem_compute_pressure (Problem.cpp:234-237)
+template <typename T> __device__ __forceinline__ T pressure(T gamma, T ethm) {
+  return gamma > T(1.0) ? (gamma - T(1.0)) * ethm : ethm;
+}
+
+// Fluid::to_primitive (Fluid.h:61-98), total-energy form
+__device__ __forceinline__ void to_primitive(const double *U, double volume, double *W) {
+  const double iM = 1.0 / U[MASS];
+  const double iV = 1.0 / volume;
+  W[DENS] = U[MASS] * iV;
+  W[VELX] = U[MOMX] * iM;
+  W[VELY] = U[MOMY] * iM;
+  W[VELZ] = U[MOMZ] * iM;
+  W[BX] = U[WBX] * iV;
+  W[BY] = U[WBY] * iV;
+  W[BZ] = U[WBZ] * iV;
+  W[PSI] = U[MPSI] * iM;
+  W[ENTR] = U[MENTR] * iM;
+  W[ETHM] = U[ENER] * iV - 0.5 * (sqr(W[BX]) + sqr(W[BY]) + sqr(W[BZ])) -
+            0.5 * (sqr(W[VELX]) + sqr(W[VELY]) + sqr(W[VELZ])) * W[DENS];
+}
+
+// Fluid::to_conservative (Fluid.h:100-135)
+__device__ __forceinline__ void to_conservative(const double *W, double volume, double *U) {
+  const double V = volume;
+  const double M = W[DENS] * volume;
+  U[MASS] = W[DENS] * V;
+  U[MOMX] = W[VELX] * M;
+  U[MOMY] = W[VELY] * M;
+  U[MOMZ] = W[VELZ] * M;
+  U[WBX] = W[BX] * V;
+  U[WBY] = W[BY] * V;
+  U[WBZ] = W[BZ] * V;
+  U[MPSI] = W[PSI] * M;
+  U[MENTR] = W[ENTR] * M;
+  U[ENER] = W[ETHM] * V + 0.5 * (sqr(W[BX]) + sqr(W[BY]) + sqr(W[BZ])) * V +
+            0.5 * (sqr(W[VELX]) + sqr(W[VELY]) + sqr(W[VELZ])) * M;
+}
+
+// Phi of the limiter modes (computeReconstruction.cpp:5-17, slopeLimiter.cpp:5-18,
+// MPI/compute_reconstruction.cpp:5-15)
+__device__ __forceinline__ double phi(int mode, double x) {
+  if (mode == LIM_SHIPPED) return 1.0;
+  if (mode == LIM_MINMOD) return smin(1.0, x);
+  const double fac = 0.2, eps1 = 1.0e-16;
+  return (x * x + 2.0 * x + fac * eps1) / (x * x + x + 2.0 + fac * eps1);
+}
+// computeReconstruction.cpp:19-40
+__device__ __forceinline__ double limiter_rec(int mode, double dWi, double mn, double mx) {
+  if (dabs(dWi) <= 1.0e-16 * smax(dabs(mx), dabs(mn))) return 1.0;
+  const double invWi = 1.0 / dWi;
+  if (dWi < 0.0) return phi(mode, mn * invWi);
+  return phi(mode, mx * invWi);
+}
+// slopeLimiter.cpp:20-25
+__device__ __forceinline__ double limiter_sl(int mode, double dWi, double mn, double mx) {
+  if (dabs(dWi) <= 1.0e-16 * smax(dabs(mx), dabs(mn))) return 1.0;
+  if (dWi < 0.0) return phi(mode, mn / dWi);
+  return phi(mode, mx / dWi);
+}
+
+// riemannSolver_HLLD (computeFlux.cpp:178-425): HLLD sampled at the face speed w,
+// flux of the moving interface in the a*U** + b*U* + c*U form.
+template <typename T>
+__device__ __forceinline__ void riemann_hlld(T gamma_gas, T *flux, const T Bx, const T w,
+                                             const T dens_L, const T pres_L, const T ethm_L,
+                                             const T velx_L, const T vely_L, const T velz_L,
+                                             const T By_L, const T Bz_L, const T dens_R,
+                                             const T pres_R, const T ethm_R, const T velx_R,
+                                             const T vely_R, const T velz_R, const T By_R,
+                                             const T Bz_R) {
+  const T c0 = T(0.0), c1 = T(1.0);
+  const T signBx = Bx == c0 ? c0 : (Bx > c0 ? +c1 : -c1);
+  const T momx_L = dens_L * velx_L;
+  const T momy_L = dens_L * vely_L;
+  const T momz_L = dens_L * velz_L;
+  const T momx_R = dens_R * velx_R;
+  const T momy_R = dens_R * vely_R;
+  const T momz_R = dens_R * velz_R;
+  const T chalf = T(0.5);
+  const T B2_L = sqr(Bx) + sqr(By_L) + sqr(Bz_L);
+  const T v2_L = sqr(velx_L) + sqr(vely_L) + sqr(velz_L);
+  const T etot_L = ethm_L + chalf * (dens_L * v2_L + B2_L);
+  const T B2_R = sqr(Bx) + sqr(By_R) + sqr(Bz_R);
+  const T v2_R = sqr(velx_R) + sqr(vely_R) + sqr(velz_R);
+  const T etot_R = ethm_R + chalf * (dens_R * v2_R + B2_R);
+  const T gpl = gamma_gas * pres_L;
+  const T gpr = gamma_gas * pres_R;
+  const T gpbl = gpl + B2_L;
+  const T gpbr = gpr + B2_R;
+  const T cfl2 = gpbl / dens_L;
+  const T cfr2 = gpbr / dens_R;
+  const T cfmax = dsqrt(smax(cfl2, cfr2));
+  const T S_L = smin(velx_L, velx_R) - cfmax;
+  const T S_R = smax(velx_L, velx_R) + cfmax;
+  const T pT_L = pres_L + chalf * B2_L;
+  const T pT_R = pres_R + chalf * B2_R;
+  const T iSM = c1 / ((S_R - velx_R) * dens_R - (S_L - velx_L) * dens_L);
+  const T S_M = iSM * ((S_R - velx_R) * momx_R - (S_L - velx_L) * momx_L - pT_R + pT_L);
+  const T ipTs = c1 / ((S_R - velx_R) * dens_R - (S_L - velx_L) * dens_L);
+  const T pT_s = ipTs * ((S_R - velx_R) * dens_R * pT_L - (S_L - velx_L) * dens_L * pT_R +
+                         dens_L * dens_R * (S_R - velx_R) * (S_L - velx_L) * (velx_R - velx_L));
+  const T velx_L_s = S_M, velx_L_ss = S_M, velx_R_s = S_M, velx_R_ss = S_M;
+  const T B2x = Bx * Bx;
+  const T iSLmSM = c1 / (S_L - S_M);
+  const T iSRmSM = c1 / (S_R - S_M);
+  const T dens_L_s = dens_L * (S_L - velx_L) * iSLmSM;
+  const T dens_R_s = dens_R * (S_R - velx_R) * iSRmSM;
+  const T divL = dens_L * (S_L - velx_L) * (S_L - S_M) - B2x;
+  const T divR = dens_R * (S_R - velx_R) * (S_R - S_M) - B2x;
+  const T idivL = (divL != c0) ? c1 / divL : c0;
+  const T idivR = (divR != c0) ? c1 / divR : c0;
+  const T vely_L_s = vely_L - Bx * By_L * (S_M - velx_L) * idivL;
+  const T velz_L_s = velz_L - Bx * Bz_L * (S_M - velx_L) * idivL;
+  const T By_L_s = By_L * (dens_L * sqr(S_L - velx_L) - B2x) * idivL;
+  const T Bz_L_s = Bz_L * (dens_L * sqr(S_L - velx_L) - B2x) * idivL;
+  const T vely_R_s = vely_R - Bx * By_R * (S_M - velx_R) * idivR;
+  const T velz_R_s = velz_R - Bx * Bz_R * (S_M - velx_R) * idivR;
+  const T By_R_s = By_R * (dens_R * sqr(S_R - velx_R) - B2x) * idivR;
+  const T Bz_R_s = Bz_R * (dens_R * sqr(S_R - velx_R) - B2x) * idivR;
+  const T vB_L = velx_L * Bx + vely_L * By_L + velz_L * Bz_L;
+  const T vB_L_s = velx_L_s * Bx + vely_L_s * By_L_s + velz_L_s * Bz_L_s;
+  const T etot_L_s =
+      ((S_L - velx_L) * etot_L - pT_L * velx_L + pT_s * S_M + Bx * (vB_L - vB_L_s)) * iSLmSM;
+  const T vB_R = velx_R * Bx + vely_R * By_R + velz_R * Bz_R;
+  const T vB_R_s = velx_R_s * Bx + vely_R_s * By_R_s + velz_R_s * Bz_R_s;
+  const T etot_R_s =
+      ((S_R - velx_R) * etot_R - pT_R * velx_R + pT_s * S_M + Bx * (vB_R - vB_R_s)) * iSRmSM;
+  const T dens_L_ss = dens_L_s, dens_R_ss = dens_R_s;
+  const T sDens_L_s = dsqrt(dens_L_s);
+  const T sDens_R_s = dsqrt(dens_R_s);
+  const T S_L_s = S_M - dabs(Bx / sDens_L_s);
+  const T S_R_s = S_M + dabs(Bx / sDens_R_s);
+  const T idsqroot = c1 / (sDens_L_s + sDens_R_s);
+  const T vely_ss =
+      idsqroot * (sDens_L_s * vely_L_s + sDens_R_s * vely_R_s + (By_R_s - By_L_s) * signBx);
+  const T velz_ss =
+      idsqroot * (sDens_L_s * velz_L_s + sDens_R_s * velz_R_s + (Bz_R_s - Bz_L_s) * signBx);
+  const T By_ss = idsqroot * (sDens_L_s * By_R_s + sDens_R_s * By_L_s +
+                              sDens_L_s * sDens_R_s * (vely_R_s - vely_L_s) * signBx);
+  const T Bz_ss = idsqroot * (sDens_L_s * Bz_R_s + sDens_R_s * Bz_L_s +
+                              sDens_L_s * sDens_R_s * (velz_R_s - velz_L_s) * signBx);
+  const T vB_L_ss = velx_L_ss * Bx + vely_ss * By_ss + velz_ss * Bz_ss;
+  const T etot_L_ss = etot_L_s - sDens_L_s * (vB_L_s - vB_L_ss) * signBx;
+  const T vB_R_ss = velx_R_ss * Bx + vely_ss * By_ss + velz_ss * Bz_ss;
+  const T etot_R_ss = etot_R_s + sDens_R_s * (vB_R_s - vB_R_ss) * signBx;
+  const T Fdens_L = dens_L * velx_L;
+  const T Fmomx_L = momx_L * velx_L + pT_L - B2x;
+  const T Fmomy_L = momy_L * velx_L - Bx * By_L;
+  const T Fmomz_L = momz_L * velx_L - Bx * Bz_L;
+  const T Fetot_L = etot_L * velx_L + pT_L * velx_L - Bx * vB_L;
+  const T Fdens_R = dens_R * velx_R;
+  const T Fmomx_R = momx_R * velx_R + pT_R - B2x;
+  const T Fmomy_R = momy_R * velx_R - Bx * By_R;
+  const T Fmomz_R = momz_R * velx_R - Bx * Bz_R;
+  const T Fetot_R = etot_R * velx_R + pT_R * velx_R - Bx * vB_R;
+  const T Fby_L = By_L * velx_L - Bx * vely_L;
+  const T Fbz_L = Bz_L * velx_L - Bx * velz_L;
+  const T Fby_R = By_R * velx_R - Bx * vely_R;
+  const T Fbz_R = Bz_R * velx_R - Bx * velz_R;
+  const bool w_lt_SL = w <= S_L;
+  const bool w_le_SLs = w <= S_L_s;
+  const bool w_le_SM = w <= S_M;
+  const bool w_le_SRs = w <= S_R_s;
+  const bool w_le_SR = w <= S_R;
+  const T cnull = c0;
+  const T a = (w_le_SLs ? cnull : (w_le_SM ? S_L_s - w : (w_le_SRs ? S_R_s - w : cnull)));
+  const T b =
+      (w_lt_SL ? cnull
+               : (w_le_SLs ? +S_L - w
+                           : (w_le_SM ? -S_L_s + S_L
+                                      : (w_le_SRs ? -S_R_s + S_R : (w_le_SR ? +S_R - w : cnull)))));
+  const T c =
+      (w_lt_SL ? -w : (w_le_SLs ? -S_L : (w_le_SM ? -S_L : (w_le_SRs ? -S_R : (w_le_SR ? -S_R : -w)))));
+  // left/right selection by the contact (w <= S_M)
+  const T fdens = w_le_SM ? Fdens_L : Fdens_R;
+  const T fetot = w_le_SM ? Fetot_L : Fetot_R;
+  const T fmomx = w_le_SM ? Fmomx_L : Fmomx_R;
+  const T fmomy = w_le_SM ? Fmomy_L : Fmomy_R;
+  const T fmomz = w_le_SM ? Fmomz_L : Fmomz_R;
+  const T fby = w_le_SM ? Fby_L : Fby_R;
+  const T fbz = w_le_SM ? Fbz_L : Fbz_R;
+  const T dens = w_le_SM ? dens_L : dens_R;
+  const T dens_s = w_le_SM ? dens_L_s : dens_R_s;
+  const T dens_ss = w_le_SM ? dens_L_ss : dens_R_ss;
+  const T etot = w_le_SM ? etot_L : etot_R;
+  const T etot_s = w_le_SM ? etot_L_s : etot_R_s;
+  const T etot_ss = w_le_SM ? etot_L_ss : etot_R_ss;
+  const T momx = w_le_SM ? momx_L : momx_R;
+  const T momx_s = w_le_SM ? dens_L_s * velx_L_s : dens_R_s * velx_R_s;
+  const T momx_ss = w_le_SM ? dens_L_ss * velx_L_ss : dens_R_ss * velx_R_ss;
+  const T momy = w_le_SM ? momy_L : momy_R;
+  const T momy_s = w_le_SM ? dens_L_s * vely_L_s : dens_R_s * vely_R_s;
+  const T momy_ss = w_le_SM ? dens_L_ss * vely_ss : dens_R_ss * vely_ss;
+  const T momz = w_le_SM ? momz_L : momz_R;
+  const T momz_s = w_le_SM ? dens_L_s * velz_L_s : dens_R_s * velz_R_s;
+  const T momz_ss = w_le_SM ? dens_L_ss * velz_ss : dens_R_ss * velz_ss;
+  const T by = w_le_SM ? By_L : By_R;
+  const T by_s = w_le_SM ? By_L_s : By_R_s;
+  const T by_ss = By_ss;
+  const T bz = w_le_SM ? Bz_L : Bz_R;
+  const T bz_s = w_le_SM ? Bz_L_s : Bz_R_s;
+  const T bz_ss = Bz_ss;
+
+  flux[MASS] = fdens + a * dens_ss + b * dens_s + c * dens;
+  flux[ENER] = fetot + a * etot_ss + b * etot_s + c * etot;
+  flux[MOMX] = fmomx + a * momx_ss + b * momx_s + c * momx;
+  flux[MOMY] = fmomy + a * momy_ss + b * momy_s + c * momy;
+  flux[MOMZ] = fmomz + a * momz_ss + b * momz_s + c * momz;
+  flux[WBY] = fby + a * by_ss + b * by_s + c * by;
+  flux[WBZ] = fbz + a * bz_ss + b * bz_s + c * bz;
+  flux[WBX] = -w * Bx;
+}
+
+// System::computeFlux (computeFlux.cpp:6-148): rotate into the face frame,
+// boundary ghost states, Dedner face values Bn/psi, HLLD, rotate back, upwind
+// psi and entropy fluxes by the sign of the mass flux.
+template <typename T>
+__device__ __forceinline__ void compute_flux(const T gamma_gas, const T *wij, const T *Wi,
+                                             const T *Wj, const int jbnd, const T *normal,
+                                             T &psi_ij, T &Bn_ij, T *flux) {
+  const T c0 = T(0.0), c1 = T(1.0);
+  const T ex = normal[0];
+  const T ey = normal[1];
+  const T ez = normal[2];
+  const T ds = dsqrt(ex * ex + ey * ey);
+  const T ids = (ds != c0) ? c1 / ds : c0;
+  const T cosph = (ids == c0) ? c1 : ex * ids;
+  const T sinph = (ids == c0) ? c0 : ey * ids;
+  const T costh = ez;
+  const T sinth = ds;
+  const T Axx = cosph * sinth;
+  const T Axy = sinth * sinph;
+  const T Axz = costh;
+  const T Ayx = -sinph;
+  const T Ayy = cosph;
+  const T Ayz = c0;
+  const T Azx = -costh * cosph;
+  const T Azy = -costh * sinph;
+  const T Azz = sinth;
+  T dens_L = Wi[DENS];
+  T ethm_L = Wi[ETHM];
+  T velx_L = Axx * Wi[VELX] + Axy * Wi[VELY] + Axz * Wi[VELZ];
+  T vely_L = Ayx * Wi[VELX] + Ayy * Wi[VELY] + Ayz * Wi[VELZ];
+  T velz_L = Azx * Wi[VELX] + Azy * Wi[VELY] + Azz * Wi[VELZ];
+  T Bx_L = Axx * Wi[BX] + Axy * Wi[BY] + Axz * Wi[BZ];
+  T By_L = Ayx * Wi[BX] + Ayy * Wi[BY] + Ayz * Wi[BZ];
+  T Bz_L = Azx * Wi[BX] + Azy * Wi[BY] + Azz * Wi[BZ];
+  T psiL = Wi[PSI];
+  T entrL = Wi[ENTR];
+  T dens_R = Wj[DENS];
+  T ethm_R = Wj[ETHM];
+  T velx_R = Axx * Wj[VELX] + Axy * Wj[VELY] + Axz * Wj[VELZ];
+  T vely_R = Ayx * Wj[VELX] + Ayy * Wj[VELY] + Ayz * Wj[VELZ];
+  T velz_R = Azx * Wj[VELX] + Azy * Wj[VELY] + Azz * Wj[VELZ];
+  T Bx_R = Axx * Wj[BX] + Axy * Wj[BY] + Axz * Wj[BZ];
+  T By_R = Ayx * Wj[BX] + Ayy * Wj[BY] + Ayz * Wj[BZ];
+  T Bz_R = Azx * Wj[BX] + Azy * Wj[BY] + Azz * Wj[BZ];
+  T psiR = Wj[PSI];
+  T entrR = Wj[ENTR];
+
+  T pres_L = pressure(gamma_gas, Wi[ETHM]);
+  T pres_R = pressure(gamma_gas, Wj[ETHM]);
+
+  const T wn_ij = Axx * wij[0] + Axy * wij[1] + Axz * wij[2];
+  switch (jbnd) {
+    case REFLECTING:
+      velx_R = -velx_R;
+      break;
+    case DIOD:
+      velx_R = smax(velx_L - c1 * wn_ij, c0) + c1 * wn_ij;
+      vely_R = vely_L;
+      velz_R = velz_L;
+      dens_R = dens_L;
+      pres_R = pres_L;
+      Bx_R = Bx_L;
+      By_R = By_L;
+      Bz_R = Bz_L;
+      psiR = psiL;
+      entrR = entrL;
+      break;
+    case OUTFLOW:
+      dens_R = dens_L;
+      pres_R = pres_L;
+      Bx_R = Bx_L;
+      By_R = By_L;
+      Bz_R = Bz_L;
+      psiR = psiL;
+      entrR = entrL;
+      break;
+    default:
+      break;
+  }
+
+  const T cfl2 = (gamma_gas * pres_L + sqr(Bx_L) + sqr(By_L) + sqr(Bz_L)) / dens_L;
+  const T cfr2 = (gamma_gas * pres_R + sqr(Bx_R) + sqr(By_R) + sqr(Bz_R)) / dens_R;
+  const T chL = dsqrt(cfl2 + c0 * sqr(velx_L - wn_ij));
+  const T chR = dsqrt(cfr2 + c0 * sqr(velx_R - wn_ij));
+
+  Bn_ij = (Bx_L * chL + Bx_R * chR + (psiL - psiR)) / (chL + chR);
+  psi_ij = (psiL * chR + psiR * chL + chR * chL * (Bx_L - Bx_R)) / (chL + chR);
+
+  T F[NF];
+  riemann_hlld<T>(gamma_gas, F, Bn_ij, wn_ij, dens_L, pres_L, ethm_L, velx_L, vely_L, velz_L, By_L,
+                  Bz_L, dens_R, pres_R, ethm_R, velx_R, vely_R, velz_R, By_R, Bz_R);
+
+  const T iAxx = cosph * sinth;
+  const T iAxy = -sinph;
+  const T iAxz = -costh * cosph;
+  const T iAyx = sinth * sinph;
+  const T iAyy = cosph;
+  const T iAyz = -costh * sinph;
+  const T iAzx = costh;
+  const T iAzy = c0;
+  const T iAzz = sinth;
+  flux[MASS] = F[MASS];
+  flux[ENER] = F[ENER];
+  flux[MOMX] = iAxx * F[MOMX] + iAxy * F[MOMY] + iAxz * F[MOMZ];
+  flux[MOMY] = iAyx * F[MOMX] + iAyy * F[MOMY] + iAyz * F[MOMZ];
+  flux[MOMZ] = iAzx * F[MOMX] + iAzy * F[MOMY] + iAzz * F[MOMZ];
+  flux[WBX] = iAxx * F[WBX] + iAxy * F[WBY] + iAxz * F[WBZ];
+  flux[WBY] = iAyx * F[WBX] + iAyy * F[WBY] + iAyz * F[WBZ];
+  flux[WBZ] = iAzx * F[WBX] + iAzy * F[WBY] + iAzz * F[WBZ];
+  flux[MPSI] = F[MASS] * (F[MASS] > c0 ? psiL : psiR);
+  flux[MENTR] = F[MASS] * (F[MASS] > c0 ? entrL : entrR);
+}
+
+}  // namespace fvm

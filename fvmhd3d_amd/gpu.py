@@ -1,0 +1,222 @@
+"""ctypes binding of the C ABI in include/fvm_gpu.h (libfvm_gpu.so).
+
+The binding fails loudly when the HIP library has not been built; there is no
+CPU fallback anywhere on this path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfvm_gpu.so")
+
+FVM_OK, FVM_EARG, FVM_EHIP, FVM_EPOS, FVM_ESTATE = 0, -1, -2, -3, -4
+LIMITER_SHIPPED, LIMITER_MINMOD, LIMITER_SMOOTH = 0, 1, 2
+STORAGE_F64, STORAGE_SHIPPED = 0, 1
+NO_BOUNDARY, REFLECTING, OUTFLOW, DIOD, INFLOW = 0, 1, 2, 3, 4
+
+# per-cell fields: name -> (id, width, kind) ; kind: 'f' double, 's' float-as-shipped, 'i' int
+FIELDS = {
+    "w": (0, 10, "f"), "x": (1, 10, "s"), "y": (2, 10, "s"), "z": (3, 10, "s"), "t": (4, 10, "s"),
+    "pos": (5, 3, "f"), "vel": (6, 3, "s"), "J": (7, 3, "s"), "tend": (8, 1, "f"), "bnd": (9, 1, "i"),
+    "is_ngb": (10, 1, "i"), "is_active": (11, 1, "i"), "mm_lo": (12, 10, "s"), "mm_hi": (13, 10, "s"),
+    "lim": (14, 10, "s"), "dU": (15, 10, "s"), "dgp": (16, 3, "s"), "ddivB": (17, 1, "s"),
+    "U": (18, 10, "f"), "mvol": (19, 1, "f"), "tbeg": (20, 1, "f"), "mvel": (21, 3, "f"),
+    "acc0": (22, 3, "f"), "acc1": (23, 3, "f"), "dt_new": (24, 1, "f"), "divB": (25, 1, "f"),
+}
+NDIAG = 8
+DIAG_NAMES = ["positivity", "closure", "ethm_failed", "mass_failed", "dsl"]
+
+
+class FvmParams(ctypes.Structure):
+    _fields_ = [("gamma_gas", ctypes.c_double), ("courant_no", ctypes.c_double),
+                ("domain_size", ctypes.c_double * 3), ("limiter_mode", ctypes.c_int),
+                ("storage", ctypes.c_int)]
+
+
+class FvmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fvm status {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+# exported entry points and their argument types (also used by the symbol test)
+_P, _I, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+SIGNATURES = {
+    "fvm_create": [ctypes.POINTER(ctypes.c_void_p), _I, ctypes.POINTER(FvmParams)],
+    "fvm_destroy": [_P],
+    "fvm_set_params": [_P, ctypes.POINTER(FvmParams)],
+    "fvm_set_time": [_P, _D],
+    "fvm_upload_mesh": [_P, _I, _I, _I, _I, _P, _P, _P, _P],
+    "fvm_upload_field": [_P, _I, _P, _I],
+    "fvm_download_field": [_P, _I, _P, _I],
+    "fvm_get_active_faces": [_P, ctypes.POINTER(ctypes.c_int)],
+    "fvm_set_active_faces": [_P, _I, _P],
+    "fvm_download_active_faces": [_P, _P, _I],
+    "fvm_compute_reconstruction": [_P],
+    "fvm_slope_limiter": [_P],
+    "fvm_compute_fluid_update": [_P],
+    "fvm_compute_primitives": [_P, _I],
+    "fvm_compute_timestep": [_P, ctypes.POINTER(ctypes.c_double)],
+    "fvm_compute_timestep_limiter": [_P],
+    "fvm_compute_predictor": [_P, _D, _I],
+    "fvm_compute_flux": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P],
+    "fvm_get_diagnostics": [_P, ctypes.POINTER(ctypes.c_longlong), _I],
+    "fvm_reset_diagnostics": [_P],
+    "fvm_synchronize": [_P],
+}
+
+
+def lib():
+    """Load libfvm_gpu.so (raises if it is missing)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: the HIP library must be built "
+                               "(__graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        L.fvm_last_error.argtypes = [ctypes.c_void_p]
+        L.fvm_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None and a.size else None
+
+
+class Context:
+    """One fvm_ctx: the device-side counterpart of a System chare's arrays."""
+
+    def __init__(self, gamma=5.0 / 3.0, courant=0.4, box=(1.0, 1.0, 1.0), limiter=LIMITER_SHIPPED,
+                 storage=STORAGE_F64, device=0):
+        self.L = lib()
+        self.params = FvmParams(gamma, courant, (ctypes.c_double * 3)(*box), limiter, storage)
+        h = ctypes.c_void_p()
+        rc = self.L.fvm_create(ctypes.byref(h), device, ctypes.byref(self.params))
+        if rc != FVM_OK:
+            raise FvmError(rc, "fvm_create failed")
+        self.h = h
+        self.np = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fvm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != FVM_OK:
+            raise FvmError(rc, self.L.fvm_last_error(self.h).decode())
+        return rc
+
+    def set_time(self, t):
+        self._chk(self.L.fvm_set_time(self.h, float(t)))
+
+    def upload_mesh(self, np_, nactive, nimport_loc, faces, cell_off, cell_face, cvol):
+        faces = np.ascontiguousarray(faces)
+        cell_off = np.ascontiguousarray(cell_off, dtype=np.int32)
+        cell_face = np.ascontiguousarray(cell_face, dtype=np.int32)
+        cvol = np.ascontiguousarray(cvol, dtype=np.float64)
+        self._chk(self.L.fvm_upload_mesh(self.h, np_, nactive, nimport_loc, len(faces), _ptr(faces),
+                                         _ptr(cell_off), _ptr(cell_face), _ptr(cvol)))
+        self.np = np_
+
+    def upload(self, name, arr):
+        fid, width, kind = FIELDS[name]
+        dt = np.int32 if kind == "i" else np.float64
+        a = np.ascontiguousarray(arr, dtype=dt).reshape(self.np, -1) if width > 1 else \
+            np.ascontiguousarray(arr, dtype=dt).reshape(self.np)
+        self._chk(self.L.fvm_upload_field(self.h, fid, _ptr(a), self.np))
+
+    def download(self, name):
+        fid, width, kind = FIELDS[name]
+        dt = np.int32 if kind == "i" else np.float64
+        a = np.empty((self.np, width) if width > 1 else (self.np,), dtype=dt)
+        self._chk(self.L.fvm_download_field(self.h, fid, _ptr(a), self.np))
+        return a
+
+    def get_active_faces(self):
+        n = ctypes.c_int()
+        self._chk(self.L.fvm_get_active_faces(self.h, ctypes.byref(n)))
+        out = np.empty(n.value, dtype=np.int32)
+        self._chk(self.L.fvm_download_active_faces(self.h, _ptr(out), n.value))
+        return out
+
+    def set_active_faces(self, fa):
+        fa = np.ascontiguousarray(fa, dtype=np.int32)
+        self._chk(self.L.fvm_set_active_faces(self.h, len(fa), _ptr(fa)))
+
+    def compute_reconstruction(self):
+        self._chk(self.L.fvm_compute_reconstruction(self.h))
+
+    def slope_limiter(self):
+        self._chk(self.L.fvm_slope_limiter(self.h))
+
+    def compute_fluid_update(self, raise_on_positivity=True):
+        rc = self.L.fvm_compute_fluid_update(self.h)
+        if rc == FVM_EPOS and not raise_on_positivity:
+            return rc
+        return self._chk(rc)
+
+    def compute_primitives(self, predictor_flag=False):
+        self._chk(self.L.fvm_compute_primitives(self.h, int(bool(predictor_flag))))
+
+    def compute_timestep(self):
+        d = ctypes.c_double()
+        self._chk(self.L.fvm_compute_timestep(self.h, ctypes.byref(d)))
+        return d.value
+
+    def compute_timestep_limiter(self):
+        self._chk(self.L.fvm_compute_timestep_limiter(self.h))
+
+    def compute_predictor(self, dtmax, min_rung=0):
+        self._chk(self.L.fvm_compute_predictor(self.h, float(dtmax), int(min_rung)))
+
+    def compute_flux(self, wij, Wi, Wj, jbnd, normal):
+        n = len(Wi)
+        wij = np.ascontiguousarray(wij, dtype=np.float64)
+        Wi = np.ascontiguousarray(Wi, dtype=np.float64)
+        Wj = np.ascontiguousarray(Wj, dtype=np.float64)
+        jbnd = np.ascontiguousarray(jbnd, dtype=np.int32)
+        normal = np.ascontiguousarray(normal, dtype=np.float64)
+        psi = np.empty(n)
+        Bn = np.empty(n)
+        flux = np.empty((n, 10))
+        self._chk(self.L.fvm_compute_flux(self.h, n, _ptr(wij), _ptr(Wi), _ptr(Wj), _ptr(jbnd), _ptr(normal),
+                                          _ptr(psi), _ptr(Bn), _ptr(flux)))
+        return psi, Bn, flux
+
+    def diagnostics(self):
+        out = (ctypes.c_longlong * NDIAG)()
+        self._chk(self.L.fvm_get_diagnostics(self.h, out, NDIAG))
+        return {k: int(out[i]) for i, k in enumerate(DIAG_NAMES)}
+
+    def reset_diagnostics(self):
+        self._chk(self.L.fvm_reset_diagnostics(self.h))
+
+    # convenience: whole state in / out
+    def upload_state(self, st):
+        self.upload_mesh(st.np, st.nactive, st.nimport_loc, st.faces, st.cell_off, st.cell_face, st.cvol)
+        for name in FIELDS:
+            self.upload(name, getattr(st, name))
+        self.set_time(st.t_global)
+
+    def download_state(self, st, names=None):
+        for name in (names or FIELDS):
+            setattr(st, name, self.download(name).reshape(getattr(st, name).shape))
+        return st

@@ -1,0 +1,169 @@
+/*
+ * fvm_gpu.h -- C ABI of the MI355X Voronoi-face pipeline of fvmhd3d.
+ *
+ * The accelerated path is the reference's per-step fluid pipeline, five
+ * non-virtual members of the Charm++ chare System (fvmhd3d.h:514-559) plus
+ * System::computeFlux.  Each has one entry point here with the same meaning;
+ * the System member arrays they read and write (fvmhd3d.h:343-406: the ptcl_act
+ * pointer tables, cell_list, face_list, face_active_list) are uploaded as
+ * plain arrays in the reference's index order:
+ *
+ *   [0, nactive)                          active cells (cell_list exists for these)
+ *   [nactive, nactive + nimport_loc)      local inactive neighbours
+ *   [nactive + nimport_loc, np)           remote ghosts (periodic images included)
+ *
+ * (localMesh.cpp:216-322).  Host arrays are arrays of the reference's own
+ * records: Fluid = double[10], vec3 = double[3], Face = fvm_face (56 B).
+ * Float-stored fields of the shipped build (Fluid_flt, fvec3) are passed as
+ * doubles; the context's storage mode decides whether the device rounds them to
+ * float at the reference's rounding points.
+ *
+ * Rules: every call returns an int status (FVM_OK or a negative code, detail in
+ * fvm_last_error); no exceptions cross the ABI; host buffers are caller-owned
+ * and device buffers library-owned; calls on one context are stream-ordered and
+ * a context is used by one thread at a time (one context per chare/thread).
+ */
+#ifndef FVM_GPU_H
+#define FVM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fvm_ctx fvm_ctx;
+
+/* status codes */
+#define FVM_OK 0
+#define FVM_EARG (-1)    /* bad argument / size / missing upload */
+#define FVM_EHIP (-2)    /* HIP runtime error */
+#define FVM_EPOS (-3)    /* non-positive density or thermal energy at a face (the reference asserts, computeFluidUpdate.cpp:179-189) */
+#define FVM_ESTATE (-4)  /* call out of order (e.g. no mesh uploaded) */
+
+/* limiter modes: Phi of computeReconstruction.cpp:5-17 / slopeLimiter.cpp:5-18 */
+#define FVM_LIMITER_SHIPPED 0 /* Phi = 1, the shipped code */
+#define FVM_LIMITER_MINMOD 1  /* Phi = min(1, x), the dead branch of the same function */
+#define FVM_LIMITER_SMOOTH 2  /* Phi of MPI/compute_reconstruction.cpp:5-15 */
+
+/* storage of the reference's float fields (Fluid_flt, fvec3, FluidD; Fluid.h:140-206) */
+#define FVM_STORAGE_F64 0     /* all-FP64: every field double */
+#define FVM_STORAGE_SHIPPED 1 /* float, as the reference ships */
+
+/* boundary kinds, MeshPoint.h:10 */
+#define FVM_NO_BOUNDARY 0
+#define FVM_REFLECTING 1
+#define FVM_OUTFLOW 2
+#define FVM_DIOD 3
+#define FVM_INFLOW 4
+
+typedef struct {
+  double gamma_gas;      /* System::gamma_gas */
+  double courant_no;     /* System::courant_no */
+  double domain_size[3]; /* global_domain_size (periodic minimum image in the reconstruction) */
+  int limiter_mode;      /* FVM_LIMITER_* */
+  int storage;           /* FVM_STORAGE_* */
+} fvm_params;
+
+/* fvmhd3d::Face (fvmhd3d.h:262-284): n is the area vector oriented s1 -> s2 */
+typedef struct {
+  int s1, s2;
+  double centroid[3];
+  double n[3];
+} fvm_face;
+
+/* per-cell fields, host layout [np][width] */
+enum fvm_field {
+  FVM_FIELD_W = 0,          /* double[10] Fluid_rec::w */
+  FVM_FIELD_GRAD_X = 1,     /* double[10] Fluid_rec::x */
+  FVM_FIELD_GRAD_Y = 2,     /* double[10] Fluid_rec::y */
+  FVM_FIELD_GRAD_Z = 3,     /* double[10] Fluid_rec::z */
+  FVM_FIELD_DWDT = 4,       /* double[10] Fluid_rec::t */
+  FVM_FIELD_POS = 5,        /* double[3]  Fluid_rec::pos */
+  FVM_FIELD_VEL = 6,        /* double[3]  Fluid_rec::vel (face-velocity input) */
+  FVM_FIELD_J = 7,          /* double[3]  Fluid_rec::J */
+  FVM_FIELD_TEND = 8,       /* double     Fluid_rec::tend */
+  FVM_FIELD_BND = 9,        /* int        Fluid_rec::bnd == MeshPoint::boundary */
+  FVM_FIELD_IS_NGB = 10,    /* int        Particle::is_ngb() */
+  FVM_FIELD_IS_ACTIVE = 11, /* int        Particle::is_active() */
+  FVM_FIELD_MINMAX_LO = 12, /* double[10] Wrec_minmax.first */
+  FVM_FIELD_MINMAX_HI = 13, /* double[10] Wrec_minmax.second */
+  FVM_FIELD_LIMITER = 14,   /* double[10] slopeLimiter_act */
+  FVM_FIELD_DU = 15,        /* double[10] FluidD::U */
+  FVM_FIELD_DGRADPSI = 16,  /* double[3]  FluidD::gradPsi */
+  FVM_FIELD_DDIVB = 17,     /* double     FluidD::divB */
+  FVM_FIELD_U = 18,         /* double[10] U_act (conserved) */
+  FVM_FIELD_MESH_VOLUME = 19, /* double   MeshPoint::Volume */
+  FVM_FIELD_TBEG = 20,      /* double     MeshPoint::tbeg */
+  FVM_FIELD_MESH_VEL = 21,  /* double[3]  MeshPoint::vel */
+  FVM_FIELD_ACC0 = 22,      /* double[3]  MeshPoint::acc0 */
+  FVM_FIELD_ACC1 = 23,      /* double[3]  MeshPoint::acc1 */
+  FVM_FIELD_DT_NEW = 24,    /* double     MeshPoint::dt_new */
+  FVM_FIELD_DIVB = 25,      /* double     divBi (output of computePrimitives) */
+  FVM_NFIELDS = 26
+};
+
+/* diagnostics: how often a reference assert would have fired */
+enum fvm_diag {
+  FVM_DIAG_POSITIVITY = 0, /* computeFluidUpdate.cpp:179-189 */
+  FVM_DIAG_CLOSURE = 1,    /* slopeLimiter.cpp:218-237 */
+  FVM_DIAG_ETHM_FAILED = 2, /* entropy fallback, computeFluidUpdate.cpp:462-466 */
+  FVM_DIAG_MASS_FAILED = 3, /* failed-mass fallback, computeFluidUpdate.cpp:377-386 */
+  FVM_DIAG_DSL = 4,        /* dsl > 0, slopeLimiter.cpp:81, computeFluidUpdate.cpp:143-144 */
+  FVM_NDIAG = 8
+};
+
+/* ---- context (replaces the System chare's member storage) ---- */
+int fvm_create(fvm_ctx **ctx, int device, const fvm_params *params);
+int fvm_destroy(fvm_ctx *ctx);
+const char *fvm_last_error(const fvm_ctx *ctx);
+int fvm_set_params(fvm_ctx *ctx, const fvm_params *params);
+/* System::t_global */
+int fvm_set_time(fvm_ctx *ctx, double t_global);
+
+/* ---- mesh of this step (System::localMesh_build output, localMesh.cpp:333-614) ----
+ * faces: face_list; cell_face_off/cell_face: cell_list[i].ngb for i < nactive
+ * (CSR, nactive+1 offsets); cell_volume: cell_list[i].Volume. */
+int fvm_upload_mesh(fvm_ctx *ctx, int np, int nactive, int nimport_loc, int nface,
+                    const fvm_face *faces, const int *cell_face_off, const int *cell_face,
+                    const double *cell_volume);
+
+/* ---- per-cell state (the pointer tables Wrec_act, mesh_act, U_act, dU_act, ...) ---- */
+int fvm_upload_field(fvm_ctx *ctx, int field, const void *host, int np);
+int fvm_download_field(fvm_ctx *ctx, int field, void *host, int np);
+
+/* ---- the path (System members, fvmhd3d.h:514-559) ---- */
+/* System::get_active_faces (Iterate.cpp:298-324); returns the count in *nface_active */
+int fvm_get_active_faces(fvm_ctx *ctx, int *nface_active);
+/* face_active_list given by the caller (face indices), replacing fvm_get_active_faces */
+int fvm_set_active_faces(fvm_ctx *ctx, int nface_active, const int *face_active);
+int fvm_download_active_faces(fvm_ctx *ctx, int *face_active, int nface_active);
+/* System::computeReconstruction (computeReconstruction.cpp:42-264) */
+int fvm_compute_reconstruction(fvm_ctx *ctx);
+/* System::slopeLimiter compute part (slopeLimiter.cpp:27-237) followed by
+ * System::computeFluidUpdateI (computeFluidUpdate.cpp:25-75) */
+int fvm_slope_limiter(fvm_ctx *ctx);
+/* System::computeFluidUpdate (computeFluidUpdate.cpp:88-247); returns FVM_EPOS
+ * when a face state is not positive (the results are still written) */
+int fvm_compute_fluid_update(fvm_ctx *ctx);
+/* System::computePrimitives (computeFluidUpdate.cpp:347-542) */
+int fvm_compute_primitives(fvm_ctx *ctx, int predictor_flag);
+/* System::computeTimestep (computeTimestep.cpp:6-62); *dt_min = min over active cells */
+int fvm_compute_timestep(fvm_ctx *ctx, double *dt_min);
+/* System::computeTimestepLimiter (computeTimestep.cpp:64-110) */
+int fvm_compute_timestep_limiter(fvm_ctx *ctx);
+/* System::computePredictor (computePredictor.cpp:11-391); dtmax/min_rung are the
+ * Scheduler's (Scheduler.h:15-25, 94-111) for the dt/128 probe */
+int fvm_compute_predictor(fvm_ctx *ctx, double dtmax, int min_rung);
+/* System::computeFlux for n independent faces (computeFlux.cpp:6-148), host
+ * arrays: wij[n][3], Wi/Wj[n][10], jbnd[n], normal[n][3] -> psi[n], Bn[n], flux[n][10] */
+int fvm_compute_flux(fvm_ctx *ctx, int n, const double *wij, const double *Wi, const double *Wj,
+                     const int *jbnd, const double *normal, double *psi, double *Bn, double *flux);
+
+/* diagnostics counters accumulated since the last reset */
+int fvm_get_diagnostics(fvm_ctx *ctx, long long *out, int n);
+int fvm_reset_diagnostics(fvm_ctx *ctx);
+int fvm_synchronize(fvm_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FVM_GPU_H */
