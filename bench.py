@@ -1,0 +1,185 @@
+"""Benchmark: Voronoi face-flux updates/s on one MI355X.
+
+Workload (default): the MHD blast on 128^3 = 2,097,152 generators drawn
+uniform-random (Rand48, seed 123) in the periodic unit cube, rho = 1,
+p = 100 inside r < 0.1 and 0.1 outside, B = (10/sqrt2)(1,1,0), v = 0,
+gamma = 5/3, Courant 0.4, static mesh.  One step = one pass of the path over
+every face: CFL + predictor + reconstruction, limiter, flux + FP64 apply,
+on the FP32 face path (FP64 flux sums, pressure recovery and CFL).
+
+Prints ONE JSON line.  Everything else goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "Voronoi face-flux updates/s (1 GPU) at ≤1e-4 rel L1 vs FP64; % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (generator, n, ic, limiter, steps in the configuration)
+    "mhd_blast_128^3": ("random", 128, "blast", 2, 20),
+    "orszag_tang_256^3": ("lattice", 256, "ot", 2, 100),
+    "alfven_32^3": ("lattice", 32, "alfven", 0, 1),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(name):
+    from fvmhd3d_amd import mesh as M
+    from fvmhd3d_amd.state import global_step_state
+    gen, n, ic, lim, _ = WORKLOADS[name]
+    t = time.perf_counter()
+    pos = M.uniform_random(n ** 3, seed=123) if gen == "random" else M.jittered_lattice(n, seed=123)
+    pos = pos[M.morton_order(pos, (1.0, 1.0, 1.0))]
+    m = M.build_periodic(pos)
+    W = {"blast": M.ic_blast, "ot": M.ic_orszag_tang, "alfven": M.ic_alfven}[ic](m.pos)
+    st = global_step_state(m, W, dt=0.0)
+    log(f"[bench] mesh {name}: {m.ncell} cells, {m.nface} faces, built in {time.perf_counter() - t:.1f} s")
+    return m, st, lim
+
+
+def k2_bytes(ncell, nfe, prec_bytes):
+    """Algorithmic HBM bytes of one flux+apply launch: every array it must touch, once.
+    per cell: read gathered W (10), gradients (30), Wt (10) in face precision; FP64
+    W, U (10+10), volume, CSR offset; write FP64 W, U and the face-precision W copy.
+    per face-end: neighbour index + area vector (3) + centroid offset (3)."""
+    p = prec_bytes
+    per_cell = 10 * p + 30 * p + 10 * p + 80 + 80 + 8 + 4 + 80 + 80 + 10 * p
+    per_fe = 4 + 3 * p + 3 * p
+    return ncell * per_cell + nfe * per_fe
+
+
+def k1_bytes(ncell, nfe, prec_bytes):
+    """CFL + predictor + reconstruction launch: per cell read FP64 U, volume, offset,
+    gathered W; write gradients, Wt, dt_new; per face-end the same geometry stream."""
+    p = prec_bytes
+    per_cell = 80 + 8 + 4 + 10 * p + 30 * p + 10 * p + 8
+    per_fe = 4 + 3 * p + 3 * p
+    return ncell * per_cell + nfe * per_fe
+
+
+def cpu_baseline(st, lim, budget_s=30.0):
+    """The oracle (CPU restatement, one core) on one global step of the same mesh."""
+    from oracle.oracle import Oracle
+    o = Oracle("f64", limiter=lim)
+    s = st.copy()
+    o.get_active_faces(s)
+    t = time.perf_counter()
+    o.step_global(s, 1.0, 1)
+    dt = time.perf_counter() - t
+    return {"value": len(s.face_act) / dt, "unit": "face-flux updates/s", "cores": 1, "kind": "port",
+            "sample": f"1 global step of the same mesh ({st.np} cells, {len(s.face_act)} faces), "
+                      f"oracle/fvm_oracle.c FP64, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="mhd_blast_128^3", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-verify", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md, not run", file=sys.stderr)
+        sys.exit(2)
+
+    from fvmhd3d_amd import gpu as G
+    m, st, lim = build_workload(args.workload)
+    nfe = int(m.cell_off[-1])
+
+    def fresh(precision):
+        ctx = G.Context(gamma=st.gamma, courant=st.courant, box=st.box, limiter=lim)
+        ctx.upload_state(st)
+        ctx.fused_setup(precision)
+        return ctx
+
+    ctx = fresh(G.FUSED_FP32)
+    ctx.fused_run(args.warmup, 1.0)
+    ctx.reset_diagnostics()
+    ctx.fused_timing(True)
+    ctx.L.fvm_synchronize(ctx.h)
+    t0 = time.perf_counter()
+    ctx.fused_run(args.steps, 1.0)
+    ctx.L.fvm_synchronize(ctx.h)
+    t1 = time.perf_counter()
+    ms, nl = ctx.fused_kernel_ms()
+    diag = ctx.diagnostics()
+    W32 = ctx.download("w")
+    U32 = ctx.download("U")
+    dts = ctx.fused_dt_history(args.steps)
+    ctx.close()
+    elapsed = t1 - t0
+    value = m.nface * args.steps / elapsed
+    log(f"[bench] {args.steps} steps in {elapsed * 1e3:.1f} ms; kernel ms/launch: "
+        f"recon {ms[0] / max(nl[0], 1):.3f} limit {ms[1] / max(nl[1], 1):.3f} "
+        f"flux {ms[2] / max(nl[2], 1):.3f} advance {ms[3] / max(nl[3], 1):.4f}; dt {dts[0]:.3e}; diag {diag}")
+
+    k2_ms = ms[2] / max(nl[2], 1)
+    k2_b = k2_bytes(m.ncell, nfe, 4)
+    achieved = k2_b / (k2_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_flux_apply_{args.workload.replace('^', '')}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "k_flux_apply<float> (flux + FP64 apply)", "bytes_per_launch": k2_b,
+                "ms_per_launch": round(k2_ms, 4)}
+    k1_ms = ms[0] / max(nl[0], 1)
+    k1_b = k1_bytes(m.ncell, nfe, 4)
+
+    extra = {"kernels": {
+        "k_cfl_predict_recon": {"ms": round(k1_ms, 4), "GB/s": round(k1_b / (k1_ms * 1e-3) / 1e9, 1)},
+        "k_limit": {"ms": round(ms[1] / max(nl[1], 1), 4)},
+        "k_flux_apply": {"ms": round(k2_ms, 4), "GB/s": round(achieved, 1)}},
+        "step_GB_alg": round((k1_b + k2_b) / 1e9, 3)}
+
+    if not args.no_verify:
+        c64 = fresh(G.FUSED_FP64)
+        c64.fused_run(args.warmup + args.steps, 1.0)
+        W64 = c64.download("w")
+        U64 = c64.download("U")
+        c64.close()
+        l1 = lambda a, b: float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
+        extra["l1_rel_vs_fp64"] = {"rho": l1(W32[:, 0], W64[:, 0]), "E": l1(U32[:, 4], U64[:, 4]),
+                                   "ethm": l1(W32[:, 4], W64[:, 4]), "steps": args.warmup + args.steps}
+        log(f"[bench] FP32 vs FP64 after {args.warmup + args.steps} steps: {extra['l1_rel_vs_fp64']}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(st, lim)
+        log(f"[bench] cpu baseline: {cpu}")
+
+    gen, n, ic, _, cfg_steps = WORKLOADS[args.workload]
+    out = {
+        "metric": METRIC, "value": value, "unit": "face-flux updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 face arithmetic, fp64 sums/pressure/CFL",
+        "data": "synthetic (Rand48-seeded generators, analytic initial state)",
+        "config": {"workload": args.workload, "cells": m.ncell, "faces": m.nface,
+                   "generators": gen, "limiter": ["shipped", "minmod", "smooth"][lim], "mesh": "static"},
+        "roofline": roofline, "cpu_baseline": cpu, **extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

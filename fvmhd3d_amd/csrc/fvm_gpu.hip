@@ -840,40 +840,14 @@ __global__ void k_end_sort(const int *soff, int *sent, int n) {
 // ===========================================================================
 // context
 // ===========================================================================
-struct DBuf {
-  void *p = nullptr;
-  size_t bytes = 0;
-};
-
-struct fvm_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  fvm_params prm{};
-  std::string err;
-  double t_global = 0.0;
-  int np = 0, nactive = 0, nimport_loc = 0, nface = 0, nref = 0, nfa = -1;
-  bool have_mesh = false;
-  DBuf field[FVM_NFIELDS];
-  DBuf m_s1, m_s2, m_c, m_n, m_coff, m_cface, m_cvol, m_fact, m_soff, m_sent;
-  DBuf rec, swp, tmp0, tmp1, tmp2, stage;
-  DBuf diag;      // unsigned long long[FVM_NDIAG]
-  DBuf scalars;   // unsigned long long[8]
-  fvm::ScanScratch scan;
-};
-
-static int fail(fvm_ctx *c, int code, const std::string &msg) {
+int fvm_fail(fvm_ctx *c, int code, const std::string &msg) {
   if (c) c->err = msg;
   return code;
 }
+static int fail(fvm_ctx *c, int code, const std::string &msg) { return fvm_fail(c, code, msg); }
+#define HIPCHK(c, call) FVM_HIPCHK(c, call)
 
-#define HIPCHK(c, call)                                                                 \
-  do {                                                                                  \
-    hipError_t e_ = (call);                                                             \
-    if (e_ != hipSuccess)                                                               \
-      return fail((c), FVM_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
-  } while (0)
-
-static int ensure(fvm_ctx *c, DBuf &b, size_t bytes) {
+int fvm_ensure(fvm_ctx *c, DBuf &b, size_t bytes) {
   if (b.bytes >= bytes && b.p) return FVM_OK;
   if (b.p) HIPCHK(c, hipFree(b.p));
   b.p = nullptr;
@@ -883,11 +857,13 @@ static int ensure(fvm_ctx *c, DBuf &b, size_t bytes) {
   b.bytes = bytes;
   return FVM_OK;
 }
-static void release(DBuf &b) {
+static int ensure(fvm_ctx *c, DBuf &b, size_t bytes) { return fvm_ensure(c, b, bytes); }
+void fvm_release(DBuf &b) {
   if (b.p) (void)hipFree(b.p);
   b.p = nullptr;
   b.bytes = 0;
 }
+static void release(DBuf &b) { fvm_release(b); }
 
 static size_t store_size(const fvm_ctx *c) { return c->prm.storage == FVM_STORAGE_SHIPPED ? sizeof(float) : sizeof(double); }
 static size_t field_bytes(const fvm_ctx *c, int f, int np) {

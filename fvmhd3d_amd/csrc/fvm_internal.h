@@ -2,7 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-struct fvm_ctx;
+#include <string>
+#include <vector>
+
+#include "../../include/fvm_gpu.h"
 
 namespace fvm {
 
@@ -16,7 +19,7 @@ struct ScanScratch {
   size_t nblocks = 0;
 };
 
-// out[i] = sum_{k<i} in[i] over n ints; out may alias nothing in in.
+// out[i] = sum_{k<i} in[k] over n ints.
 int exclusive_scan(ScanScratch &s, const int *in, int *out, int n, hipStream_t st);
 // same for 0/1 flags with wavefront ballots; *total receives the count (host).
 int exclusive_scan_flags(ScanScratch &s, const int *flags, int *out, int n, int *total, hipStream_t st);
@@ -24,5 +27,52 @@ void scan_release(ScanScratch &s);
 
 }  // namespace fvm
 
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+// device-resident stepping state (fvm_fast.hip)
+struct FusedState {
+  bool ready = false;
+  int precision = 0;  // FVM_FUSED_FP64 / FVM_FUSED_FP32
+  int moving = 0;
+  int E = 0;
+  int cur = 0;
+  DBuf nbr, fn, drL, cen, Wg[2], grad, Wt, mm, dt_new, dtmin, tdev, hist;
+  bool timing = false;
+  std::vector<hipEvent_t> ev;  // 5 per timed step
+  double ms[4] = {0, 0, 0, 0};
+  long long launches[4] = {0, 0, 0, 0};
+};
+
+struct fvm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  fvm_params prm{};
+  std::string err;
+  double t_global = 0.0;
+  int np = 0, nactive = 0, nimport_loc = 0, nface = 0, nref = 0, nfa = -1;
+  bool have_mesh = false;
+  DBuf field[FVM_NFIELDS];
+  DBuf m_s1, m_s2, m_c, m_n, m_coff, m_cface, m_cvol, m_fact, m_soff, m_sent;
+  DBuf rec, swp, tmp0, tmp1, tmp2, stage;
+  DBuf diag;     // unsigned long long[FVM_NDIAG]
+  DBuf scalars;  // unsigned long long[8]
+  fvm::ScanScratch scan;
+  FusedState fz;
+};
+
+int fvm_fail(fvm_ctx *c, int code, const std::string &msg);
+int fvm_ensure(fvm_ctx *c, DBuf &b, size_t bytes);
+void fvm_release(DBuf &b);
+
 // release of the fused-path buffers of a context (fvm_fast.hip)
 void fvm_fast_release(fvm_ctx *c);
+
+#define FVM_HIPCHK(c, call)                                                                \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fvm_fail((c), FVM_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));   \
+  } while (0)

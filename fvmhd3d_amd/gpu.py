@@ -69,7 +69,14 @@ SIGNATURES = {
     "fvm_get_diagnostics": [_P, ctypes.POINTER(ctypes.c_longlong), _I],
     "fvm_reset_diagnostics": [_P],
     "fvm_synchronize": [_P],
+    "fvm_fused_setup": [_P, _I, _I],
+    "fvm_fused_run": [_P, _I, _D, ctypes.POINTER(ctypes.c_double)],
+    "fvm_fused_dt_history": [_P, _P, _I],
+    "fvm_fused_sync": [_P],
+    "fvm_fused_timing": [_P, _I],
+    "fvm_fused_kernel_ms": [_P, _P, _P, _I],
 }
+FUSED_FP64, FUSED_FP32 = 0, 1
 
 
 def lib():
@@ -208,6 +215,32 @@ class Context:
 
     def reset_diagnostics(self):
         self._chk(self.L.fvm_reset_diagnostics(self.h))
+
+    # device-resident stepping
+    def fused_setup(self, precision=FUSED_FP32, moving=False):
+        self._chk(self.L.fvm_fused_setup(self.h, int(precision), int(bool(moving))))
+
+    def fused_run(self, nsteps, dtmax):
+        t = ctypes.c_double()
+        self._chk(self.L.fvm_fused_run(self.h, int(nsteps), float(dtmax), ctypes.byref(t)))
+        return t.value
+
+    def fused_dt_history(self, n):
+        out = np.empty(n)
+        self._chk(self.L.fvm_fused_dt_history(self.h, _ptr(out), n))
+        return out
+
+    def fused_sync(self):
+        self._chk(self.L.fvm_fused_sync(self.h))
+
+    def fused_timing(self, enable=True):
+        self._chk(self.L.fvm_fused_timing(self.h, int(bool(enable))))
+
+    def fused_kernel_ms(self):
+        ms = np.zeros(4)
+        nl = np.zeros(4, dtype=np.int64)
+        self._chk(self.L.fvm_fused_kernel_ms(self.h, _ptr(ms), _ptr(nl), 4))
+        return ms, nl
 
     # convenience: whole state in / out
     def upload_state(self, st):

@@ -158,6 +158,27 @@ int fvm_compute_predictor(fvm_ctx *ctx, double dtmax, int min_rung);
 int fvm_compute_flux(fvm_ctx *ctx, int n, const double *wij, const double *Wi, const double *Wj,
                      const int *jbnd, const double *normal, double *psi, double *Bn, double *flux);
 
+/* ---- device-resident stepping: static mesh, every cell active, one global
+ * timestep (all cells on the smallest rung of the Scheduler, dt = rung of the
+ * minimum CFL step).  No counterpart entry point in the reference: it chains
+ * computeTimestep -> computePredictor -> computeReconstruction ->
+ * slopeLimiter -> computeFluidUpdate -> computePrimitives (Iterate.cpp:92-202)
+ * on the device with the state resident in HBM.  Uses the mesh and the fields
+ * W, U, POS, MESH_VEL uploaded before fvm_fused_setup; W and U are updated in
+ * place. */
+#define FVM_FUSED_FP64 0 /* FP64 face arithmetic: the plain FP64 path */
+#define FVM_FUSED_FP32 1 /* FP32 face reconstruction + HLLD; FP64 flux sums, pressure recovery, CFL */
+int fvm_fused_setup(fvm_ctx *ctx, int precision, int moving_mesh);
+int fvm_fused_run(fvm_ctx *ctx, int nsteps, double dtmax, double *t_out);
+/* dt of each step of the last fvm_fused_run */
+int fvm_fused_dt_history(fvm_ctx *ctx, double *dts, int n);
+/* copy the last step's gradients, Wt and dt_new into the per-cell fields */
+int fvm_fused_sync(fvm_ctx *ctx);
+/* per-kernel HIP-event timing of fvm_fused_run on the context's stream:
+ * ms[0] CFL+predictor+reconstruction, ms[1] limiter, ms[2] flux+apply, ms[3] advance */
+int fvm_fused_timing(fvm_ctx *ctx, int enable);
+int fvm_fused_kernel_ms(fvm_ctx *ctx, double *ms, long long *launches, int n);
+
 /* diagnostics counters accumulated since the last reset */
 int fvm_get_diagnostics(fvm_ctx *ctx, long long *out, int n);
 int fvm_reset_diagnostics(fvm_ctx *ctx);

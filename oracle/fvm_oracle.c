@@ -1184,3 +1184,32 @@ API void o_compute_predictor(o_state *s, double dtmax, int min_rung) {
 
 API int o_sizeof_sreal(void) { return (int)sizeof(sreal); }
 API int o_sizeof_state(void) { return (int)sizeof(o_state); }
+
+/* ------------------------------------------------------------------ */
+/* Global-timestep driver: one step of Iterate.cpp:92-202 with every    */
+/* cell on the smallest rung.  CFL (computeTimestep) -> predictor       */
+/* (probe from each cell's own dt_new) -> dt = rung of the global       */
+/* minimum dt_new (Scheduler.h:94-111, System.cpp:258-266) -> tend ->   */
+/* reconstruction -> limiter -> flux scatter -> computePrimitives at    */
+/* t + dt.  face_act must hold the active face list.  Returns dt.      */
+/* ------------------------------------------------------------------ */
+API double o_step_global(o_state *s, double dtmax) {
+  o_compute_timestep(s);
+  o_compute_predictor(s, dtmax, 0);
+  double dtmin = REF_HUGE;
+  for (int i = 0; i < s->nactive; i++) dtmin = smin(dtmin, s->dt_new[i]);
+  {
+    const double dt = o_rung_dt(dtmax, 0, dtmin);
+    const double t0 = s->t_global;
+    for (int i = 0; i < s->np; i++) {
+      s->tbeg[i] = t0;
+      s->tend[i] = t0 + dt;
+    }
+    o_compute_reconstruction(s);
+    o_slope_limiter(s);
+    o_compute_fluid_update(s);
+    s->t_global = t0 + dt;
+    o_compute_primitives(s, 0);
+    return dt;
+  }
+}

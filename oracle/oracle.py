@@ -63,6 +63,8 @@ def _lib(mode):
         L.o_get_active_faces.restype = ctypes.c_int
         L.o_compute_flux_batch.argtypes = [ctypes.POINTER(OState), ctypes.c_int] + [_P] * 8
         L.o_compute_flux_batch.restype = None
+        L.o_step_global.argtypes = [ctypes.POINTER(OState), ctypes.c_double]
+        L.o_step_global.restype = ctypes.c_double
         L.o_rung_dt.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_double]
         L.o_rung_dt.restype = ctypes.c_double
         L.o_sizeof_state.restype = ctypes.c_int
@@ -157,6 +159,16 @@ class Oracle:
 
     def compute_predictor(self, st, dtmax, min_rung=0):
         self._run(st, self.L.o_compute_predictor, float(dtmax), int(min_rung))
+
+    def step_global(self, st, dtmax, nsteps=1):
+        """nsteps of the global-timestep driver; returns the list of dt."""
+        o, keep = self._pack(st)
+        dts = []
+        for _ in range(nsteps):
+            dts.append(self.L.o_step_global(ctypes.byref(o), float(dtmax)))
+        st.t_global = o.t_global
+        self._unpack(st, o, keep)
+        return dts
 
     def rung_dt(self, dtmax, min_rung, dt):
         return self.L.o_rung_dt(dtmax, min_rung, dt)
