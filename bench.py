@@ -94,6 +94,7 @@ def main():
     ap.add_argument("--workload", default="mhd_blast_128^3", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--copy-only", action="store_true", help="only the stream-copy probe (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -101,6 +102,16 @@ def main():
         sys.exit(2)
 
     from fvmhd3d_amd import gpu as G
+    if args.copy_only:
+        c = G.Context()
+        gb = c.copy_bandwidth(1 << 30, 10)
+        c.close()
+        print(json.dumps({"stream_copy_GBps": gb, "bytes_per_launch": 2 * (1 << 30)}), flush=True)
+        return
+    probe = G.Context()
+    copy_gbps = probe.copy_bandwidth(1 << 30, 10)
+    probe.close()
+    log(f"[bench] stream copy {copy_gbps:.0f} GB/s")
     m, st, lim = build_workload(args.workload)
     nfe = int(m.cell_off[-1])
 
@@ -150,7 +161,7 @@ def main():
         "k_cfl_predict_recon": {"ms": round(k1_ms, 4), "GB/s": round(k1_b / (k1_ms * 1e-3) / 1e9, 1)},
         "k_limit": {"ms": round(ms[1] / max(nl[1], 1), 4)},
         "k_flux_apply": {"ms": round(k2_ms, 4), "GB/s": round(achieved, 1)}},
-        "step_GB_alg": round((k1_b + k2_b) / 1e9, 3)}
+        "step_GB_alg": round((k1_b + k2_b) / 1e9, 3), "stream_copy_GBps": round(copy_gbps, 1)}
 
     if not args.no_verify:
         c64 = fresh(G.FUSED_FP64)

@@ -50,22 +50,41 @@ __device__ __forceinline__ int xcd_cell(int nb) {
   return tile * FB + (int)threadIdx.x;
 }
 
+// Per-cell gather record: the state a neighbour needs, contiguous, so one
+// gather is a few 16-byte vector loads from two cache lines.
+//   [0,10) W   [10,20) Wt   [20,50) gradients x[10] y[10] z[10]   (stride 64)
+constexpr int REC = 64;
+enum { R_W = 0, R_T = 10, R_G = 20 };
+
+// Per face-end record (CSR order of the cell's face list): the neighbour
+// (j, or ~j when this cell is s2) and the face geometry in canonical
+// orientation s1 -> s2; FP64 also carries the reconstruction centroid of the
+// own side as the reference forms it.
+template <typename T> struct FaceEndRec;
+template <> struct alignas(16) FaceEndRec<float> {
+  int nbr;
+  float fn[3];
+  float drL[3];
+  float pad;
+};
+template <> struct alignas(16) FaceEndRec<double> {
+  int nbr, pad;
+  double fn[3];
+  double drL[3];
+  double cen[3];
+};
+
 template <typename T> struct FusedArgs {
   int n;                    // cells (all active)
   const int *off;           // [n+1] face-end offsets
-  const int *nbr;           // [E] neighbour j (>= 0: this cell is L = s1; ~j: this cell is R)
-  const T *fn;              // [3][E] face.n (area vector, s1 -> s2)
-  const T *drL;             // [3][E] centroid - pos[s1]
-  const T *cen;             // [3][E] wrapped centroid - pos[this cell] (FP64 only)
-  int E;
+  const FaceEndRec<T> *fe;  // [E]
   const double *cvol;       // [n]
   const double *mvel;       // [3][n] generator velocity (moving mesh only)
-  const T *Wg;              // [10][n] gathered primitive state (this step)
-  T *Wg_next;               // [10][n]
+  const T *rec;             // [n][REC] records of this step
+  T *rec_w;                 // records written by this launch (K1: rec itself; K2: next step's)
+  T *rec_next;
   double *W64;              // [10][n] FP64 primitive state (own cell only)
   double *U;                // [10][n] FP64 conserved state (own cell only)
-  T *grad;                  // [30][n] x[10], y[10], z[10]
-  T *Wt;                    // [10][n]
   T *mm;                    // [20][n] min/max (opt-in limiters)
   double *dt_new;           // [n]
   unsigned long long *dtmin_bits;
@@ -78,8 +97,41 @@ template <typename T> struct FusedArgs {
 #define SF(p, k, i, n) (p)[(size_t)(k) * (size_t)(n) + (size_t)(i)]
 #define FLD(p, k, i, n) SF(p, k, i, n)
 
-// geometry of one face-end: canonical (L->R) normal, area, drL, and this cell's
-// reconstruction centroid/normal
+// vector loads/stores of record slices (offsets are multiples of 16 bytes)
+template <int N> __device__ __forceinline__ void ldv(const float *__restrict__ p, float *o) {
+  int k = 0;
+#pragma unroll
+  for (; k + 4 <= N; k += 4) {
+    const float4 v = *reinterpret_cast<const float4 *>(p + k);
+    o[k] = v.x; o[k + 1] = v.y; o[k + 2] = v.z; o[k + 3] = v.w;
+  }
+#pragma unroll
+  for (; k < N; k += 2) {
+    const float2 v = *reinterpret_cast<const float2 *>(p + k);
+    o[k] = v.x; o[k + 1] = v.y;
+  }
+}
+template <int N> __device__ __forceinline__ void ldv(const double *__restrict__ p, double *o) {
+#pragma unroll
+  for (int k = 0; k < N; k += 2) {
+    const double2 v = *reinterpret_cast<const double2 *>(p + k);
+    o[k] = v.x; o[k + 1] = v.y;
+  }
+}
+template <int N> __device__ __forceinline__ void stv(float *__restrict__ p, const float *o) {
+  int k = 0;
+#pragma unroll
+  for (; k + 4 <= N; k += 4) *reinterpret_cast<float4 *>(p + k) = make_float4(o[k], o[k + 1], o[k + 2], o[k + 3]);
+#pragma unroll
+  for (; k < N; k += 2) *reinterpret_cast<float2 *>(p + k) = make_float2(o[k], o[k + 1]);
+}
+template <int N> __device__ __forceinline__ void stv(double *__restrict__ p, const double *o) {
+#pragma unroll
+  for (int k = 0; k < N; k += 2) *reinterpret_cast<double2 *>(p + k) = make_double2(o[k], o[k + 1]);
+}
+
+// geometry of one face-end: canonical (s1->s2) normal, area, drL, drR, and
+// this cell's reconstruction centroid and outward normal
 template <typename T> struct FE {
   int j;
   bool isL;
@@ -88,22 +140,22 @@ template <typename T> struct FE {
 
 template <typename T>
 __device__ __forceinline__ void load_fe(const FusedArgs<T> &A, int e, FE<T> &g) {
-  const int v = A.nbr[e];
-  g.isL = v >= 0;
-  g.j = g.isL ? v : ~v;
-  const T fx = SF(A.fn, 0, e, A.E), fy = SF(A.fn, 1, e, A.E), fz = SF(A.fn, 2, e, A.E);
-  g.area = dsqrt(fx * fx + fy * fy + fz * fz);
-  const T ia = T(1.0) / g.area;
+  const FaceEndRec<T> r = A.fe[e];
+  g.isL = r.nbr >= 0;
+  g.j = g.isL ? r.nbr : ~r.nbr;
+  const T fx = r.fn[0], fy = r.fn[1], fz = r.fn[2];
+  g.area = fsqrt(fx * fx + fy * fy + fz * fz);
+  const T ia = frcp(g.area);
   g.nrm[0] = fx * ia;
   g.nrm[1] = fy * ia;
   g.nrm[2] = fz * ia;
-  for (int d = 0; d < 3; d++) g.drL[d] = SF(A.drL, d, e, A.E);
+  for (int d = 0; d < 3; d++) g.drL[d] = r.drL[d];
   const T dsh = g.drL[0] * g.nrm[0] + g.drL[1] * g.nrm[1] + g.drL[2] * g.nrm[2];
   const T dsl = T(2.0) * dsh;
   for (int d = 0; d < 3; d++) g.drR[d] = g.drL[d] - g.nrm[d] * dsl;
   if constexpr (std::is_same<T, double>::value) {
     // the reference's reconstruction geometry (computeReconstruction.cpp:97-123)
-    for (int d = 0; d < 3; d++) g.cen[d] = SF(A.cen, d, e, A.E);
+    for (int d = 0; d < 3; d++) g.cen[d] = r.cen[d];
     const T cn = g.cen[0] * fx + g.cen[1] * fy + g.cen[2] * fz;
     const T sc = (cn < 0.0) ? (-1.0 / g.area) : (1.0 / g.area);
     g.ncen[0] = fx * sc;
@@ -138,9 +190,9 @@ __device__ __forceinline__ void face_velocity(const FusedArgs<T> &A, int L, int 
       rij[d] = g.nrm[d] * dsl;
     }
     const T fdv = (vj[0] - vi[0]) * fij[0] + (vj[1] - vi[1]) * fij[1] + (vj[2] - vi[2]) * fij[2];
-    const T r = T(1.0) / sqr(dsl);
+    const T r = frcp(sqr(dsl));
     for (int d = 0; d < 3; d++) wij[d] = (vj[d] + vi[d]) * T(0.5) - rij[d] * fdv * r;
-    dV = g.area * (rij[0] * wij[0] + rij[1] * wij[1] + rij[2] * wij[2]) / dsl;
+    dV = fdiv(g.area * (rij[0] * wij[0] + rij[1] * wij[1] + rij[2] * wij[2]), dsl);
   }
 }
 
@@ -170,7 +222,7 @@ __device__ __forceinline__ double entropy_from_ethm_d(double g, const double *W)
 // ---------------------------------------------------------------------------
 // K1: computeTimestep + computePredictor + computeReconstruction
 // ---------------------------------------------------------------------------
-template <typename T, bool MOVING>
+template <typename T, bool MOVING, bool LIMIT>
 __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
   const int c = xcd_cell(A.n);
   const bool live = c < A.n;
@@ -180,21 +232,28 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
     const double g64 = A.gamma;
     const T g = (T)A.gamma;
     const double V = A.cvol[c];
-    double U0[NF], W0[NF];
-    for (int k = 0; k < NF; k++) U0[k] = SF(A.U, k, c, n);
-    to_primitive(U0, V, W0);
+    const T *rc = A.rec + (size_t)c * REC;
     T Wc[NF];
-    for (int k = 0; k < NF; k++) Wc[k] = SF(A.Wg, k, c, n);
+    ldv<NF>(rc + R_W, Wc);
 
-    // CFL (computeTimestep.cpp:13-31)
-    const double h = pow(V / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
+    // CFL (computeTimestep.cpp:13-31), FP64 from U
+    double v0[3], csi, h;
+    {
+      double U0[NF], W0[NF];
+      for (int k = 0; k < NF; k++) U0[k] = SF(A.U, k, c, n);
+      to_primitive(U0, V, W0);
+      h = pow(V / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
+      const double Pi = pressure(g64, W0[ETHM]) * g64;
+      const double B2i = W0[BX] * W0[BX] + W0[BY] * W0[BY] + W0[BZ] * W0[BZ];
+      csi = dsqrt((Pi + B2i) / W0[DENS]);
+      v0[0] = W0[VELX];
+      v0[1] = W0[VELY];
+      v0[2] = W0[VELZ];
+    }
     double pv[3] = {0.0, 0.0, 0.0};
     if constexpr (MOVING)
       for (int d = 0; d < 3; d++) pv[d] = SF(A.mvel, d, c, n);
-    const double Pi = pressure(g64, W0[ETHM]) * g64;
-    const double B2i = W0[BX] * W0[BX] + W0[BY] * W0[BY] + W0[BZ] * W0[BZ];
-    const double csi = dsqrt((Pi + B2i) / W0[DENS]);
-    const double dvr[3] = {W0[VELX] - pv[0], W0[VELY] - pv[1], W0[VELZ] - pv[2]};
+    const double dvr[3] = {v0[0] - pv[0], v0[1] - pv[1], v0[2] - pv[2]};
     const double vrel = dsqrt(dvr[0] * dvr[0] + dvr[1] * dvr[1] + dvr[2] * dvr[2]);
     double dti = h / (csi + vrel + 1.0 / REF_HUGE);
     double arg_max = -1.0;  // min_j h/(cs + sqrt(a_j)) = h/(cs + sqrt(max_j a_j)), monotone roundings
@@ -210,12 +269,12 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
       load_fe(A, e, fe);
       const int j = fe.j;
       T Wj[NF];
-      for (int k = 0; k < NF; k++) Wj[k] = SF(A.Wg, k, j, n);
+      ldv<NF>(A.rec + (size_t)j * REC + R_W, Wj);
 
       // --- reconstruction sums (computeReconstruction.cpp:108-149)
       {
         const T dsh = fe.cen[0] * fe.ncen[0] + fe.cen[1] * fe.ncen[1] + fe.cen[2] * fe.ncen[2];
-        const T ids = (dsh != T(0.0)) ? T(0.5) / dsh : T(0.0);
+        const T ids = (dsh != T(0.0)) ? fdiv(T(0.5), dsh) : T(0.0);
         const T idsA = fe.area * ids;
         T drh[3], fij[3];
         for (int d = 0; d < 3; d++) {
@@ -225,8 +284,10 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
         for (int k = 0; k < NF; k++) {
           const T sum = Wj[k] + Wc[k];
           const T diff = Wj[k] - Wc[k];
-          dmn[k] = smin(dmn[k], diff);
-          dmx[k] = smax(dmx[k], diff);
+          if constexpr (LIMIT) {
+            dmn[k] = smin(dmn[k], diff);
+            dmx[k] = smax(dmx[k], diff);
+          }
           gx[k] += (sum * drh[0] + diff * fij[0]) * idsA;
           gy[k] += (sum * drh[1] + diff * fij[1]) * idsA;
           gz[k] += (sum * drh[2] + diff * fij[2]) * idsA;
@@ -237,8 +298,7 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
         double dwi[3] = {0.0, 0.0, 0.0};
         if constexpr (MOVING)
           for (int d = 0; d < 3; d++) dwi[d] = (double)(T)SF(A.mvel, d, j, n) - pv[d];
-        const double dvi[3] = {(double)Wj[VELX] - W0[VELX], (double)Wj[VELY] - W0[VELY],
-                               (double)Wj[VELZ] - W0[VELZ]};
+        const double dvi[3] = {(double)Wj[VELX] - v0[0], (double)Wj[VELY] - v0[1], (double)Wj[VELZ] - v0[2]};
         const double a = (dwi[0] * dwi[0] + dwi[1] * dwi[1] + dwi[2] * dwi[2]) +
                          1.0 * (dvi[0] * dvi[0] + dvi[1] * dvi[1] + dvi[2] * dvi[2]) + 1.0 / REF_HUGE;
         arg_max = (arg_max < a) ? a : arg_max;
@@ -246,8 +306,11 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
       // --- predictor face flux, first order (computePredictor.cpp:44-136)
       {
         const int L = fe.isL ? c : j, R = fe.isL ? j : c;
-        const T *WL = fe.isL ? Wc : Wj;
-        const T *WR = fe.isL ? Wj : Wc;
+        T WL[NF], WR[NF];
+        for (int k = 0; k < NF; k++) {
+          WL[k] = fe.isL ? Wc[k] : Wj[k];
+          WR[k] = fe.isL ? Wj[k] : Wc[k];
+        }
         T wij[3], dV;
         face_velocity<T, MOVING>(A, L, R, fe, wij, dV);
         T psi, Bn, flux[NF];
@@ -270,7 +333,11 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
     dtn = dti * A.courant;
 
     // --- predictor cell update (computePredictor.cpp:141-379), FP64
+    T tv[NF];
     {
+      double U0[NF], W0[NF];
+      for (int k = 0; k < NF; k++) U0[k] = SF(A.U, k, c, n);
+      to_primitive(U0, V, W0);
       while (U0[MASS] + (double)pU[MASS] * dtn <= 0.0) dtn *= 0.5;
       const double dt = rung_dt0(A.dtmax, dtn) / 128;
       const double divB = (double)pdv;
@@ -297,17 +364,18 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
       else W1[ENTR] = entropy_from_ethm_d(g64, W1);
       wave_count(&A.diag[FVM_DIAG_ETHM_FAILED], ef);
       const double idt = 1.0 / dt;
-      T tv[NF];
       for (int k = 0; k < NF; k++) tv[k] = (T)((W1[k] - W0[k]) * idt);
+    }
 
-      // --- reconstruction finish (computeReconstruction.cpp:162-239)
+    // --- reconstruction finish (computeReconstruction.cpp:162-239)
+    {
       const double invV = 1.0 / V;
       for (int k = 0; k < NF; k++) {
         gx[k] = (T)((double)gx[k] * invV);
         gy[k] = (T)((double)gy[k] * invV);
         gz[k] = (T)((double)gz[k] * invV);
       }
-      if (A.limiter != LIM_SHIPPED) {
+      if constexpr (LIMIT) {
         double lm[NF];
         for (int k = 0; k < NF; k++) lm[k] = 1.0;
         for (int e = eb; e < ee; e++) {
@@ -327,12 +395,11 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
           gz[k] = (T)((double)gz[k] * lm[k]);
         }
       }
-      for (int k = 0; k < NF; k++) {
-        SF(A.grad, k, c, n) = gx[k];
-        SF(A.grad, NF + k, c, n) = gy[k];
-        SF(A.grad, 2 * NF + k, c, n) = gz[k];
-        SF(A.Wt, k, c, n) = tv[k];
-      }
+      T *wc = A.rec_w + (size_t)c * REC;
+      stv<NF>(wc + R_T, tv);
+      stv<NF>(wc + R_G, gx);
+      stv<NF>(wc + R_G + NF, gy);
+      stv<NF>(wc + R_G + 2 * NF, gz);
     }
     A.dt_new[c] = dtn;
   }
@@ -357,30 +424,43 @@ __global__ void __launch_bounds__(FB) k_limit(FusedArgs<T> A) {
   const double tend = t0 + dt_glob;
   const double dtI = tend - t0;
   const double dth = 0.5 * smin(dtI, dtI);
-  double wc[NF], lo[NF], hi[NF], lm[NF];
+  T *rc = A.rec_w + (size_t)c * REC;
+  T W[NF], tv[NF], gg[3 * NF];
+  ldv<NF>(rc + R_W, W);
+  ldv<NF>(rc + R_T, tv);
+  ldv<3 * NF>(rc + R_G, gg);
+  T lo[NF], hi[NF];
   for (int k = 0; k < NF; k++) {
-    wc[k] = (double)SF(A.Wg, k, c, n);
-    lo[k] = (double)SF(A.mm, k, c, n);
-    hi[k] = (double)SF(A.mm, NF + k, c, n);
-    lm[k] = 1.0;
+    lo[k] = SF(A.mm, k, c, n) - W[k];
+    hi[k] = SF(A.mm, NF + k, c, n) - W[k];
+    lo[k] = smin(lo[k], T(0.0));
+    hi[k] = smax(hi[k], T(0.0));
   }
+  T lm[NF];
+  for (int k = 0; k < NF; k++) lm[k] = T(1.0);
+  const T dtt = (T)(dtI - dth);
   for (int e = A.off[c]; e < A.off[c + 1]; e++) {
     FE<T> fe;
     load_fe(A, e, fe);
     const T *dr = fe.isL ? fe.drL : fe.drR;
     for (int k = 0; k < NF; k++) {
-      const double dWt = (double)SF(A.Wt, k, c, n) * (dtI - dth);
-      const double dWx = (double)SF(A.grad, k, c, n) * (double)dr[0] + (double)SF(A.grad, NF + k, c, n) * (double)dr[1] +
-                         (double)SF(A.grad, 2 * NF + k, c, n) * (double)dr[2];
-      lm[k] = smin(lm[k], limiter_sl(A.limiter, dWx + dWt, smin(lo[k] - wc[k], 0.0), smax(hi[k] - wc[k], 0.0)));
+      const T dW = gg[k] * dr[0] + gg[NF + k] * dr[1] + gg[2 * NF + k] * dr[2] + tv[k] * dtt;
+      if (dabs(dW) <= T(1.0e-16) * smax(dabs(hi[k]), dabs(lo[k]))) continue;
+      const T x = fdiv(dW < T(0.0) ? lo[k] : hi[k], dW);
+      T ph;
+      if (A.limiter == LIM_MINMOD) ph = smin(T(1.0), x);
+      else ph = fdiv(x * x + T(2.0) * x + T(0.2e-16), x * x + x + T(2.0) + T(0.2e-16));
+      lm[k] = smin(lm[k], ph);
     }
   }
   for (int k = 0; k < NF; k++) {
-    SF(A.grad, k, c, n) = (T)((double)SF(A.grad, k, c, n) * lm[k]);
-    SF(A.grad, NF + k, c, n) = (T)((double)SF(A.grad, NF + k, c, n) * lm[k]);
-    SF(A.grad, 2 * NF + k, c, n) = (T)((double)SF(A.grad, 2 * NF + k, c, n) * lm[k]);
-    SF(A.Wt, k, c, n) = (T)((double)SF(A.Wt, k, c, n) * lm[k]);
+    tv[k] *= lm[k];
+    gg[k] *= lm[k];
+    gg[NF + k] *= lm[k];
+    gg[2 * NF + k] *= lm[k];
   }
+  stv<NF>(rc + R_T, tv);
+  stv<3 * NF>(rc + R_G, gg);
 }
 
 // ---------------------------------------------------------------------------
@@ -402,12 +482,11 @@ __global__ void __launch_bounds__(FB) k_flux_apply(FusedArgs<T> A) {
   const double dtf = smin(dtI, dtI);
   const T dti = (T)(dtI - 0.5 * dtf);
 
-  T Wc[NF], gc[3 * NF], tc[NF];
-  for (int k = 0; k < NF; k++) {
-    Wc[k] = SF(A.Wg, k, c, n);
-    tc[k] = SF(A.Wt, k, c, n);
-  }
-  for (int k = 0; k < 3 * NF; k++) gc[k] = SF(A.grad, k, c, n);
+  const T *rc = A.rec + (size_t)c * REC;
+  T Wc[NF], tc[NF], gc[3 * NF];
+  ldv<NF>(rc + R_W, Wc);
+  ldv<NF>(rc + R_T, tc);
+  ldv<3 * NF>(rc + R_G, gc);
 
   double dU[NF], gp[3], dv = 0.0;
   for (int k = 0; k < NF; k++) dU[k] = 0.0;
@@ -418,19 +497,26 @@ __global__ void __launch_bounds__(FB) k_flux_apply(FusedArgs<T> A) {
     FE<T> fe;
     load_fe(A, e, fe);
     const int j = fe.j;
-    T Wj[NF], gj[3 * NF], tj[NF];
-    for (int k = 0; k < NF; k++) {
-      Wj[k] = SF(A.Wg, k, j, n);
-      tj[k] = SF(A.Wt, k, j, n);
+    // neighbour face state, extrapolated as soon as it is loaded
+    T Wn[NF];
+    {
+      const T *rj = A.rec + (size_t)j * REC;
+      T Wj[NF], tj[NF], gj[3 * NF];
+      ldv<NF>(rj + R_W, Wj);
+      ldv<NF>(rj + R_T, tj);
+      ldv<3 * NF>(rj + R_G, gj);
+      const T *dr = fe.isL ? fe.drR : fe.drL;
+      for (int k = 0; k < NF; k++) Wn[k] = Wj[k] + (gj[k] * dr[0] + gj[NF + k] * dr[1] + gj[2 * NF + k] * dr[2] + tj[k] * dti);
     }
-    for (int k = 0; k < 3 * NF; k++) gj[k] = SF(A.grad, k, j, n);
-    const T *W0L = fe.isL ? Wc : Wj, *W0R = fe.isL ? Wj : Wc;
-    const T *gL = fe.isL ? gc : gj, *gR = fe.isL ? gj : gc;
-    const T *tL = fe.isL ? tc : tj, *tR = fe.isL ? tj : tc;
+    T Wo[NF];
+    {
+      const T *dr = fe.isL ? fe.drL : fe.drR;
+      for (int k = 0; k < NF; k++) Wo[k] = Wc[k] + (gc[k] * dr[0] + gc[NF + k] * dr[1] + gc[2 * NF + k] * dr[2] + tc[k] * dti);
+    }
     T WL[NF], WR[NF];
     for (int k = 0; k < NF; k++) {
-      WL[k] = W0L[k] + (gL[k] * fe.drL[0] + gL[NF + k] * fe.drL[1] + gL[2 * NF + k] * fe.drL[2] + tL[k] * dti);
-      WR[k] = W0R[k] + (gR[k] * fe.drR[0] + gR[NF + k] * fe.drR[1] + gR[2 * NF + k] * fe.drR[2] + tR[k] * dti);
+      WL[k] = fe.isL ? Wo[k] : Wn[k];
+      WR[k] = fe.isL ? Wn[k] : Wo[k];
     }
     bad = bad || !(WL[DENS] > T(0.0)) || !(WL[ETHM] > T(0.0)) || !(WR[DENS] > T(0.0)) || !(WR[ETHM] > T(0.0));
     T wij[3], dV;
@@ -507,11 +593,13 @@ __global__ void __launch_bounds__(FB) k_flux_apply(FusedArgs<T> A) {
   wave_count(&A.diag[FVM_DIAG_ETHM_FAILED], ef);
   wave_count(&A.diag[FVM_DIAG_MASS_FAILED], failed);
   to_conservative(W1, V, Ui);
+  T Wn[NF];
   for (int k = 0; k < NF; k++) {
     SF(A.W64, k, c, n) = W1[k];
     SF(A.U, k, c, n) = Ui[k];
-    SF(A.Wg_next, k, c, n) = (T)W1[k];
+    Wn[k] = (T)W1[k];
   }
+  stv<NF>(A.rec_next + (size_t)c * REC + R_W, Wn);
 }
 
 __global__ void k_advance(double *tdev, unsigned long long *dtmin_bits, double dtmax, double *hist, int step) {
@@ -525,13 +613,13 @@ __global__ void k_advance(double *tdev, unsigned long long *dtmin_bits, double d
   }
 }
 
-// face-ends from the uploaded mesh: cell c's list cell_face[off[c]..off[c+1]);
+// face-end records from the uploaded mesh: cell c's list cell_face[off[c]..off[c+1]);
 // drL = centroid - pos[s1] and the reconstruction centroid wrap(centroid - pos[c])
 // are formed in FP64 exactly as the reference forms them
 template <typename T>
 __global__ void k_build_faceends(int n, const int *coff, const int *cface, const int *s1, const int *s2,
                                  const double *fc, const double *fnv, int nface, const double *pos, double Lx,
-                                 double Ly, double Lz, int E, int *nbr, T *fn, T *drL, T *cen) {
+                                 double Ly, double Lz, FaceEndRec<T> *fe) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const double L[3] = {Lx, Ly, Lz};
@@ -539,24 +627,36 @@ __global__ void k_build_faceends(int n, const int *coff, const int *cface, const
     const int f = cface[e];
     const int a = s1[f], b = s2[f];
     const bool isL = a == c;
-    nbr[e] = isL ? b : ~a;
+    FaceEndRec<T> r;
+    r.nbr = isL ? b : ~a;
     for (int d = 0; d < 3; d++) {
       const double cf = FLD(fc, d, f, nface);
-      SF(fn, d, e, E) = (T)FLD(fnv, d, f, nface);
-      SF(drL, d, e, E) = (T)(cf - FLD(pos, d, a, n));
-      if (cen) {
+      r.fn[d] = (T)FLD(fnv, d, f, nface);
+      r.drL[d] = (T)(cf - FLD(pos, d, a, n));
+      if constexpr (std::is_same<T, double>::value) {
         double v = cf - FLD(pos, d, c, n);
         if (v > 0.5 * L[d]) v -= L[d];
         else if (v < -0.5 * L[d]) v += L[d];
-        SF(cen, d, e, E) = (T)v;
+        r.cen[d] = v;
       }
     }
+    r.pad = 0;
+    fe[e] = r;
   }
 }
 
-template <typename S, typename D> __global__ void k_convert(const S *src, D *dst, size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = (D)src[i];
+// W field (SoA FP64) -> W slice of the records; records -> fields
+template <typename T> __global__ void k_rec_from_W(const double *W, T *rec, int n) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  T *r = rec + (size_t)c * REC;
+  for (int k = 0; k < REC; k++) r[k] = T(0.0);
+  for (int k = 0; k < NF; k++) r[R_W + k] = (T)SF(W, k, c, n);
+}
+template <typename T, typename S> __global__ void k_rec_to_field(const T *rec, int off, S *dst, int n) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  for (int k = 0; k < NF; k++) SF(dst, k, c, n) = (S)rec[(size_t)c * REC + off + k];
 }
 
 }  // namespace fvm
@@ -573,9 +673,7 @@ static unsigned fused_grid(int n) {
 
 void fvm_fast_release(fvm_ctx *c) {
   FusedState &z = c->fz;
-  for (DBuf *b : {&z.nbr, &z.fn, &z.drL, &z.cen, &z.Wg[0], &z.Wg[1], &z.grad, &z.Wt, &z.mm, &z.dt_new, &z.dtmin,
-                  &z.tdev, &z.hist})
-    fvm_release(*b);
+  for (DBuf *b : {&z.fe, &z.rec[0], &z.rec[1], &z.mm, &z.dt_new, &z.dtmin, &z.tdev, &z.hist}) fvm_release(*b);
   for (auto &e : z.ev)
     if (e) (void)hipEventDestroy(e);
   z.ev.clear();
@@ -587,19 +685,14 @@ template <typename T> static fvm::FusedArgs<T> fused_args(fvm_ctx *c, double dtm
   fvm::FusedArgs<T> A;
   A.n = c->np;
   A.off = (const int *)c->m_coff.p;
-  A.nbr = (const int *)z.nbr.p;
-  A.fn = (const T *)z.fn.p;
-  A.drL = (const T *)z.drL.p;
-  A.cen = (const T *)z.cen.p;
-  A.E = z.E;
+  A.fe = (const fvm::FaceEndRec<T> *)z.fe.p;
   A.cvol = (const double *)c->m_cvol.p;
   A.mvel = (const double *)c->field[FVM_FIELD_MESH_VEL].p;
-  A.Wg = (const T *)z.Wg[z.cur].p;
-  A.Wg_next = (T *)z.Wg[z.cur ^ 1].p;
+  A.rec = (const T *)z.rec[z.cur].p;
+  A.rec_w = (T *)z.rec[z.cur].p;
+  A.rec_next = (T *)z.rec[z.cur ^ 1].p;
   A.W64 = (double *)c->field[FVM_FIELD_W].p;
   A.U = (double *)c->field[FVM_FIELD_U].p;
-  A.grad = (T *)z.grad.p;
-  A.Wt = (T *)z.Wt.p;
   A.mm = (T *)z.mm.p;
   A.dt_new = (double *)z.dt_new.p;
   A.dtmin_bits = (unsigned long long *)z.dtmin.p;
@@ -618,11 +711,8 @@ template <typename T> static int fused_setup_t(fvm_ctx *c) {
   z.E = E;
   int rc;
   const size_t es = sizeof(T);
-  if ((rc = fvm_ensure(c, z.nbr, sizeof(int) * (size_t)E)) || (rc = fvm_ensure(c, z.fn, es * 3 * (size_t)E)) ||
-      (rc = fvm_ensure(c, z.drL, es * 3 * (size_t)E)) ||
-      (rc = fvm_ensure(c, z.cen, std::is_same<T, double>::value ? es * 3 * (size_t)E : 8)) ||
-      (rc = fvm_ensure(c, z.Wg[0], es * 10 * (size_t)n)) || (rc = fvm_ensure(c, z.Wg[1], es * 10 * (size_t)n)) ||
-      (rc = fvm_ensure(c, z.grad, es * 30 * (size_t)n)) || (rc = fvm_ensure(c, z.Wt, es * 10 * (size_t)n)) ||
+  if ((rc = fvm_ensure(c, z.fe, sizeof(fvm::FaceEndRec<T>) * (size_t)E)) ||
+      (rc = fvm_ensure(c, z.rec[0], es * fvm::REC * (size_t)n)) || (rc = fvm_ensure(c, z.rec[1], es * fvm::REC * (size_t)n)) ||
       (rc = fvm_ensure(c, z.mm, es * 20 * (size_t)n)) || (rc = fvm_ensure(c, z.dt_new, sizeof(double) * (size_t)n)) ||
       (rc = fvm_ensure(c, z.dtmin, sizeof(unsigned long long))) || (rc = fvm_ensure(c, z.tdev, sizeof(double) * 2)))
     return rc;
@@ -630,19 +720,17 @@ template <typename T> static int fused_setup_t(fvm_ctx *c) {
   hipLaunchKernelGGL(fvm::k_build_faceends<T>, dim3(nb), dim3(256), 0, c->stream, n, (const int *)c->m_coff.p,
                      (const int *)c->m_cface.p, (const int *)c->m_s1.p, (const int *)c->m_s2.p, (const double *)c->m_c.p,
                      (const double *)c->m_n.p, c->nface, (const double *)c->field[FVM_FIELD_POS].p,
-                     c->prm.domain_size[0], c->prm.domain_size[1], c->prm.domain_size[2], E, (int *)z.nbr.p, (T *)z.fn.p,
-                     (T *)z.drL.p, std::is_same<T, double>::value ? (T *)z.cen.p : (T *)nullptr);
+                     c->prm.domain_size[0], c->prm.domain_size[1], c->prm.domain_size[2], (fvm::FaceEndRec<T> *)z.fe.p);
   FVM_HIPCHK(c, hipGetLastError());
-  const size_t nw = 10 * (size_t)n;
-  hipLaunchKernelGGL((fvm::k_convert<double, T>), dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, c->stream,
-                     (const double *)c->field[FVM_FIELD_W].p, (T *)z.Wg[0].p, nw);
-  FVM_HIPCHK(c, hipGetLastError());
+  for (int q = 0; q < 2; q++) {
+    hipLaunchKernelGGL(fvm::k_rec_from_W<T>, dim3(nb), dim3(256), 0, c->stream, (const double *)c->field[FVM_FIELD_W].p,
+                       (T *)z.rec[q].p, n);
+    FVM_HIPCHK(c, hipGetLastError());
+  }
   const unsigned long long inf = 0x7FF0000000000000ULL;
   const double t2[2] = {c->t_global, 0.0};
   FVM_HIPCHK(c, hipMemcpyAsync(z.dtmin.p, &inf, sizeof inf, hipMemcpyHostToDevice, c->stream));
   FVM_HIPCHK(c, hipMemcpyAsync(z.tdev.p, t2, sizeof t2, hipMemcpyHostToDevice, c->stream));
-  FVM_HIPCHK(c, hipMemsetAsync(z.grad.p, 0, es * 30 * (size_t)n, c->stream));
-  FVM_HIPCHK(c, hipMemsetAsync(z.Wt.p, 0, es * 10 * (size_t)n, c->stream));
   FVM_HIPCHK(c, hipStreamSynchronize(c->stream));
   z.cur = 0;
   return FVM_OK;
@@ -686,7 +774,8 @@ template <typename T, bool MOVING> static int fused_run_t(fvm_ctx *c, int nsteps
     fvm::FusedArgs<T> A = fused_args<T>(c, dtmax);
     hipEvent_t *ev = z.timing ? &z.ev[5 * s] : nullptr;
     if (ev) FVM_HIPCHK(c, hipEventRecord(ev[0], c->stream));
-    hipLaunchKernelGGL((fvm::k_cfl_predict_recon<T, MOVING>), dim3(grid), dim3(FB), 0, c->stream, A);
+    if (lim) hipLaunchKernelGGL((fvm::k_cfl_predict_recon<T, MOVING, true>), dim3(grid), dim3(FB), 0, c->stream, A);
+    else hipLaunchKernelGGL((fvm::k_cfl_predict_recon<T, MOVING, false>), dim3(grid), dim3(FB), 0, c->stream, A);
     FVM_HIPCHK(c, hipGetLastError());
     if (ev) FVM_HIPCHK(c, hipEventRecord(ev[1], c->stream));
     if (lim) {
@@ -745,32 +834,32 @@ int fvm_fused_dt_history(fvm_ctx *c, double *dts, int n) {
 }
 
 // gradients and Wt of the last step into the per-cell fields
+template <typename T, typename S> static int fused_sync_t(fvm_ctx *c) {
+  FusedState &z = c->fz;
+  const int n = c->np;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  // the last completed step's gradients sit in the records it read (z.cur ^ 1)
+  const T *rec = (const T *)z.rec[z.cur ^ 1].p;
+  const int dstf[4] = {FVM_FIELD_GRAD_X, FVM_FIELD_GRAD_Y, FVM_FIELD_GRAD_Z, FVM_FIELD_DWDT};
+  const int off[4] = {fvm::R_G, fvm::R_G + fvm::NF, fvm::R_G + 2 * fvm::NF, fvm::R_T};
+  for (int q = 0; q < 4; q++) {
+    hipLaunchKernelGGL((fvm::k_rec_to_field<T, S>), dim3(nb), dim3(256), 0, c->stream, rec, off[q],
+                       (S *)c->field[dstf[q]].p, n);
+    FVM_HIPCHK(c, hipGetLastError());
+  }
+  FVM_HIPCHK(c, hipMemcpyAsync(c->field[FVM_FIELD_DT_NEW].p, z.dt_new.p, sizeof(double) * (size_t)n,
+                               hipMemcpyDeviceToDevice, c->stream));
+  FVM_HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FVM_OK;
+}
+
 int fvm_fused_sync(fvm_ctx *c) {
   if (!c) return FVM_EARG;
   if (!c->fz.ready) return fvm_fail(c, FVM_ESTATE, "no fused state");
   (void)hipSetDevice(c->device);
-  FusedState &z = c->fz;
-  const size_t n = (size_t)c->np;
   const bool shipped = c->prm.storage == FVM_STORAGE_SHIPPED;
-  const int dstf[4] = {FVM_FIELD_GRAD_X, FVM_FIELD_GRAD_Y, FVM_FIELD_GRAD_Z, FVM_FIELD_DWDT};
-  for (int q = 0; q < 4; q++) {
-    const size_t cnt = 10 * n;
-    const unsigned nb = (unsigned)((cnt + 255) / 256);
-    const char *src = (q < 3) ? (const char *)z.grad.p + (size_t)q * cnt * (z.precision == FVM_FUSED_FP64 ? 8 : 4)
-                              : (const char *)z.Wt.p;
-    void *dst = c->field[dstf[q]].p;
-    if (z.precision == FVM_FUSED_FP64) {
-      if (shipped) hipLaunchKernelGGL((fvm::k_convert<double, float>), dim3(nb), dim3(256), 0, c->stream, (const double *)src, (float *)dst, cnt);
-      else hipLaunchKernelGGL((fvm::k_convert<double, double>), dim3(nb), dim3(256), 0, c->stream, (const double *)src, (double *)dst, cnt);
-    } else {
-      if (shipped) hipLaunchKernelGGL((fvm::k_convert<float, float>), dim3(nb), dim3(256), 0, c->stream, (const float *)src, (float *)dst, cnt);
-      else hipLaunchKernelGGL((fvm::k_convert<float, double>), dim3(nb), dim3(256), 0, c->stream, (const float *)src, (double *)dst, cnt);
-    }
-    FVM_HIPCHK(c, hipGetLastError());
-  }
-  FVM_HIPCHK(c, hipMemcpyAsync(c->field[FVM_FIELD_DT_NEW].p, z.dt_new.p, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
-  FVM_HIPCHK(c, hipStreamSynchronize(c->stream));
-  return FVM_OK;
+  if (c->fz.precision == FVM_FUSED_FP64) return shipped ? fused_sync_t<double, float>(c) : fused_sync_t<double, double>(c);
+  return shipped ? fused_sync_t<float, float>(c) : fused_sync_t<float, double>(c);
 }
 
 int fvm_fused_timing(fvm_ctx *c, int enable) {
@@ -789,5 +878,49 @@ int fvm_fused_kernel_ms(fvm_ctx *c, double *ms, long long *launches, int n) {
     ms[k] = c->fz.ms[k];
     if (launches) launches[k] = c->fz.launches[k];
   }
+  return FVM_OK;
+}
+
+// ===========================================================================
+// stream-copy bandwidth probe (the measured HBM ceiling next to the 8 TB/s
+// data-sheet peak; also the calibration kernel for counter-based traffic)
+// ===========================================================================
+namespace fvm {
+__global__ void __launch_bounds__(256) k_stream_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+}  // namespace fvm
+
+int fvm_measure_copy_bandwidth(fvm_ctx *c, long long bytes, int iters, double *gbps) {
+  if (!c || bytes < (1 << 20) || iters < 1 || !gbps) return FVM_EARG;
+  (void)hipSetDevice(c->device);
+  const size_t n = (size_t)bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  FVM_HIPCHK(c, hipMalloc(&a, n * 16));
+  if (hipMalloc(&b, n * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return fvm_fail(c, FVM_EHIP, "hipMalloc failed");
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipMemsetAsync(a, 0, n * 16, c->stream);
+  const unsigned grid = 256 * 8 * 4;
+  hipLaunchKernelGGL(fvm::k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const float4 *)a, (float4 *)b, n);
+  (void)hipEventRecord(e0, c->stream);
+  for (int k = 0; k < iters; k++)
+    hipLaunchKernelGGL(fvm::k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const float4 *)a, (float4 *)b, n);
+  (void)hipEventRecord(e1, c->stream);
+  hipError_t err = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (err != hipSuccess) return fvm_fail(c, FVM_EHIP, hipGetErrorString(err));
+  *gbps = 2.0 * (double)(n * 16) * iters / (ms * 1e-3) / 1e9;
   return FVM_OK;
 }

@@ -39,7 +39,7 @@ struct FusedState {
   int moving = 0;
   int E = 0;
   int cur = 0;
-  DBuf nbr, fn, drL, cen, Wg[2], grad, Wt, mm, dt_new, dtmin, tdev, hist;
+  DBuf fe, rec[2], mm, dt_new, dtmin, tdev, hist;
   bool timing = false;
   std::vector<hipEvent_t> ev;  // 5 per timed step
   double ms[4] = {0, 0, 0, 0};

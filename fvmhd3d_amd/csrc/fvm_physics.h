@@ -27,6 +27,16 @@ __device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
 
+// Division and square root of the face arithmetic: IEEE double for the FP64
+// path (operation for operation the reference's); for the FP32 face path the
+// hardware reciprocal and square root (v_rcp_f32 / v_sqrt_f32, about 1 ulp).
+__device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double fsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Problem_compute_pressure (Problem.cpp:234-237)
 template <typename T> __device__ __forceinline__ T pressure(T gamma, T ethm) {
   return gamma > T(1.0) ? (gamma - T(1.0)) * ethm : ethm;
@@ -117,28 +127,28 @@ __device__ __forceinline__ void riemann_hlld(T gamma_gas, T *flux, const T Bx, c
   const T gpr = gamma_gas * pres_R;
   const T gpbl = gpl + B2_L;
   const T gpbr = gpr + B2_R;
-  const T cfl2 = gpbl / dens_L;
-  const T cfr2 = gpbr / dens_R;
-  const T cfmax = dsqrt(smax(cfl2, cfr2));
+  const T cfl2 = fdiv(gpbl, dens_L);
+  const T cfr2 = fdiv(gpbr, dens_R);
+  const T cfmax = fsqrt(smax(cfl2, cfr2));
   const T S_L = smin(velx_L, velx_R) - cfmax;
   const T S_R = smax(velx_L, velx_R) + cfmax;
   const T pT_L = pres_L + chalf * B2_L;
   const T pT_R = pres_R + chalf * B2_R;
-  const T iSM = c1 / ((S_R - velx_R) * dens_R - (S_L - velx_L) * dens_L);
+  const T iSM = frcp((S_R - velx_R) * dens_R - (S_L - velx_L) * dens_L);
   const T S_M = iSM * ((S_R - velx_R) * momx_R - (S_L - velx_L) * momx_L - pT_R + pT_L);
-  const T ipTs = c1 / ((S_R - velx_R) * dens_R - (S_L - velx_L) * dens_L);
+  const T ipTs = frcp((S_R - velx_R) * dens_R - (S_L - velx_L) * dens_L);
   const T pT_s = ipTs * ((S_R - velx_R) * dens_R * pT_L - (S_L - velx_L) * dens_L * pT_R +
                          dens_L * dens_R * (S_R - velx_R) * (S_L - velx_L) * (velx_R - velx_L));
   const T velx_L_s = S_M, velx_L_ss = S_M, velx_R_s = S_M, velx_R_ss = S_M;
   const T B2x = Bx * Bx;
-  const T iSLmSM = c1 / (S_L - S_M);
-  const T iSRmSM = c1 / (S_R - S_M);
+  const T iSLmSM = frcp(S_L - S_M);
+  const T iSRmSM = frcp(S_R - S_M);
   const T dens_L_s = dens_L * (S_L - velx_L) * iSLmSM;
   const T dens_R_s = dens_R * (S_R - velx_R) * iSRmSM;
   const T divL = dens_L * (S_L - velx_L) * (S_L - S_M) - B2x;
   const T divR = dens_R * (S_R - velx_R) * (S_R - S_M) - B2x;
-  const T idivL = (divL != c0) ? c1 / divL : c0;
-  const T idivR = (divR != c0) ? c1 / divR : c0;
+  const T idivL = (divL != c0) ? frcp(divL) : c0;
+  const T idivR = (divR != c0) ? frcp(divR) : c0;
   const T vely_L_s = vely_L - Bx * By_L * (S_M - velx_L) * idivL;
   const T velz_L_s = velz_L - Bx * Bz_L * (S_M - velx_L) * idivL;
   const T By_L_s = By_L * (dens_L * sqr(S_L - velx_L) - B2x) * idivL;
@@ -156,11 +166,11 @@ __device__ __forceinline__ void riemann_hlld(T gamma_gas, T *flux, const T Bx, c
   const T etot_R_s =
       ((S_R - velx_R) * etot_R - pT_R * velx_R + pT_s * S_M + Bx * (vB_R - vB_R_s)) * iSRmSM;
   const T dens_L_ss = dens_L_s, dens_R_ss = dens_R_s;
-  const T sDens_L_s = dsqrt(dens_L_s);
-  const T sDens_R_s = dsqrt(dens_R_s);
-  const T S_L_s = S_M - dabs(Bx / sDens_L_s);
-  const T S_R_s = S_M + dabs(Bx / sDens_R_s);
-  const T idsqroot = c1 / (sDens_L_s + sDens_R_s);
+  const T sDens_L_s = fsqrt(dens_L_s);
+  const T sDens_R_s = fsqrt(dens_R_s);
+  const T S_L_s = S_M - dabs(fdiv(Bx, sDens_L_s));
+  const T S_R_s = S_M + dabs(fdiv(Bx, sDens_R_s));
+  const T idsqroot = frcp(sDens_L_s + sDens_R_s);
   const T vely_ss =
       idsqroot * (sDens_L_s * vely_L_s + sDens_R_s * vely_R_s + (By_R_s - By_L_s) * signBx);
   const T velz_ss =
@@ -252,8 +262,8 @@ __device__ __forceinline__ void compute_flux(const T gamma_gas, const T *wij, co
   const T ex = normal[0];
   const T ey = normal[1];
   const T ez = normal[2];
-  const T ds = dsqrt(ex * ex + ey * ey);
-  const T ids = (ds != c0) ? c1 / ds : c0;
+  const T ds = fsqrt(ex * ex + ey * ey);
+  const T ids = (ds != c0) ? frcp(ds) : c0;
   const T cosph = (ids == c0) ? c1 : ex * ids;
   const T sinph = (ids == c0) ? c0 : ey * ids;
   const T costh = ez;
@@ -321,13 +331,13 @@ __device__ __forceinline__ void compute_flux(const T gamma_gas, const T *wij, co
       break;
   }
 
-  const T cfl2 = (gamma_gas * pres_L + sqr(Bx_L) + sqr(By_L) + sqr(Bz_L)) / dens_L;
-  const T cfr2 = (gamma_gas * pres_R + sqr(Bx_R) + sqr(By_R) + sqr(Bz_R)) / dens_R;
-  const T chL = dsqrt(cfl2 + c0 * sqr(velx_L - wn_ij));
-  const T chR = dsqrt(cfr2 + c0 * sqr(velx_R - wn_ij));
+  const T cfl2 = fdiv(gamma_gas * pres_L + sqr(Bx_L) + sqr(By_L) + sqr(Bz_L), dens_L);
+  const T cfr2 = fdiv(gamma_gas * pres_R + sqr(Bx_R) + sqr(By_R) + sqr(Bz_R), dens_R);
+  const T chL = fsqrt(cfl2 + c0 * sqr(velx_L - wn_ij));
+  const T chR = fsqrt(cfr2 + c0 * sqr(velx_R - wn_ij));
 
-  Bn_ij = (Bx_L * chL + Bx_R * chR + (psiL - psiR)) / (chL + chR);
-  psi_ij = (psiL * chR + psiR * chL + chR * chL * (Bx_L - Bx_R)) / (chL + chR);
+  Bn_ij = fdiv(Bx_L * chL + Bx_R * chR + (psiL - psiR), chL + chR);
+  psi_ij = fdiv(psiL * chR + psiR * chL + chR * chL * (Bx_L - Bx_R), chL + chR);
 
   T F[NF];
   riemann_hlld<T>(gamma_gas, F, Bn_ij, wn_ij, dens_L, pres_L, ethm_L, velx_L, vely_L, velz_L, By_L,

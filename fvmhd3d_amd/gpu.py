@@ -75,6 +75,7 @@ SIGNATURES = {
     "fvm_fused_sync": [_P],
     "fvm_fused_timing": [_P, _I],
     "fvm_fused_kernel_ms": [_P, _P, _P, _I],
+    "fvm_measure_copy_bandwidth": [_P, ctypes.c_longlong, _I, ctypes.POINTER(ctypes.c_double)],
 }
 FUSED_FP64, FUSED_FP32 = 0, 1
 
@@ -241,6 +242,11 @@ class Context:
         nl = np.zeros(4, dtype=np.int64)
         self._chk(self.L.fvm_fused_kernel_ms(self.h, _ptr(ms), _ptr(nl), 4))
         return ms, nl
+
+    def copy_bandwidth(self, nbytes=1 << 30, iters=20):
+        g = ctypes.c_double()
+        self._chk(self.L.fvm_measure_copy_bandwidth(self.h, int(nbytes), int(iters), ctypes.byref(g)))
+        return g.value
 
     # convenience: whole state in / out
     def upload_state(self, st):

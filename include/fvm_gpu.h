@@ -179,6 +179,9 @@ int fvm_fused_sync(fvm_ctx *ctx);
 int fvm_fused_timing(fvm_ctx *ctx, int enable);
 int fvm_fused_kernel_ms(fvm_ctx *ctx, double *ms, long long *launches, int n);
 
+/* read+write bandwidth of a plain 16-byte-vector copy of `bytes` bytes, GB/s */
+int fvm_measure_copy_bandwidth(fvm_ctx *ctx, long long bytes, int iters, double *gbps);
+
 /* diagnostics counters accumulated since the last reset */
 int fvm_get_diagnostics(fvm_ctx *ctx, long long *out, int n);
 int fvm_reset_diagnostics(fvm_ctx *ctx);
