@@ -54,21 +54,22 @@ def build_workload(name):
 
 def k2_bytes(ncell, nfe, prec_bytes):
     """Algorithmic HBM bytes of one flux+apply launch: every array it must touch, once.
-    per cell: read gathered W (10), gradients (30), Wt (10) in face precision; FP64
-    W, U (10+10), volume, CSR offset; write FP64 W, U and the face-precision W copy.
-    per face-end: neighbour index + area vector (3) + centroid offset (3)."""
+    per cell: the gather record (W 10, Wt 10, gradients 30 in face precision), FP64 W and
+    U (10+10), volume, CSR offset; writes FP64 W, U and next step's W slice (10).
+    per face-end: one 32-byte face-end record (neighbour, area vector, centroid offset)."""
     p = prec_bytes
-    per_cell = 10 * p + 30 * p + 10 * p + 80 + 80 + 8 + 4 + 80 + 80 + 10 * p
-    per_fe = 4 + 3 * p + 3 * p
+    per_cell = 50 * p + 80 + 80 + 8 + 4 + 80 + 80 + 10 * p
+    per_fe = 32 if p == 4 else 80
     return ncell * per_cell + nfe * per_fe
 
 
-def k1_bytes(ncell, nfe, prec_bytes):
-    """CFL + predictor + reconstruction launch: per cell read FP64 U, volume, offset,
-    gathered W; write gradients, Wt, dt_new; per face-end the same geometry stream."""
+def k1_bytes(ncell, nfe, prec_bytes, limiter):
+    """CFL + predictor + reconstruction launch: per cell read FP64 U, volume, offset and the
+    W slice of the record; write Wt + gradients (40), dt_new, and min/max (20) for the
+    opt-in limiters; per face-end the same 32-byte record stream."""
     p = prec_bytes
-    per_cell = 80 + 8 + 4 + 10 * p + 30 * p + 10 * p + 8
-    per_fe = 4 + 3 * p + 3 * p
+    per_cell = 80 + 8 + 4 + 10 * p + 40 * p + 8 + (20 * p if limiter else 0)
+    per_fe = 32 if p == 4 else 80
     return ncell * per_cell + nfe * per_fe
 
 
@@ -155,7 +156,7 @@ def main():
                 "kernel": "k_flux_apply<float> (flux + FP64 apply)", "bytes_per_launch": k2_b,
                 "ms_per_launch": round(k2_ms, 4)}
     k1_ms = ms[0] / max(nl[0], 1)
-    k1_b = k1_bytes(m.ncell, nfe, 4)
+    k1_b = k1_bytes(m.ncell, nfe, 4, lim != 0)
 
     extra = {"kernels": {
         "k_cfl_predict_recon": {"ms": round(k1_ms, 4), "GB/s": round(k1_b / (k1_ms * 1e-3) / 1e9, 1)},
