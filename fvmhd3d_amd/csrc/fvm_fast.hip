@@ -376,23 +376,23 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
         gz[k] = (T)((double)gz[k] * invV);
       }
       if constexpr (LIMIT) {
-        double lm[NF];
-        for (int k = 0; k < NF; k++) lm[k] = 1.0;
+        T lm[NF];
+        for (int k = 0; k < NF; k++) lm[k] = T(1.0);
         for (int e = eb; e < ee; e++) {
           FE<T> fe;
           load_fe(A, e, fe);
           for (int k = 0; k < NF; k++) {
-            const double dWx = (double)(gx[k] * fe.cen[0] + gy[k] * fe.cen[1] + gz[k] * fe.cen[2]);
-            lm[k] = smin(lm[k], limiter_rec(A.limiter, dWx, (double)dmn[k], (double)dmx[k]));
+            const T dWx = gx[k] * fe.cen[0] + gy[k] * fe.cen[1] + gz[k] * fe.cen[2];
+            lm[k] = smin(lm[k], limiter_rec_t<T>(A.limiter, dWx, dmn[k], dmx[k]));
           }
         }
         for (int k = 0; k < NF; k++) {
-          SF(A.mm, k, c, n) = (T)((double)dmn[k] + (double)Wc[k]);
-          SF(A.mm, NF + k, c, n) = (T)((double)dmx[k] + (double)Wc[k]);
-          tv[k] = (T)((double)tv[k] * lm[k]);
-          gx[k] = (T)((double)gx[k] * lm[k]);
-          gy[k] = (T)((double)gy[k] * lm[k]);
-          gz[k] = (T)((double)gz[k] * lm[k]);
+          SF(A.mm, k, c, n) = dmn[k] + Wc[k];
+          SF(A.mm, NF + k, c, n) = dmx[k] + Wc[k];
+          tv[k] = tv[k] * lm[k];
+          gx[k] = gx[k] * lm[k];
+          gy[k] = gy[k] * lm[k];
+          gz[k] = gz[k] * lm[k];
         }
       }
       T *wc = A.rec_w + (size_t)c * REC;
@@ -445,12 +445,7 @@ __global__ void __launch_bounds__(FB) k_limit(FusedArgs<T> A) {
     const T *dr = fe.isL ? fe.drL : fe.drR;
     for (int k = 0; k < NF; k++) {
       const T dW = gg[k] * dr[0] + gg[NF + k] * dr[1] + gg[2 * NF + k] * dr[2] + tv[k] * dtt;
-      if (dabs(dW) <= T(1.0e-16) * smax(dabs(hi[k]), dabs(lo[k]))) continue;
-      const T x = fdiv(dW < T(0.0) ? lo[k] : hi[k], dW);
-      T ph;
-      if (A.limiter == LIM_MINMOD) ph = smin(T(1.0), x);
-      else ph = fdiv(x * x + T(2.0) * x + T(0.2e-16), x * x + x + T(2.0) + T(0.2e-16));
-      lm[k] = smin(lm[k], ph);
+      lm[k] = smin(lm[k], limiter_sl_t<T>(A.limiter, dW, lo[k], hi[k]));
     }
   }
   for (int k = 0; k < NF; k++) {

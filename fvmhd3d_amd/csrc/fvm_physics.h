@@ -98,6 +98,32 @@ __device__ __forceinline__ double limiter_sl(int mode, double dWi, double mn, do
   return phi(mode, mx / dWi);
 }
 
+// limiters in the face precision T: the exact double versions above for the
+// FP64 path, hardware reciprocal for the FP32 face path
+template <typename T> __device__ __forceinline__ T phi_t(int mode, T x) {
+  if (mode == LIM_SHIPPED) return T(1.0);
+  if (mode == LIM_MINMOD) return smin(T(1.0), x);
+  const T fac = T(0.2), eps1 = T(1.0e-16);
+  return fdiv(x * x + T(2.0) * x + fac * eps1, x * x + x + T(2.0) + fac * eps1);
+}
+template <typename T> __device__ __forceinline__ T limiter_rec_t(int mode, T dWi, T mn, T mx) {
+  if constexpr (sizeof(T) == 8) {
+    return limiter_rec(mode, dWi, mn, mx);
+  } else {
+    if (dabs(dWi) <= T(1.0e-16) * smax(dabs(mx), dabs(mn))) return T(1.0);
+    const T invWi = frcp(dWi);
+    return phi_t<T>(mode, (dWi < T(0.0) ? mn : mx) * invWi);
+  }
+}
+template <typename T> __device__ __forceinline__ T limiter_sl_t(int mode, T dWi, T mn, T mx) {
+  if constexpr (sizeof(T) == 8) {
+    return limiter_sl(mode, dWi, mn, mx);
+  } else {
+    if (dabs(dWi) <= T(1.0e-16) * smax(dabs(mx), dabs(mn))) return T(1.0);
+    return phi_t<T>(mode, fdiv(dWi < T(0.0) ? mn : mx, dWi));
+  }
+}
+
 // riemannSolver_HLLD (computeFlux.cpp:178-425): HLLD sampled at the face speed w,
 // flux of the moving interface in the a*U** + b*U* + c*U form.
 template <typename T>
