@@ -92,6 +92,7 @@ template <typename T> struct FusedArgs {
   unsigned long long *diag;
   double gamma, courant, dtmax;
   int limiter;
+  int write_w64;            // FP32 path: store FP64 W (last step of a run)
 };
 
 #define SF(p, k, i, n) (p)[(size_t)(k) * (size_t)(n) + (size_t)(i)]
@@ -138,9 +139,40 @@ template <typename T> struct FE {
   T area, nrm[3], drL[3], drR[3], cen[3], ncen[3];
 };
 
+// streamed once per launch: non-temporal loads keep them from evicting the
+// gathered cell records from L2
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ FaceEndRec<float> nt_load(const FaceEndRec<float> *p) {
+  const v4f *q = reinterpret_cast<const v4f *>(p);
+  const v4f a = __builtin_nontemporal_load(q);
+  const v4f b = __builtin_nontemporal_load(q + 1);
+  FaceEndRec<float> r;
+  r.nbr = __float_as_int(a.x);
+  r.fn[0] = a.y; r.fn[1] = a.z; r.fn[2] = a.w;
+  r.drL[0] = b.x; r.drL[1] = b.y; r.drL[2] = b.z;
+  r.pad = b.w;
+  return r;
+}
+__device__ __forceinline__ FaceEndRec<double> nt_load(const FaceEndRec<double> *p) {
+  const v2d *q = reinterpret_cast<const v2d *>(p);
+  const v2d a = __builtin_nontemporal_load(q), b = __builtin_nontemporal_load(q + 1),
+            cc = __builtin_nontemporal_load(q + 2), d = __builtin_nontemporal_load(q + 3),
+            e = __builtin_nontemporal_load(q + 4);
+  FaceEndRec<double> r;
+  r.nbr = (int)(__double_as_longlong(a.x) & 0xffffffffLL);
+  r.pad = 0;
+  r.fn[0] = a.y; r.fn[1] = b.x; r.fn[2] = b.y;
+  r.drL[0] = cc.x; r.drL[1] = cc.y; r.drL[2] = d.x;
+  r.cen[0] = d.y; r.cen[1] = e.x; r.cen[2] = e.y;
+  return r;
+}
+template <typename T> __device__ __forceinline__ T ntl(const T *p) { return __builtin_nontemporal_load(p); }
+template <typename T> __device__ __forceinline__ void nts(T v, T *p) { __builtin_nontemporal_store(v, p); }
+
 template <typename T>
 __device__ __forceinline__ void load_fe(const FusedArgs<T> &A, int e, FE<T> &g) {
-  const FaceEndRec<T> r = A.fe[e];
+  const FaceEndRec<T> r = nt_load(A.fe + e);
   g.isL = r.nbr >= 0;
   g.j = g.isL ? r.nbr : ~r.nbr;
   const T fx = r.fn[0], fy = r.fn[1], fz = r.fn[2];
@@ -231,7 +263,7 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
     const int n = A.n;
     const double g64 = A.gamma;
     const T g = (T)A.gamma;
-    const double V = A.cvol[c];
+    const double V = ntl(A.cvol + c);
     const T *rc = A.rec + (size_t)c * REC;
     T Wc[NF];
     ldv<NF>(rc + R_W, Wc);
@@ -240,7 +272,7 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
     double v0[3], csi, h;
     {
       double U0[NF], W0[NF];
-      for (int k = 0; k < NF; k++) U0[k] = SF(A.U, k, c, n);
+      for (int k = 0; k < NF; k++) U0[k] = ntl(&SF(A.U, k, c, n));
       to_primitive(U0, V, W0);
       h = pow(V / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
       const double Pi = pressure(g64, W0[ETHM]) * g64;
@@ -534,13 +566,17 @@ __global__ void __launch_bounds__(FB) k_flux_apply(FusedArgs<T> A) {
   // computePrimitives at t1 = t0 + dt (computeFluidUpdate.cpp:367-505)
   const double t1 = t0 + dt_glob;
   const double dt = t1 - t0;
-  const double V = A.cvol[c];
+  const double V = ntl(A.cvol + c);
   double Ui[NF], wi[NF], Uc[NF], W1[NF];
-  for (int k = 0; k < NF; k++) {
-    Ui[k] = SF(A.U, k, c, n);
-    wi[k] = SF(A.W64, k, c, n);
-    Uc[k] = Ui[k] + (0.0 + dU[k]);
+  for (int k = 0; k < NF; k++) Ui[k] = ntl(&SF(A.U, k, c, n));
+  if constexpr (sizeof(T) == 8) {
+    // the FP64 path keeps the reference's stored primitives
+    for (int k = 0; k < NF; k++) wi[k] = ntl(&SF(A.W64, k, c, n));
+  } else {
+    // the FP32 face path re-derives them from U (they differ by rounding only)
+    to_primitive(Ui, V, wi);
   }
+  for (int k = 0; k < NF; k++) Uc[k] = Ui[k] + (0.0 + dU[k]);
   const double divB = 0.0 + dv;
   const bool failed = Uc[MASS] <= 0.0;
   bool ef = false;
@@ -590,8 +626,8 @@ __global__ void __launch_bounds__(FB) k_flux_apply(FusedArgs<T> A) {
   to_conservative(W1, V, Ui);
   T Wn[NF];
   for (int k = 0; k < NF; k++) {
-    SF(A.W64, k, c, n) = W1[k];
-    SF(A.U, k, c, n) = Ui[k];
+    if (sizeof(T) == 8 || A.write_w64) nts(W1[k], &SF(A.W64, k, c, n));
+    nts(Ui[k], &SF(A.U, k, c, n));
     Wn[k] = (T)W1[k];
   }
   stv<NF>(A.rec_next + (size_t)c * REC + R_W, Wn);
@@ -697,6 +733,7 @@ template <typename T> static fvm::FusedArgs<T> fused_args(fvm_ctx *c, double dtm
   A.courant = c->prm.courant_no;
   A.dtmax = dtmax;
   A.limiter = c->prm.limiter_mode;
+  A.write_w64 = 0;
   return A;
 }
 
@@ -767,6 +804,7 @@ template <typename T, bool MOVING> static int fused_run_t(fvm_ctx *c, int nsteps
     }
   for (int s = 0; s < nsteps; s++) {
     fvm::FusedArgs<T> A = fused_args<T>(c, dtmax);
+    A.write_w64 = (s == nsteps - 1) ? 1 : 0;
     hipEvent_t *ev = z.timing ? &z.ev[5 * s] : nullptr;
     if (ev) FVM_HIPCHK(c, hipEventRecord(ev[0], c->stream));
     if (lim) hipLaunchKernelGGL((fvm::k_cfl_predict_recon<T, MOVING, true>), dim3(grid), dim3(FB), 0, c->stream, A);
