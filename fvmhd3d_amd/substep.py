@@ -1,0 +1,305 @@
+"""Individual timesteps: the host side around the path.
+
+The reference keeps the rung bookkeeping on the host (Scheduler.h, Iterate.cpp);
+the path functions run on the active subset of each substep in the reference's
+index layout (active cells, then their inactive neighbours; localMesh.cpp:216-322).
+This module is that host side for a static periodic mesh:
+
+* ``RungClock``   power-of-two rungs on an integer tick clock with the
+                  Scheduler's semantics (Scheduler.h:4-206): dt(rung) =
+                  dtmax/2^rung, get_rung from exponent bits, pull of every rung
+                  whose boundary is reached, rung changes limited to rungs that
+                  are synchronised now (min_rung).
+* ``substep_layout`` the per-substep reference layout and mesh views.
+* ``Substepper``  one substep of Iterate.cpp:92-202 (GetActiveList,
+                  ComputeFluidUpdate .. computeFluidUpdate) against a backend:
+                  either the HIP library (``GpuBackend``) or any object with the
+                  same methods (the oracle, in the tests).
+"""
+from __future__ import annotations
+
+import math
+import struct
+
+import numpy as np
+
+from .state import CellState, to_conservative
+
+RUNGMAX = 32
+
+
+def exp_of(dt: float) -> int:
+    return (struct.unpack("<Q", struct.pack("<d", dt))[0] >> 52) & 0xFFF
+
+
+class RungClock:
+    """Power-of-two rungs on an integer tick clock (Scheduler.h)."""
+
+    def __init__(self, dtmax: float):
+        self.dtmax = dtmax
+        self.dt_tick = dtmax / (1 << (RUNGMAX - 1))
+        self.tsysU = 0
+        self.min_rung = 0
+
+    def get_rung(self, dt: float, floor_rung: int | None = None) -> int:
+        r = max(exp_of(self.dtmax) - exp_of(dt), self.min_rung if floor_rung is None else floor_rung)
+        assert r < RUNGMAX
+        return r
+
+    def get_dt(self, rung: int) -> float:
+        return self.dt_tick * float(1 << (RUNGMAX - 1 - rung))
+
+    def tsys(self) -> float:
+        return self.dt_tick * self.tsysU
+
+    def pull(self, rung: np.ndarray) -> tuple[np.ndarray, float]:
+        """Advance to the next synchronisation time; return (active mask, dt_global)."""
+        max_rung = int(rung.max())
+        assert max_rung != RUNGMAX - 1
+        dtU = 1 << (RUNGMAX - 1 - max_rung)
+        tnextU = (self.tsysU // dtU + 1) * dtU
+        flags = self.tsysU ^ tnextU
+        active = np.zeros(len(rung), dtype=bool)
+        for i in range(RUNGMAX):
+            if flags & (1 << i):
+                active |= rung == (RUNGMAX - 1 - i)
+                self.min_rung = RUNGMAX - 1 - i
+        prev = self.tsysU
+        self.tsysU = tnextU
+        return active, self.dt_tick * (self.tsysU - prev)
+
+
+def substep_layout(mesh, active: np.ndarray, bnd: np.ndarray):
+    """Reference layout of one substep.
+
+    Returns (perm, nactive, nimport_loc, faces, cell_off, cell_face, cvol, is_ngb)
+    where perm maps local index -> mesh cell: active cells (mesh order) first,
+    then the inactive cells that share a face with an active cell.  Faces are the
+    faces of active cells, oriented so that s1 is active (the creating cell,
+    localMesh.cpp:497-591) with the centroid in s1's frame."""
+    A = np.nonzero(active)[0]
+    n = mesh.ncell
+    starts, ends = mesh.cell_off[A], mesh.cell_off[A + 1]
+    lens = ends - starts
+    idx = np.repeat(starts, lens) + (np.arange(lens.sum()) - np.repeat(np.cumsum(lens) - lens, lens))
+    fl = np.unique(mesh.cell_face[idx])  # faces of active cells, ascending
+    f = mesh.faces[fl]
+    s1 = f["s1"].astype(np.int64)
+    s2 = f["s2"].astype(np.int64)
+    touched = np.unique(np.concatenate([s1, s2]))
+    inact = touched[~active[touched]]
+    perm = np.concatenate([A, inact])
+    g2l = np.full(n, -1, dtype=np.int64)
+    g2l[perm] = np.arange(len(perm))
+    # orient: s1 must be active; swap when only s2 is
+    swap = ~active[s1] & active[s2]
+    c = f["centroid"].copy()
+    nv = f["n"].copy()
+    if np.any(swap):
+        box = np.asarray(mesh.box)
+        p_new = mesh.pos[s2[swap]]
+        d = c[swap] - p_new
+        d -= np.round(d / box) * box
+        c[swap] = p_new + d
+        nv[swap] = -nv[swap]
+    faces = np.empty(len(fl), dtype=mesh.faces.dtype)
+    faces["s1"] = g2l[np.where(swap, s2, s1)]
+    faces["s2"] = g2l[np.where(swap, s1, s2)]
+    faces["centroid"] = c
+    faces["n"] = nv
+    # cell lists of the active cells, local face ids in ascending mesh-face order
+    f2l = np.full(mesh.nface, -1, dtype=np.int64)
+    f2l[fl] = np.arange(len(fl))
+    cell_face = f2l[mesh.cell_face[idx]].astype(np.int32)
+    cell_off = np.zeros(len(A) + 1, dtype=np.int32)
+    cell_off[1:] = np.cumsum(lens)
+    cvol = mesh.volume[A].copy()
+    is_ngb = np.zeros(len(perm), dtype=np.int32)
+    is_ngb[len(A):] = 1
+    return perm, len(A), len(inact), faces, cell_off, cell_face, cvol, is_ngb
+
+
+class Substepper:
+    """Individual-timestep driver for a static periodic mesh (global arrays in mesh order).
+
+    State kept per mesh cell, as the reference keeps it in ptcl_list / mesh_pnts /
+    Wrec_list / U_list / dU_list: W, U, gradients, Wt, min/max, dU (with
+    gradPsi, divB), rung, tbeg, tend, dt_new."""
+
+    def __init__(self, mesh, W, gamma=5.0 / 3.0, courant=0.4, dtmax=1.0 / 64, limiter=0, nrung=6):
+        n = mesh.ncell
+        self.mesh = mesh
+        self.gamma, self.courant, self.limiter, self.nrung = gamma, courant, limiter, nrung
+        self.clock = RungClock(dtmax)
+        self.W = np.array(W, dtype=np.float64)
+        self.U = to_conservative(self.W, mesh.volume)
+        z10 = lambda: np.zeros((n, 10))
+        self.x, self.y, self.z, self.t = z10(), z10(), z10(), z10()
+        self.mm_lo, self.mm_hi = z10(), z10()
+        self.J = np.zeros((n, 3))
+        self.dU, self.dgp, self.ddivB = z10(), np.zeros((n, 3)), np.zeros(n)
+        self.mvol = mesh.volume.copy()
+        self.bnd = np.zeros(n, dtype=np.int32)
+        self.rung = np.zeros(n, dtype=np.int64)
+        self.tbeg = np.zeros(n)
+        self.tend = np.zeros(n)
+        self.dt_new = np.zeros(n)
+        self.t_global = 0.0
+        self.history = []
+
+    # --- initial rungs from the CFL of every cell (System.cpp:255-300), but
+    # per cell instead of the global minimum, clamped to nrung rungs
+    def init_rungs(self, backend):
+        st, perm = self._state(np.ones(self.mesh.ncell, dtype=bool))
+        backend.load(st)
+        backend.compute_timestep(st)
+        backend.fetch(st, ["dt_new"])
+        dtn = np.empty(self.mesh.ncell)
+        dtn[perm] = st.dt_new
+        for i in range(self.mesh.ncell):
+            self.rung[i] = min(self.clock.get_rung(dtn[i]), self.nrung - 1)
+        self.tend = self.tbeg + np.array([self.clock.get_dt(int(r)) for r in self.rung])
+
+    def _state(self, active):
+        perm, nact, nimp, faces, coff, cface, cvol, is_ngb = substep_layout(self.mesh, active, self.bnd)
+        st = CellState(np=len(perm), nactive=nact, nimport_loc=nimp, faces=faces, cell_off=coff,
+                       cell_face=cface, cvol=cvol, t_global=self.t_global, gamma=self.gamma,
+                       courant=self.courant, box=tuple(float(b) for b in self.mesh.box))
+        from .gpu import FIELDS
+        for name, (_, width, kind) in FIELDS.items():
+            st.arrays[name] = np.zeros(len(perm), dtype=np.int32) if kind == "i" else \
+                np.zeros((len(perm), width) if width > 1 else len(perm))
+        st.w = self.W[perm].copy()
+        st.U = self.U[perm].copy()
+        for name in ("x", "y", "z", "t", "mm_lo", "mm_hi", "J", "dU", "dgp", "ddivB", "mvol", "bnd", "tbeg",
+                     "tend", "dt_new"):
+            setattr(st, name, getattr(self, name)[perm].copy())
+        st.pos = self.mesh.pos[perm].copy()
+        st.is_ngb = is_ngb
+        st.is_active = np.zeros(len(perm), dtype=np.int32)
+        st.is_active[:nact] = 1
+        st.lim = np.ones((len(perm), 10))
+        # local inactive cells: their dU is a fresh buffer that is added back
+        # to the owner afterwards (localMesh.cpp:266, computeFluidUpdate.cpp:320-345)
+        st.dU[nact:] = 0.0
+        st.dgp[nact:] = 0.0
+        st.ddivB[nact:] = 0.0
+        return st, perm
+
+    def substep(self, backend):
+        """One substep; returns the number of active cells and active faces."""
+        active, _ = self.clock.pull(self.rung)
+        self.t_global = self.clock.tsys()
+        assert np.all(self.tend[active] == self.t_global)
+        st, perm = self._state(active)
+        nact = st.nactive
+        # predict inactive local cells to the current time (Iterate.cpp:99-106), on the copy
+        dtp = self.t_global - st.tbeg[nact:]
+        st.w[nact:] = st.w[nact:] + st.t[nact:] * dtp[:, None]
+        backend.load(st)
+        nfa = backend.get_active_faces(st)
+        backend.compute_primitives(st)
+        backend.compute_timestep(st)
+        backend.compute_timestep_limiter(st)
+        backend.compute_predictor(st, self.clock.dtmax, self.clock.min_rung)
+        backend.fetch(st, ["dt_new"])
+        # new rungs of the active cells (Iterate.cpp:146-168)
+        for i in range(nact):
+            r = min(self.clock.get_rung(st.dt_new[i]), self.nrung - 1)
+            r = max(r, self.clock.min_rung)
+            self.rung[perm[i]] = r
+            st.tbeg[i] = self.t_global
+            st.tend[i] = self.t_global + self.clock.get_dt(r)
+        backend.put(st, ["tbeg", "tend"])
+        backend.compute_reconstruction(st)
+        backend.slope_limiter(st)
+        backend.compute_fluid_update(st)
+        backend.fetch(st, None)
+        # write back: active cells keep everything; inactive neighbours add their dU
+        A = perm[:nact]
+        for name in ("W", "U", "x", "y", "z", "t", "mm_lo", "mm_hi", "J", "mvol", "tbeg", "tend", "dt_new",
+                     "dU", "dgp", "ddivB"):
+            src = getattr(st, "w" if name == "W" else name)
+            getattr(self, name)[A] = src[:nact]
+        I = perm[nact:]
+        self.dU[I] += st.dU[nact:]
+        self.dgp[I] += st.dgp[nact:]
+        self.ddivB[I] += st.ddivB[nact:]
+        self.history.append((self.t_global, nact, nfa))
+        return nact, nfa
+
+
+class GpuBackend:
+    """The HIP library behind the Substepper (one upload/download per substep)."""
+
+    def __init__(self, gamma, courant, box, limiter=0):
+        from . import gpu as G
+        self.ctx = G.Context(gamma=gamma, courant=courant, box=box, limiter=limiter)
+        self.kernel_s = 0.0
+
+    def load(self, st):
+        self.ctx.upload_state(st)
+
+    def put(self, st, names):
+        for n in names:
+            self.ctx.upload(n, getattr(st, n))
+
+    def fetch(self, st, names):
+        self.ctx.download_state(st, names)
+
+    def get_active_faces(self, st):
+        st.face_act = self.ctx.get_active_faces()
+        return len(st.face_act)
+
+    def _timed(self, fn, *a):
+        import time
+        t = time.perf_counter()
+        r = fn(*a)
+        self.kernel_s += time.perf_counter() - t
+        return r
+
+    def compute_primitives(self, st):
+        self._timed(self.ctx.compute_primitives, False)
+
+    def compute_timestep(self, st):
+        self._timed(self.ctx.compute_timestep)
+
+    def compute_timestep_limiter(self, st):
+        self._timed(self.ctx.compute_timestep_limiter)
+
+    def compute_predictor(self, st, dtmax, min_rung):
+        self._timed(self.ctx.compute_predictor, dtmax, min_rung)
+
+    def compute_reconstruction(self, st):
+        self._timed(self.ctx.compute_reconstruction)
+
+    def slope_limiter(self, st):
+        self._timed(self.ctx.slope_limiter)
+
+    def compute_fluid_update(self, st):
+        self._timed(self.ctx.compute_fluid_update, False)
+
+
+def ic_disk(pos, gamma=5.0 / 3.0, H=0.1, beta=400.0, noise=0.01, seed=7, rsoft=0.05, r_out=0.4):
+    """Keplerian-like disk in the periodic unit cube, centred at (0.5, 0.5, 0.5):
+    rho ~ exp(-z^2/2H^2) (floor 1e-2), v_phi ~ (r^2+rsoft^2)^(-1/4) tapered to zero
+    between 0.75 r_out and r_out so the periodic images stay at rest, vertical B at
+    plasma beta 400 (mid-plane), 1% seeded velocity noise."""
+    from .mesh import DENS, VELX, VELY, VELZ, ETHM, BZ, ENTR, NFLUID, entropy_from_ethm
+    x, y, z = pos[:, 0] - 0.5, pos[:, 1] - 0.5, pos[:, 2] - 0.5
+    r = np.sqrt(x * x + y * y)
+    rho = np.maximum(np.exp(-z * z / (2 * H * H)), 1e-2)
+    p = 0.1 * rho
+    s = np.clip((r_out - r) / (0.25 * r_out), 0.0, 1.0)
+    taper = s * s * (3.0 - 2.0 * s)
+    vphi = 0.3 * (r * r + rsoft * rsoft) ** -0.25 * taper
+    rng = np.random.default_rng(seed)
+    W = np.zeros((len(pos), NFLUID))
+    W[:, DENS] = rho
+    W[:, VELX] = -vphi * y / np.maximum(r, 1e-12)
+    W[:, VELY] = vphi * x / np.maximum(r, 1e-12)
+    W[:, VELX:VELZ + 1] += noise * (0.3 / math.sqrt(rsoft)) * rng.standard_normal((len(pos), 3))
+    W[:, ETHM] = p / (gamma - 1.0)
+    W[:, BZ] = math.sqrt(2.0 * 0.1 / beta)
+    W[:, ENTR] = entropy_from_ethm(W, gamma)
+    return W
