@@ -54,11 +54,12 @@ def build_workload(name):
 
 def k2_bytes(ncell, nfe, prec_bytes):
     """Algorithmic HBM bytes of one flux+apply launch: every array it must touch, once.
-    per cell: the gather record (W 10, Wt 10, gradients 30 in face precision), FP64 W and
-    U (10+10), volume, CSR offset; writes FP64 W, U and next step's W slice (10).
+    per cell: the gather record (W 10, Wt 10, gradients 30 in face precision), FP64 U,
+    volume, CSR offset; writes FP64 U and next step's W slice (10).  The FP64 path also
+    reads and writes the stored FP64 primitives (the FP32 path re-derives them from U).
     per face-end: one 32-byte face-end record (neighbour, area vector, centroid offset)."""
     p = prec_bytes
-    per_cell = 50 * p + 80 + 80 + 8 + 4 + 80 + 80 + 10 * p
+    per_cell = 50 * p + 80 + 8 + 4 + 80 + 10 * p + (160 if p == 8 else 0)
     per_fe = 32 if p == 4 else 80
     return ncell * per_cell + nfe * per_fe
 

@@ -76,8 +76,9 @@ template <> struct alignas(16) FaceEndRec<double> {
 
 template <typename T> struct FusedArgs {
   int n;                    // cells (all active)
-  const int *off;           // [n+1] face-end offsets
-  const FaceEndRec<T> *fe;  // [E]
+  const int *off;           // [n+1] CSR offsets (degree of a cell = off[c+1] - off[c])
+  const int *gbase;         // [n/64] first slot of each 64-cell group in fe
+  const FaceEndRec<T> *fe;  // SELL-64: face-end k of cell c at gbase[c/64] + 64*k + c%64
   const double *cvol;       // [n]
   const double *mvel;       // [3][n] generator velocity (moving mesh only)
   const T *rec;             // [n][REC] records of this step
@@ -172,7 +173,7 @@ template <typename T> __device__ __forceinline__ void nts(T v, T *p) { __builtin
 
 template <typename T>
 __device__ __forceinline__ void load_fe(const FusedArgs<T> &A, int e, FE<T> &g) {
-  const FaceEndRec<T> r = nt_load(A.fe + e);
+  const FaceEndRec<T> r = A.fe[e];
   g.isL = r.nbr >= 0;
   g.j = g.isL ? r.nbr : ~r.nbr;
   const T fx = r.fn[0], fy = r.fn[1], fz = r.fn[2];
@@ -295,8 +296,9 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
     for (int k = 0; k < NF; k++) gx[k] = gy[k] = gz[k] = dmn[k] = dmx[k] = pU[k] = T(0.0);
     pgp[0] = pgp[1] = pgp[2] = pdv = T(0.0);
 
-    const int eb = A.off[c], ee = A.off[c + 1];
-    for (int e = eb; e < ee; e++) {
+    const int deg = A.off[c + 1] - A.off[c];
+    const int e0 = A.gbase[c >> 6] + (c & 63);
+    for (int q = 0, e = e0; q < deg; q++, e += 64) {
       FE<T> fe;
       load_fe(A, e, fe);
       const int j = fe.j;
@@ -410,7 +412,7 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
       if constexpr (LIMIT) {
         T lm[NF];
         for (int k = 0; k < NF; k++) lm[k] = T(1.0);
-        for (int e = eb; e < ee; e++) {
+        for (int q = 0, e = e0; q < deg; q++, e += 64) {
           FE<T> fe;
           load_fe(A, e, fe);
           for (int k = 0; k < NF; k++) {
@@ -471,7 +473,8 @@ __global__ void __launch_bounds__(FB) k_limit(FusedArgs<T> A) {
   T lm[NF];
   for (int k = 0; k < NF; k++) lm[k] = T(1.0);
   const T dtt = (T)(dtI - dth);
-  for (int e = A.off[c]; e < A.off[c + 1]; e++) {
+  const int deg = A.off[c + 1] - A.off[c];
+  for (int q = 0, e = A.gbase[c >> 6] + (c & 63); q < deg; q++, e += 64) {
     FE<T> fe;
     load_fe(A, e, fe);
     const T *dr = fe.isL ? fe.drL : fe.drR;
@@ -520,7 +523,8 @@ __global__ void __launch_bounds__(FB) k_flux_apply(FusedArgs<T> A) {
   gp[0] = gp[1] = gp[2] = 0.0;
   bool bad = false;
 
-  for (int e = A.off[c]; e < A.off[c + 1]; e++) {
+  const int deg = A.off[c + 1] - A.off[c];
+  for (int q = 0, e = A.gbase[c >> 6] + (c & 63); q < deg; q++, e += 64) {
     FE<T> fe;
     load_fe(A, e, fe);
     const int j = fe.j;
@@ -650,11 +654,12 @@ __global__ void k_advance(double *tdev, unsigned long long *dtmin_bits, double d
 template <typename T>
 __global__ void k_build_faceends(int n, const int *coff, const int *cface, const int *s1, const int *s2,
                                  const double *fc, const double *fnv, int nface, const double *pos, double Lx,
-                                 double Ly, double Lz, FaceEndRec<T> *fe) {
+                                 double Ly, double Lz, const int *gbase, FaceEndRec<T> *fe) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const double L[3] = {Lx, Ly, Lz};
   for (int e = coff[c]; e < coff[c + 1]; e++) {
+    const int slot = gbase[c >> 6] + 64 * (e - coff[c]) + (c & 63);
     const int f = cface[e];
     const int a = s1[f], b = s2[f];
     const bool isL = a == c;
@@ -672,7 +677,7 @@ __global__ void k_build_faceends(int n, const int *coff, const int *cface, const
       }
     }
     r.pad = 0;
-    fe[e] = r;
+    fe[slot] = r;
   }
 }
 
@@ -704,7 +709,7 @@ static unsigned fused_grid(int n) {
 
 void fvm_fast_release(fvm_ctx *c) {
   FusedState &z = c->fz;
-  for (DBuf *b : {&z.fe, &z.rec[0], &z.rec[1], &z.mm, &z.dt_new, &z.dtmin, &z.tdev, &z.hist}) fvm_release(*b);
+  for (DBuf *b : {&z.fe, &z.gbase, &z.rec[0], &z.rec[1], &z.mm, &z.dt_new, &z.dtmin, &z.tdev, &z.hist}) fvm_release(*b);
   for (auto &e : z.ev)
     if (e) (void)hipEventDestroy(e);
   z.ev.clear();
@@ -716,6 +721,7 @@ template <typename T> static fvm::FusedArgs<T> fused_args(fvm_ctx *c, double dtm
   fvm::FusedArgs<T> A;
   A.n = c->np;
   A.off = (const int *)c->m_coff.p;
+  A.gbase = (const int *)z.gbase.p;
   A.fe = (const fvm::FaceEndRec<T> *)z.fe.p;
   A.cvol = (const double *)c->m_cvol.p;
   A.mvel = (const double *)c->field[FVM_FIELD_MESH_VEL].p;
@@ -743,7 +749,24 @@ template <typename T> static int fused_setup_t(fvm_ctx *c) {
   z.E = E;
   int rc;
   const size_t es = sizeof(T);
-  if ((rc = fvm_ensure(c, z.fe, sizeof(fvm::FaceEndRec<T>) * (size_t)E)) ||
+  // SELL-64 slots: each 64-cell group gets 64 x (largest degree in the group) records
+  std::vector<int> off(n + 1);
+  FVM_HIPCHK(c, hipMemcpy(off.data(), c->m_coff.p, sizeof(int) * (size_t)(n + 1), hipMemcpyDeviceToHost));
+  const int ng = (n + 63) / 64;
+  std::vector<int> gb(ng + 1, 0);
+  long long acc = 0;
+  for (int g = 0; g < ng; g++) {
+    int mx = 0;
+    for (int i = g * 64; i < std::min(n, g * 64 + 64); i++) mx = std::max(mx, off[i + 1] - off[i]);
+    acc += 64LL * mx;
+    if (acc > 0x7fffffffLL) return fvm_fail(c, FVM_EARG, "fused path: face-end slots exceed 2^31");
+    gb[g + 1] = (int)acc;
+  }
+  const size_t slots = (size_t)gb[ng];
+  if ((rc = fvm_ensure(c, z.gbase, sizeof(int) * (size_t)(ng + 1)))) return rc;
+  FVM_HIPCHK(c, hipMemcpy(z.gbase.p, gb.data(), sizeof(int) * (size_t)(ng + 1), hipMemcpyHostToDevice));
+  z.slots = slots;
+  if ((rc = fvm_ensure(c, z.fe, sizeof(fvm::FaceEndRec<T>) * slots)) ||
       (rc = fvm_ensure(c, z.rec[0], es * fvm::REC * (size_t)n)) || (rc = fvm_ensure(c, z.rec[1], es * fvm::REC * (size_t)n)) ||
       (rc = fvm_ensure(c, z.mm, es * 20 * (size_t)n)) || (rc = fvm_ensure(c, z.dt_new, sizeof(double) * (size_t)n)) ||
       (rc = fvm_ensure(c, z.dtmin, sizeof(unsigned long long))) || (rc = fvm_ensure(c, z.tdev, sizeof(double) * 2)))
@@ -752,7 +775,8 @@ template <typename T> static int fused_setup_t(fvm_ctx *c) {
   hipLaunchKernelGGL(fvm::k_build_faceends<T>, dim3(nb), dim3(256), 0, c->stream, n, (const int *)c->m_coff.p,
                      (const int *)c->m_cface.p, (const int *)c->m_s1.p, (const int *)c->m_s2.p, (const double *)c->m_c.p,
                      (const double *)c->m_n.p, c->nface, (const double *)c->field[FVM_FIELD_POS].p,
-                     c->prm.domain_size[0], c->prm.domain_size[1], c->prm.domain_size[2], (fvm::FaceEndRec<T> *)z.fe.p);
+                     c->prm.domain_size[0], c->prm.domain_size[1], c->prm.domain_size[2], (const int *)z.gbase.p,
+                     (fvm::FaceEndRec<T> *)z.fe.p);
   FVM_HIPCHK(c, hipGetLastError());
   for (int q = 0; q < 2; q++) {
     hipLaunchKernelGGL(fvm::k_rec_from_W<T>, dim3(nb), dim3(256), 0, c->stream, (const double *)c->field[FVM_FIELD_W].p,

@@ -39,7 +39,8 @@ struct FusedState {
   int moving = 0;
   int E = 0;
   int cur = 0;
-  DBuf fe, rec[2], mm, dt_new, dtmin, tdev, hist;
+  DBuf fe, gbase, rec[2], mm, dt_new, dtmin, tdev, hist;
+  size_t slots = 0;
   bool timing = false;
   std::vector<hipEvent_t> ev;  // 5 per timed step
   double ms[4] = {0, 0, 0, 0};
