@@ -1,795 +1,22 @@
-// Device-resident stepping of the Voronoi-face pipeline ("fused" path):
-// static mesh, every cell active, one global timestep (all cells on the
-// smallest rung).  Per step three launches:
-//
-//   K1 k_cfl_predict_recon  one thread per cell, one pass over its face-ends:
-//        computeTimestep (CFL, FP64), computePredictor (first-order HLLD sweep
-//        + FP64 cell update -> Wt), computeReconstruction (gradients, min/max)
-//   K1b k_limit             opt-in limiters only (slopeLimiter + apply)
-//   K2 k_flux_apply         one thread per cell, one pass over its face-ends:
-//        space-time extrapolation + HLLD (computeFluidUpdate), FP64 sums of
-//        flux*dt*A in the cell's face order, then computePrimitives (FP64
-//        pressure recovery, Dedner/Powell terms, fallbacks)
-//   K3 k_advance            t += dt, reset the CFL minimum
-//
-// Each face is evaluated by both of its cells in the same canonical
-// orientation (L = s1, R = s2, normal L->R), so both see bit-identical fluxes:
-// the scatter is conservative, deterministic and needs no atomics.  Face
-// arithmetic runs in T = double (the plain FP64 path, operation-for-operation
-// the reference's) or T = float (FP32 face path); accumulation into conserved
-// variables, pressure recovery and the CFL reduction are FP64 in both.
-//
-// Face-ends are stored per cell in CSR order (the cell's face list order),
-// structure of arrays, so the streams of a cell block are read sequentially;
-// cells are expected in a space-filling-curve order so that neighbour gathers
-// hit L2.  Blocks are mapped XCD-aware: the 8 XCDs each sweep a contiguous
-// eighth of the cells, keeping neighbour reuse inside one XCD's L2.
+// C ABI of the fused stepping (the kernels and their host drivers are in
+// fvm_fused.h, compiled once per face precision) and the stream-copy probe.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cstring>
-#include <string>
-#include <type_traits>
 #include <vector>
 
 #include "../../include/fvm_gpu.h"
 #include "fvm_internal.h"
-#include "fvm_physics.h"
-
-using namespace fvm;
-
-namespace fvm {
-
-constexpr int FB = 256;  // threads per block of the fused kernels
-
-__device__ __forceinline__ int xcd_cell(int nb) {
-  // gridDim.x is padded to a multiple of 8; block b runs on XCD b % 8
-  const int per = (int)gridDim.x >> 3;
-  const int tile = ((int)blockIdx.x & 7) * per + ((int)blockIdx.x >> 3);
-  (void)nb;
-  return tile * FB + (int)threadIdx.x;
-}
-
-// Per-cell gather record: the state a neighbour needs, contiguous, so one
-// gather is a few 16-byte vector loads from two cache lines.
-//   [0,10) W   [10,20) Wt   [20,50) gradients x[10] y[10] z[10]   (stride 64)
-constexpr int REC = 64;
-enum { R_W = 0, R_T = 10, R_G = 20 };
-
-// Per face-end record (CSR order of the cell's face list): the neighbour
-// (j, or ~j when this cell is s2) and the face geometry in canonical
-// orientation s1 -> s2; FP64 also carries the reconstruction centroid of the
-// own side as the reference forms it.
-template <typename T> struct FaceEndRec;
-template <> struct alignas(16) FaceEndRec<float> {
-  int nbr;
-  float fn[3];
-  float drL[3];
-  float pad;
-};
-template <> struct alignas(16) FaceEndRec<double> {
-  int nbr, pad;
-  double fn[3];
-  double drL[3];
-  double cen[3];
-};
-
-template <typename T> struct FusedArgs {
-  int n;                    // cells (all active)
-  const int *off;           // [n+1] CSR offsets (degree of a cell = off[c+1] - off[c])
-  const int *gbase;         // [n/64] first slot of each 64-cell group in fe
-  const FaceEndRec<T> *fe;  // SELL-64: face-end k of cell c at gbase[c/64] + 64*k + c%64
-  const double *cvol;       // [n]
-  const double *mvel;       // [3][n] generator velocity (moving mesh only)
-  const T *rec;             // [n][REC] records of this step
-  T *rec_w;                 // records written by this launch (K1: rec itself; K2: next step's)
-  T *rec_next;
-  double *W64;              // [10][n] FP64 primitive state (own cell only)
-  double *U;                // [10][n] FP64 conserved state (own cell only)
-  T *mm;                    // [20][n] min/max (opt-in limiters)
-  double *dt_new;           // [n]
-  unsigned long long *dtmin_bits;
-  double *tdev;             // [0] t_global, [1] dt of the last step
-  unsigned long long *diag;
-  double gamma, courant, dtmax;
-  int limiter;
-  int write_w64;            // FP32 path: store FP64 W (last step of a run)
-};
-
-#define SF(p, k, i, n) (p)[(size_t)(k) * (size_t)(n) + (size_t)(i)]
-#define FLD(p, k, i, n) SF(p, k, i, n)
-
-// vector loads/stores of record slices (offsets are multiples of 16 bytes)
-template <int N> __device__ __forceinline__ void ldv(const float *__restrict__ p, float *o) {
-  int k = 0;
-#pragma unroll
-  for (; k + 4 <= N; k += 4) {
-    const float4 v = *reinterpret_cast<const float4 *>(p + k);
-    o[k] = v.x; o[k + 1] = v.y; o[k + 2] = v.z; o[k + 3] = v.w;
-  }
-#pragma unroll
-  for (; k < N; k += 2) {
-    const float2 v = *reinterpret_cast<const float2 *>(p + k);
-    o[k] = v.x; o[k + 1] = v.y;
-  }
-}
-template <int N> __device__ __forceinline__ void ldv(const double *__restrict__ p, double *o) {
-#pragma unroll
-  for (int k = 0; k < N; k += 2) {
-    const double2 v = *reinterpret_cast<const double2 *>(p + k);
-    o[k] = v.x; o[k + 1] = v.y;
-  }
-}
-template <int N> __device__ __forceinline__ void stv(float *__restrict__ p, const float *o) {
-  int k = 0;
-#pragma unroll
-  for (; k + 4 <= N; k += 4) *reinterpret_cast<float4 *>(p + k) = make_float4(o[k], o[k + 1], o[k + 2], o[k + 3]);
-#pragma unroll
-  for (; k < N; k += 2) *reinterpret_cast<float2 *>(p + k) = make_float2(o[k], o[k + 1]);
-}
-template <int N> __device__ __forceinline__ void stv(double *__restrict__ p, const double *o) {
-#pragma unroll
-  for (int k = 0; k < N; k += 2) *reinterpret_cast<double2 *>(p + k) = make_double2(o[k], o[k + 1]);
-}
-
-// geometry of one face-end: canonical (s1->s2) normal, area, drL, drR, and
-// this cell's reconstruction centroid and outward normal
-template <typename T> struct FE {
-  int j;
-  bool isL;
-  T area, nrm[3], drL[3], drR[3], cen[3], ncen[3];
-};
-
-// streamed once per launch: non-temporal loads keep them from evicting the
-// gathered cell records from L2
-typedef float v4f __attribute__((ext_vector_type(4)));
-typedef double v2d __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ FaceEndRec<float> nt_load(const FaceEndRec<float> *p) {
-  const v4f *q = reinterpret_cast<const v4f *>(p);
-  const v4f a = __builtin_nontemporal_load(q);
-  const v4f b = __builtin_nontemporal_load(q + 1);
-  FaceEndRec<float> r;
-  r.nbr = __float_as_int(a.x);
-  r.fn[0] = a.y; r.fn[1] = a.z; r.fn[2] = a.w;
-  r.drL[0] = b.x; r.drL[1] = b.y; r.drL[2] = b.z;
-  r.pad = b.w;
-  return r;
-}
-__device__ __forceinline__ FaceEndRec<double> nt_load(const FaceEndRec<double> *p) {
-  const v2d *q = reinterpret_cast<const v2d *>(p);
-  const v2d a = __builtin_nontemporal_load(q), b = __builtin_nontemporal_load(q + 1),
-            cc = __builtin_nontemporal_load(q + 2), d = __builtin_nontemporal_load(q + 3),
-            e = __builtin_nontemporal_load(q + 4);
-  FaceEndRec<double> r;
-  r.nbr = (int)(__double_as_longlong(a.x) & 0xffffffffLL);
-  r.pad = 0;
-  r.fn[0] = a.y; r.fn[1] = b.x; r.fn[2] = b.y;
-  r.drL[0] = cc.x; r.drL[1] = cc.y; r.drL[2] = d.x;
-  r.cen[0] = d.y; r.cen[1] = e.x; r.cen[2] = e.y;
-  return r;
-}
-template <typename T> __device__ __forceinline__ T ntl(const T *p) { return __builtin_nontemporal_load(p); }
-template <typename T> __device__ __forceinline__ void nts(T v, T *p) { __builtin_nontemporal_store(v, p); }
-
-template <typename T>
-__device__ __forceinline__ void load_fe(const FusedArgs<T> &A, int e, FE<T> &g) {
-  const FaceEndRec<T> r = A.fe[e];
-  g.isL = r.nbr >= 0;
-  g.j = g.isL ? r.nbr : ~r.nbr;
-  const T fx = r.fn[0], fy = r.fn[1], fz = r.fn[2];
-  g.area = fsqrt(fx * fx + fy * fy + fz * fz);
-  const T ia = frcp(g.area);
-  g.nrm[0] = fx * ia;
-  g.nrm[1] = fy * ia;
-  g.nrm[2] = fz * ia;
-  for (int d = 0; d < 3; d++) g.drL[d] = r.drL[d];
-  const T dsh = g.drL[0] * g.nrm[0] + g.drL[1] * g.nrm[1] + g.drL[2] * g.nrm[2];
-  const T dsl = T(2.0) * dsh;
-  for (int d = 0; d < 3; d++) g.drR[d] = g.drL[d] - g.nrm[d] * dsl;
-  if constexpr (std::is_same<T, double>::value) {
-    // the reference's reconstruction geometry (computeReconstruction.cpp:97-123)
-    for (int d = 0; d < 3; d++) g.cen[d] = r.cen[d];
-    const T cn = g.cen[0] * fx + g.cen[1] * fy + g.cen[2] * fz;
-    const T sc = (cn < 0.0) ? (-1.0 / g.area) : (1.0 / g.area);
-    g.ncen[0] = fx * sc;
-    g.ncen[1] = fy * sc;
-    g.ncen[2] = fz * sc;
-  } else {
-    for (int d = 0; d < 3; d++) {
-      g.cen[d] = g.isL ? g.drL[d] : g.drR[d];
-      g.ncen[d] = g.isL ? g.nrm[d] : -g.nrm[d];
-    }
-  }
-}
-
-// face velocity with Springel's correction (computeFluidUpdate.cpp:194-207), canonical orientation
-template <typename T, bool MOVING>
-__device__ __forceinline__ void face_velocity(const FusedArgs<T> &A, int L, int R, const FE<T> &g, T *wij,
-                                              T &dV) {
-  if constexpr (!MOVING) {
-    wij[0] = wij[1] = wij[2] = T(0.0);
-    dV = T(0.0);
-  } else {
-    T vi[3], vj[3];
-    for (int d = 0; d < 3; d++) {
-      vi[d] = (T)SF(A.mvel, d, L, A.n);
-      vj[d] = (T)SF(A.mvel, d, R, A.n);
-    }
-    const T dsh = g.drL[0] * g.nrm[0] + g.drL[1] * g.nrm[1] + g.drL[2] * g.nrm[2];
-    const T dsl = T(2.0) * dsh;
-    T fij[3], rij[3];
-    for (int d = 0; d < 3; d++) {
-      fij[d] = g.drL[d] - g.nrm[d] * dsh;
-      rij[d] = g.nrm[d] * dsl;
-    }
-    const T fdv = (vj[0] - vi[0]) * fij[0] + (vj[1] - vi[1]) * fij[1] + (vj[2] - vi[2]) * fij[2];
-    const T r = frcp(sqr(dsl));
-    for (int d = 0; d < 3; d++) wij[d] = (vj[d] + vi[d]) * T(0.5) - rij[d] * fdv * r;
-    dV = fdiv(g.area * (rij[0] * wij[0] + rij[1] * wij[1] + rij[2] * wij[2]), dsl);
-  }
-}
-
-__device__ __forceinline__ int exp_of_d(double dt) {
-  return (int)(((unsigned long long)__double_as_longlong(dt) >> 52) & 0xfff);
-}
-// Scheduler::get_dt(get_rung(dt)) with min_rung = 0 (Scheduler.h:94-111)
-__device__ __forceinline__ double rung_dt0(double dtmax, double dt) {
-  int rung = exp_of_d(dtmax) - exp_of_d(dt);
-  if (rung < 0) rung = 0;
-  const double dt_tick = dtmax / (double)(1ULL << 31);
-  return dt_tick * (double)(1ULL << (31 - rung));
-}
-
-__device__ __forceinline__ void wave_count(unsigned long long *ctr, bool pred) {
-  const unsigned long long m = __ballot(pred);
-  if (m && __lane_id() == (__ffsll((long long)m) - 1)) atomicAdd(ctr, (unsigned long long)__popcll(m));
-}
-
-__device__ __forceinline__ double ethm_from_entropy_d(double g, const double *W) {
-  return W[ENTR] * pow(W[DENS], g) / (g - 1.0);
-}
-__device__ __forceinline__ double entropy_from_ethm_d(double g, const double *W) {
-  return (g - 1.0) * W[ETHM] / pow(W[DENS], g);
-}
-
-// ---------------------------------------------------------------------------
-// K1: computeTimestep + computePredictor + computeReconstruction
-// ---------------------------------------------------------------------------
-template <typename T, bool MOVING, bool LIMIT>
-__global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
-  const int c = xcd_cell(A.n);
-  const bool live = c < A.n;
-  double dtn = REF_HUGE;
-  if (live) {
-    const int n = A.n;
-    const double g64 = A.gamma;
-    const T g = (T)A.gamma;
-    const double V = ntl(A.cvol + c);
-    const T *rc = A.rec + (size_t)c * REC;
-    T Wc[NF];
-    ldv<NF>(rc + R_W, Wc);
-
-    // CFL (computeTimestep.cpp:13-31), FP64 from U
-    double v0[3], csi, h;
-    {
-      double U0[NF], W0[NF];
-      for (int k = 0; k < NF; k++) U0[k] = ntl(&SF(A.U, k, c, n));
-      to_primitive(U0, V, W0);
-      h = pow(V / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
-      const double Pi = pressure(g64, W0[ETHM]) * g64;
-      const double B2i = W0[BX] * W0[BX] + W0[BY] * W0[BY] + W0[BZ] * W0[BZ];
-      csi = dsqrt((Pi + B2i) / W0[DENS]);
-      v0[0] = W0[VELX];
-      v0[1] = W0[VELY];
-      v0[2] = W0[VELZ];
-    }
-    double pv[3] = {0.0, 0.0, 0.0};
-    if constexpr (MOVING)
-      for (int d = 0; d < 3; d++) pv[d] = SF(A.mvel, d, c, n);
-    const double dvr[3] = {v0[0] - pv[0], v0[1] - pv[1], v0[2] - pv[2]};
-    const double vrel = dsqrt(dvr[0] * dvr[0] + dvr[1] * dvr[1] + dvr[2] * dvr[2]);
-    double dti = h / (csi + vrel + 1.0 / REF_HUGE);
-    double arg_max = -1.0;  // min_j h/(cs + sqrt(a_j)) = h/(cs + sqrt(max_j a_j)), monotone roundings
-
-    T gx[NF], gy[NF], gz[NF], dmn[NF], dmx[NF], pU[NF], pgp[3], pdv;
-    double pdV = 0.0;
-    for (int k = 0; k < NF; k++) gx[k] = gy[k] = gz[k] = dmn[k] = dmx[k] = pU[k] = T(0.0);
-    pgp[0] = pgp[1] = pgp[2] = pdv = T(0.0);
-
-    const int deg = A.off[c + 1] - A.off[c];
-    const int e0 = A.gbase[c >> 6] + (c & 63);
-    for (int q = 0, e = e0; q < deg; q++, e += 64) {
-      FE<T> fe;
-      load_fe(A, e, fe);
-      const int j = fe.j;
-      T Wj[NF];
-      ldv<NF>(A.rec + (size_t)j * REC + R_W, Wj);
-
-      // --- reconstruction sums (computeReconstruction.cpp:108-149)
-      {
-        const T dsh = fe.cen[0] * fe.ncen[0] + fe.cen[1] * fe.ncen[1] + fe.cen[2] * fe.ncen[2];
-        const T ids = (dsh != T(0.0)) ? fdiv(T(0.5), dsh) : T(0.0);
-        const T idsA = fe.area * ids;
-        T drh[3], fij[3];
-        for (int d = 0; d < 3; d++) {
-          drh[d] = fe.ncen[d] * dsh;
-          fij[d] = fe.cen[d] - drh[d];
-        }
-        for (int k = 0; k < NF; k++) {
-          const T sum = Wj[k] + Wc[k];
-          const T diff = Wj[k] - Wc[k];
-          if constexpr (LIMIT) {
-            dmn[k] = smin(dmn[k], diff);
-            dmx[k] = smax(dmx[k], diff);
-          }
-          gx[k] += (sum * drh[0] + diff * fij[0]) * idsA;
-          gy[k] += (sum * drh[1] + diff * fij[1]) * idsA;
-          gz[k] += (sum * drh[2] + diff * fij[2]) * idsA;
-        }
-      }
-      // --- CFL neighbour term (computeTimestep.cpp:37-54)
-      {
-        double dwi[3] = {0.0, 0.0, 0.0};
-        if constexpr (MOVING)
-          for (int d = 0; d < 3; d++) dwi[d] = (double)(T)SF(A.mvel, d, j, n) - pv[d];
-        const double dvi[3] = {(double)Wj[VELX] - v0[0], (double)Wj[VELY] - v0[1], (double)Wj[VELZ] - v0[2]};
-        const double a = (dwi[0] * dwi[0] + dwi[1] * dwi[1] + dwi[2] * dwi[2]) +
-                         1.0 * (dvi[0] * dvi[0] + dvi[1] * dvi[1] + dvi[2] * dvi[2]) + 1.0 / REF_HUGE;
-        arg_max = (arg_max < a) ? a : arg_max;
-      }
-      // --- predictor face flux, first order (computePredictor.cpp:44-136)
-      {
-        const int L = fe.isL ? c : j, R = fe.isL ? j : c;
-        T WL[NF], WR[NF];
-        for (int k = 0; k < NF; k++) {
-          WL[k] = fe.isL ? Wc[k] : Wj[k];
-          WR[k] = fe.isL ? Wj[k] : Wc[k];
-        }
-        T wij[3], dV;
-        face_velocity<T, MOVING>(A, L, R, fe, wij, dV);
-        T psi, Bn, flux[NF];
-        compute_flux<T>(g, wij, WL, WR, NO_BOUNDARY, fe.nrm, psi, Bn, flux);
-        if (fe.isL) {
-          for (int k = 0; k < NF; k++) pU[k] = pU[k] - flux[k] * fe.area;
-          for (int d = 0; d < 3; d++) pgp[d] = pgp[d] + fe.nrm[d] * (psi * fe.area);
-          pdv = pdv + Bn * fe.area;
-          pdV += (double)dV;
-        } else {
-          pdV -= (double)dV;
-          for (int k = 0; k < NF; k++) pU[k] = pU[k] + flux[k] * fe.area;
-          for (int d = 0; d < 3; d++) pgp[d] = pgp[d] - fe.nrm[d] * (psi * fe.area);
-          pdv = pdv - Bn * fe.area;
-        }
-      }
-    }
-    if (arg_max >= 0.0) dti = smin(dti, h / (csi + dsqrt(arg_max)));
-    dti = smin(dti, REF_HUGE);
-    dtn = dti * A.courant;
-
-    // --- predictor cell update (computePredictor.cpp:141-379), FP64
-    T tv[NF];
-    {
-      double U0[NF], W0[NF];
-      for (int k = 0; k < NF; k++) U0[k] = SF(A.U, k, c, n);
-      to_primitive(U0, V, W0);
-      while (U0[MASS] + (double)pU[MASS] * dtn <= 0.0) dtn *= 0.5;
-      const double dt = rung_dt0(A.dtmax, dtn) / 128;
-      const double divB = (double)pdv;
-      double Uc[NF];
-      for (int k = 0; k < NF; k++) Uc[k] = U0[k] + (double)pU[k] * dt;
-      Uc[WBX] -= W0[VELX] * divB * dt;
-      Uc[WBY] -= W0[VELY] * divB * dt;
-      Uc[WBZ] -= W0[VELZ] * divB * dt;
-      const double pres = pressure(g64, W0[ETHM]);
-      const double B2 = W0[BX] * W0[BX] + W0[BY] * W0[BY] + W0[BZ] * W0[BZ];
-      const double dcs2 = g64 * pres + B2;
-      const double ch_sig = dsqrt(dcs2 / W0[DENS]);
-      Uc[WBX] -= (double)pgp[0] * dt;
-      Uc[WBY] -= (double)pgp[1] * dt;
-      Uc[WBZ] -= (double)pgp[2] * dt;
-      Uc[MPSI] -= dcs2 * divB * dt;
-      const double cr = 0.5;
-      const double l_min = pow(V / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
-      Uc[MPSI] *= exp(-cr * ch_sig / l_min * dt);
-      double W1[NF];
-      to_primitive(Uc, V * exp(pdV * dt / V), W1);
-      const bool ef = W1[ETHM] <= 0.0;
-      if (ef) W1[ETHM] = ethm_from_entropy_d(g64, W1);
-      else W1[ENTR] = entropy_from_ethm_d(g64, W1);
-      wave_count(&A.diag[FVM_DIAG_ETHM_FAILED], ef);
-      const double idt = 1.0 / dt;
-      for (int k = 0; k < NF; k++) tv[k] = (T)((W1[k] - W0[k]) * idt);
-    }
-
-    // --- reconstruction finish (computeReconstruction.cpp:162-239)
-    {
-      const double invV = 1.0 / V;
-      for (int k = 0; k < NF; k++) {
-        gx[k] = (T)((double)gx[k] * invV);
-        gy[k] = (T)((double)gy[k] * invV);
-        gz[k] = (T)((double)gz[k] * invV);
-      }
-      if constexpr (LIMIT) {
-        T lm[NF];
-        for (int k = 0; k < NF; k++) lm[k] = T(1.0);
-        for (int q = 0, e = e0; q < deg; q++, e += 64) {
-          FE<T> fe;
-          load_fe(A, e, fe);
-          for (int k = 0; k < NF; k++) {
-            const T dWx = gx[k] * fe.cen[0] + gy[k] * fe.cen[1] + gz[k] * fe.cen[2];
-            lm[k] = smin(lm[k], limiter_rec_t<T>(A.limiter, dWx, dmn[k], dmx[k]));
-          }
-        }
-        for (int k = 0; k < NF; k++) {
-          SF(A.mm, k, c, n) = dmn[k] + Wc[k];
-          SF(A.mm, NF + k, c, n) = dmx[k] + Wc[k];
-          tv[k] = tv[k] * lm[k];
-          gx[k] = gx[k] * lm[k];
-          gy[k] = gy[k] * lm[k];
-          gz[k] = gz[k] * lm[k];
-        }
-      }
-      T *wc = A.rec_w + (size_t)c * REC;
-      stv<NF>(wc + R_T, tv);
-      stv<NF>(wc + R_G, gx);
-      stv<NF>(wc + R_G + NF, gy);
-      stv<NF>(wc + R_G + 2 * NF, gz);
-    }
-    A.dt_new[c] = dtn;
-  }
-  // global minimum of dt_new (dt > 0: IEEE bit patterns order like the values)
-  double v = dtn;
-  for (int o = 32; o > 0; o >>= 1) v = smin(v, __shfl_xor(v, o));
-  if (__lane_id() == 0 && v < REF_HUGE && v >= 0.0)
-    atomicMin(A.dtmin_bits, (unsigned long long)__double_as_longlong(v));
-}
-
-// ---------------------------------------------------------------------------
-// K1b (opt-in limiters): slopeLimiter compute part + computeFluidUpdateI for
-// the global step, per cell over its own face-ends (own data only)
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(FB) k_limit(FusedArgs<T> A) {
-  const int c = xcd_cell(A.n);
-  if (c >= A.n) return;
-  const int n = A.n;
-  const double t0 = A.tdev[0];
-  const double dt_glob = rung_dt0(A.dtmax, __longlong_as_double((long long)*A.dtmin_bits));
-  const double tend = t0 + dt_glob;
-  const double dtI = tend - t0;
-  const double dth = 0.5 * smin(dtI, dtI);
-  T *rc = A.rec_w + (size_t)c * REC;
-  T W[NF], tv[NF], gg[3 * NF];
-  ldv<NF>(rc + R_W, W);
-  ldv<NF>(rc + R_T, tv);
-  ldv<3 * NF>(rc + R_G, gg);
-  T lo[NF], hi[NF];
-  for (int k = 0; k < NF; k++) {
-    lo[k] = SF(A.mm, k, c, n) - W[k];
-    hi[k] = SF(A.mm, NF + k, c, n) - W[k];
-    lo[k] = smin(lo[k], T(0.0));
-    hi[k] = smax(hi[k], T(0.0));
-  }
-  T lm[NF];
-  for (int k = 0; k < NF; k++) lm[k] = T(1.0);
-  const T dtt = (T)(dtI - dth);
-  const int deg = A.off[c + 1] - A.off[c];
-  for (int q = 0, e = A.gbase[c >> 6] + (c & 63); q < deg; q++, e += 64) {
-    FE<T> fe;
-    load_fe(A, e, fe);
-    const T *dr = fe.isL ? fe.drL : fe.drR;
-    for (int k = 0; k < NF; k++) {
-      const T dW = gg[k] * dr[0] + gg[NF + k] * dr[1] + gg[2 * NF + k] * dr[2] + tv[k] * dtt;
-      lm[k] = smin(lm[k], limiter_sl_t<T>(A.limiter, dW, lo[k], hi[k]));
-    }
-  }
-  for (int k = 0; k < NF; k++) {
-    tv[k] *= lm[k];
-    gg[k] *= lm[k];
-    gg[NF + k] *= lm[k];
-    gg[2 * NF + k] *= lm[k];
-  }
-  stv<NF>(rc + R_T, tv);
-  stv<3 * NF>(rc + R_G, gg);
-}
-
-// ---------------------------------------------------------------------------
-// K2: computeFluidUpdate (extrapolation + HLLD + FP64 scatter sums) and
-// computePrimitives (FP64) for the own cell
-// ---------------------------------------------------------------------------
-template <typename T, bool MOVING>
-__global__ void __launch_bounds__(FB) k_flux_apply(FusedArgs<T> A) {
-  const int c = xcd_cell(A.n);
-  if (c >= A.n) return;
-  const int n = A.n;
-  const T g = (T)A.gamma;
-  const double g64 = A.gamma;
-  const double t0 = A.tdev[0];
-  const double dt_glob = rung_dt0(A.dtmax, __longlong_as_double((long long)*A.dtmin_bits));
-  const double tend = t0 + dt_glob;
-  // every cell has tend = t0 + dt: dtI = dtJ, dt = min(dtI, dtJ), dti = dtI - dt/2
-  const double dtI = tend - t0;
-  const double dtf = smin(dtI, dtI);
-  const T dti = (T)(dtI - 0.5 * dtf);
-
-  const T *rc = A.rec + (size_t)c * REC;
-  T Wc[NF], tc[NF], gc[3 * NF];
-  ldv<NF>(rc + R_W, Wc);
-  ldv<NF>(rc + R_T, tc);
-  ldv<3 * NF>(rc + R_G, gc);
-
-  double dU[NF], gp[3], dv = 0.0;
-  for (int k = 0; k < NF; k++) dU[k] = 0.0;
-  gp[0] = gp[1] = gp[2] = 0.0;
-  bool bad = false;
-
-  const int deg = A.off[c + 1] - A.off[c];
-  for (int q = 0, e = A.gbase[c >> 6] + (c & 63); q < deg; q++, e += 64) {
-    FE<T> fe;
-    load_fe(A, e, fe);
-    const int j = fe.j;
-    // neighbour face state, extrapolated as soon as it is loaded
-    T Wn[NF];
-    {
-      const T *rj = A.rec + (size_t)j * REC;
-      T Wj[NF], tj[NF], gj[3 * NF];
-      ldv<NF>(rj + R_W, Wj);
-      ldv<NF>(rj + R_T, tj);
-      ldv<3 * NF>(rj + R_G, gj);
-      const T *dr = fe.isL ? fe.drR : fe.drL;
-      for (int k = 0; k < NF; k++) Wn[k] = Wj[k] + (gj[k] * dr[0] + gj[NF + k] * dr[1] + gj[2 * NF + k] * dr[2] + tj[k] * dti);
-    }
-    T Wo[NF];
-    {
-      const T *dr = fe.isL ? fe.drL : fe.drR;
-      for (int k = 0; k < NF; k++) Wo[k] = Wc[k] + (gc[k] * dr[0] + gc[NF + k] * dr[1] + gc[2 * NF + k] * dr[2] + tc[k] * dti);
-    }
-    T WL[NF], WR[NF];
-    for (int k = 0; k < NF; k++) {
-      WL[k] = fe.isL ? Wo[k] : Wn[k];
-      WR[k] = fe.isL ? Wn[k] : Wo[k];
-    }
-    bad = bad || !(WL[DENS] > T(0.0)) || !(WL[ETHM] > T(0.0)) || !(WR[DENS] > T(0.0)) || !(WR[ETHM] > T(0.0));
-    T wij[3], dV;
-    face_velocity<T, MOVING>(A, fe.isL ? c : j, fe.isL ? j : c, fe, wij, dV);
-    T psi, Bn, flux[NF];
-    compute_flux<T>(g, wij, WL, WR, NO_BOUNDARY, fe.nrm, psi, Bn, flux);
-    const double tarea = dtf * (double)fe.area;
-    if (fe.isL) {
-      for (int k = 0; k < NF; k++) dU[k] = dU[k] - (double)flux[k] * tarea;
-      for (int d = 0; d < 3; d++) gp[d] = gp[d] + (double)fe.nrm[d] * ((double)psi * tarea);
-      dv = dv + (double)Bn * tarea;
-    } else {
-      for (int k = 0; k < NF; k++) dU[k] = dU[k] + (double)flux[k] * tarea;
-      for (int d = 0; d < 3; d++) gp[d] = gp[d] - (double)fe.nrm[d] * ((double)psi * tarea);
-      dv = dv - (double)Bn * tarea;
-    }
-  }
-  wave_count(&A.diag[FVM_DIAG_POSITIVITY], bad);
-
-  // computePrimitives at t1 = t0 + dt (computeFluidUpdate.cpp:367-505)
-  const double t1 = t0 + dt_glob;
-  const double dt = t1 - t0;
-  const double V = ntl(A.cvol + c);
-  double Ui[NF], wi[NF], Uc[NF], W1[NF];
-  for (int k = 0; k < NF; k++) Ui[k] = ntl(&SF(A.U, k, c, n));
-  if constexpr (sizeof(T) == 8) {
-    // the FP64 path keeps the reference's stored primitives
-    for (int k = 0; k < NF; k++) wi[k] = ntl(&SF(A.W64, k, c, n));
-  } else {
-    // the FP32 face path re-derives them from U (they differ by rounding only)
-    to_primitive(Ui, V, wi);
-  }
-  for (int k = 0; k < NF; k++) Uc[k] = Ui[k] + (0.0 + dU[k]);
-  const double divB = 0.0 + dv;
-  const bool failed = Uc[MASS] <= 0.0;
-  bool ef = false;
-  if (!failed) {
-    const double dth = 0.5 * dt;
-    double Wh[NF];
-    for (int k = 0; k < NF; k++) Wh[k] = wi[k] + (double)tc[k] * dth;
-    Uc[WBX] -= Wh[VELX] * divB;
-    Uc[WBY] -= Wh[VELY] * divB;
-    Uc[WBZ] -= Wh[VELZ] * divB;
-    const double pres = pressure(g64, Wh[ETHM]);
-    const double B2 = Wh[BX] * Wh[BX] + Wh[BY] * Wh[BY] + Wh[BZ] * Wh[BZ];
-    const double dcs2 = g64 * pres + B2;
-    const double ch_sig = dsqrt(dcs2 / Wh[DENS]);
-    Uc[WBX] -= 0.0 + gp[0];
-    Uc[WBY] -= 0.0 + gp[1];
-    Uc[WBZ] -= 0.0 + gp[2];
-    Uc[MPSI] -= dcs2 * divB;
-    // gravity terms with acc0 = acc1 = 0 (computeFluidUpdate.cpp:433-447)
-    {
-      const double M0 = Ui[MASS], M1 = Uc[MASS];
-      double dmom[3];
-      for (int d = 0; d < 3; d++) dmom[d] = ((0.0 * M0 + 0.0 * M1) * 0.5) * dt;
-      Uc[MOMX] += dmom[0];
-      Uc[MOMY] += dmom[1];
-      Uc[MOMZ] += dmom[2];
-      Uc[ENER] += ((Ui[MOMX] * 0.0 + Ui[MOMY] * 0.0 + Ui[MOMZ] * 0.0) + (Uc[MOMX] * 0.0 + Uc[MOMY] * 0.0 + Uc[MOMZ] * 0.0)) *
-                  dt * 0.5;
-    }
-    const double cr = 0.5;
-    const double l_min = pow(V / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
-    Uc[MPSI] *= exp(-cr * ch_sig / l_min * dt);
-    to_primitive(Uc, V, W1);
-    ef = W1[ETHM] <= 0.0;
-    if (ef) W1[ETHM] = ethm_from_entropy_d(g64, W1);
-    else W1[ENTR] = entropy_from_ethm_d(g64, W1);
-  } else {
-    double Ut[NF];
-    for (int k = 0; k < NF; k++) W1[k] = wi[k] + (double)tc[k] * dt;
-    to_conservative(W1, V, Ut);
-    to_primitive(Ut, V, W1);
-    if (W1[ETHM] <= 0.0) W1[ETHM] = ethm_from_entropy_d(g64, W1);
-    else W1[ENTR] = entropy_from_ethm_d(g64, W1);
-  }
-  wave_count(&A.diag[FVM_DIAG_ETHM_FAILED], ef);
-  wave_count(&A.diag[FVM_DIAG_MASS_FAILED], failed);
-  to_conservative(W1, V, Ui);
-  T Wn[NF];
-  for (int k = 0; k < NF; k++) {
-    if (sizeof(T) == 8 || A.write_w64) nts(W1[k], &SF(A.W64, k, c, n));
-    nts(Ui[k], &SF(A.U, k, c, n));
-    Wn[k] = (T)W1[k];
-  }
-  stv<NF>(A.rec_next + (size_t)c * REC + R_W, Wn);
-}
-
-__global__ void k_advance(double *tdev, unsigned long long *dtmin_bits, double dtmax, double *hist, int step) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    const double dt = rung_dt0(dtmax, __longlong_as_double((long long)*dtmin_bits));
-    const double t0 = tdev[0];
-    tdev[0] = t0 + dt;
-    tdev[1] = dt;
-    if (hist) hist[step] = dt;
-    *dtmin_bits = 0x7FF0000000000000ULL;
-  }
-}
-
-// face-end records from the uploaded mesh: cell c's list cell_face[off[c]..off[c+1]);
-// drL = centroid - pos[s1] and the reconstruction centroid wrap(centroid - pos[c])
-// are formed in FP64 exactly as the reference forms them
-template <typename T>
-__global__ void k_build_faceends(int n, const int *coff, const int *cface, const int *s1, const int *s2,
-                                 const double *fc, const double *fnv, int nface, const double *pos, double Lx,
-                                 double Ly, double Lz, const int *gbase, FaceEndRec<T> *fe) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  const double L[3] = {Lx, Ly, Lz};
-  for (int e = coff[c]; e < coff[c + 1]; e++) {
-    const int slot = gbase[c >> 6] + 64 * (e - coff[c]) + (c & 63);
-    const int f = cface[e];
-    const int a = s1[f], b = s2[f];
-    const bool isL = a == c;
-    FaceEndRec<T> r;
-    r.nbr = isL ? b : ~a;
-    for (int d = 0; d < 3; d++) {
-      const double cf = FLD(fc, d, f, nface);
-      r.fn[d] = (T)FLD(fnv, d, f, nface);
-      r.drL[d] = (T)(cf - FLD(pos, d, a, n));
-      if constexpr (std::is_same<T, double>::value) {
-        double v = cf - FLD(pos, d, c, n);
-        if (v > 0.5 * L[d]) v -= L[d];
-        else if (v < -0.5 * L[d]) v += L[d];
-        r.cen[d] = v;
-      }
-    }
-    r.pad = 0;
-    fe[slot] = r;
-  }
-}
-
-// W field (SoA FP64) -> W slice of the records; records -> fields
-template <typename T> __global__ void k_rec_from_W(const double *W, T *rec, int n) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  T *r = rec + (size_t)c * REC;
-  for (int k = 0; k < REC; k++) r[k] = T(0.0);
-  for (int k = 0; k < NF; k++) r[R_W + k] = (T)SF(W, k, c, n);
-}
-template <typename T, typename S> __global__ void k_rec_to_field(const T *rec, int off, S *dst, int n) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  for (int k = 0; k < NF; k++) SF(dst, k, c, n) = (S)rec[(size_t)c * REC + off + k];
-}
-
-}  // namespace fvm
-
-// ===========================================================================
-// host side
-// ===========================================================================
-using fvm::FB;
-
-static unsigned fused_grid(int n) {
-  unsigned nb = (unsigned)((n + FB - 1) / FB);
-  return (nb + 7u) & ~7u;  // multiple of 8 for the XCD mapping
-}
 
 void fvm_fast_release(fvm_ctx *c) {
   FusedState &z = c->fz;
-  for (DBuf *b : {&z.fe, &z.gbase, &z.rec[0], &z.rec[1], &z.mm, &z.dt_new, &z.dtmin, &z.tdev, &z.hist}) fvm_release(*b);
+  for (DBuf *b : {&z.perm, &z.inv, &z.deg, &z.gbase, &z.fe, &z.U, &z.W64, &z.cvol, &z.mvel, &z.rec[0], &z.rec[1], &z.mm,
+                  &z.dt_new, &z.dtmin, &z.tdev, &z.hist})
+    fvm_release(*b);
   for (auto &e : z.ev)
     if (e) (void)hipEventDestroy(e);
   z.ev.clear();
   z.ready = false;
-}
-
-template <typename T> static fvm::FusedArgs<T> fused_args(fvm_ctx *c, double dtmax) {
-  FusedState &z = c->fz;
-  fvm::FusedArgs<T> A;
-  A.n = c->np;
-  A.off = (const int *)c->m_coff.p;
-  A.gbase = (const int *)z.gbase.p;
-  A.fe = (const fvm::FaceEndRec<T> *)z.fe.p;
-  A.cvol = (const double *)c->m_cvol.p;
-  A.mvel = (const double *)c->field[FVM_FIELD_MESH_VEL].p;
-  A.rec = (const T *)z.rec[z.cur].p;
-  A.rec_w = (T *)z.rec[z.cur].p;
-  A.rec_next = (T *)z.rec[z.cur ^ 1].p;
-  A.W64 = (double *)c->field[FVM_FIELD_W].p;
-  A.U = (double *)c->field[FVM_FIELD_U].p;
-  A.mm = (T *)z.mm.p;
-  A.dt_new = (double *)z.dt_new.p;
-  A.dtmin_bits = (unsigned long long *)z.dtmin.p;
-  A.tdev = (double *)z.tdev.p;
-  A.diag = (unsigned long long *)c->diag.p;
-  A.gamma = c->prm.gamma_gas;
-  A.courant = c->prm.courant_no;
-  A.dtmax = dtmax;
-  A.limiter = c->prm.limiter_mode;
-  A.write_w64 = 0;
-  return A;
-}
-
-template <typename T> static int fused_setup_t(fvm_ctx *c) {
-  FusedState &z = c->fz;
-  const int n = c->np, E = c->nref;
-  z.E = E;
-  int rc;
-  const size_t es = sizeof(T);
-  // SELL-64 slots: each 64-cell group gets 64 x (largest degree in the group) records
-  std::vector<int> off(n + 1);
-  FVM_HIPCHK(c, hipMemcpy(off.data(), c->m_coff.p, sizeof(int) * (size_t)(n + 1), hipMemcpyDeviceToHost));
-  const int ng = (n + 63) / 64;
-  std::vector<int> gb(ng + 1, 0);
-  long long acc = 0;
-  for (int g = 0; g < ng; g++) {
-    int mx = 0;
-    for (int i = g * 64; i < std::min(n, g * 64 + 64); i++) mx = std::max(mx, off[i + 1] - off[i]);
-    acc += 64LL * mx;
-    if (acc > 0x7fffffffLL) return fvm_fail(c, FVM_EARG, "fused path: face-end slots exceed 2^31");
-    gb[g + 1] = (int)acc;
-  }
-  const size_t slots = (size_t)gb[ng];
-  if ((rc = fvm_ensure(c, z.gbase, sizeof(int) * (size_t)(ng + 1)))) return rc;
-  FVM_HIPCHK(c, hipMemcpy(z.gbase.p, gb.data(), sizeof(int) * (size_t)(ng + 1), hipMemcpyHostToDevice));
-  z.slots = slots;
-  if ((rc = fvm_ensure(c, z.fe, sizeof(fvm::FaceEndRec<T>) * slots)) ||
-      (rc = fvm_ensure(c, z.rec[0], es * fvm::REC * (size_t)n)) || (rc = fvm_ensure(c, z.rec[1], es * fvm::REC * (size_t)n)) ||
-      (rc = fvm_ensure(c, z.mm, es * 20 * (size_t)n)) || (rc = fvm_ensure(c, z.dt_new, sizeof(double) * (size_t)n)) ||
-      (rc = fvm_ensure(c, z.dtmin, sizeof(unsigned long long))) || (rc = fvm_ensure(c, z.tdev, sizeof(double) * 2)))
-    return rc;
-  const unsigned nb = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(fvm::k_build_faceends<T>, dim3(nb), dim3(256), 0, c->stream, n, (const int *)c->m_coff.p,
-                     (const int *)c->m_cface.p, (const int *)c->m_s1.p, (const int *)c->m_s2.p, (const double *)c->m_c.p,
-                     (const double *)c->m_n.p, c->nface, (const double *)c->field[FVM_FIELD_POS].p,
-                     c->prm.domain_size[0], c->prm.domain_size[1], c->prm.domain_size[2], (const int *)z.gbase.p,
-                     (fvm::FaceEndRec<T> *)z.fe.p);
-  FVM_HIPCHK(c, hipGetLastError());
-  for (int q = 0; q < 2; q++) {
-    hipLaunchKernelGGL(fvm::k_rec_from_W<T>, dim3(nb), dim3(256), 0, c->stream, (const double *)c->field[FVM_FIELD_W].p,
-                       (T *)z.rec[q].p, n);
-    FVM_HIPCHK(c, hipGetLastError());
-  }
-  const unsigned long long inf = 0x7FF0000000000000ULL;
-  const double t2[2] = {c->t_global, 0.0};
-  FVM_HIPCHK(c, hipMemcpyAsync(z.dtmin.p, &inf, sizeof inf, hipMemcpyHostToDevice, c->stream));
-  FVM_HIPCHK(c, hipMemcpyAsync(z.tdev.p, t2, sizeof t2, hipMemcpyHostToDevice, c->stream));
-  FVM_HIPCHK(c, hipStreamSynchronize(c->stream));
-  z.cur = 0;
-  return FVM_OK;
 }
 
 int fvm_fused_setup(fvm_ctx *c, int precision, int moving_mesh) {
@@ -810,55 +37,14 @@ int fvm_fused_setup(fvm_ctx *c, int precision, int moving_mesh) {
   fvm_fast_release(c);
   c->fz.precision = precision;
   c->fz.moving = moving_mesh ? 1 : 0;
-  const int rc = precision == FVM_FUSED_FP64 ? fused_setup_t<double>(c) : fused_setup_t<float>(c);
+  int rc;
+  try {
+    rc = precision == FVM_FUSED_FP64 ? fvm_fused_setup_f64(c) : fvm_fused_setup_f32(c);
+  } catch (...) {
+    return fvm_fail(c, FVM_EARG, "host allocation failed");
+  }
   if (rc) return rc;
   c->fz.ready = true;
-  return FVM_OK;
-}
-
-template <typename T, bool MOVING> static int fused_run_t(fvm_ctx *c, int nsteps, double dtmax) {
-  FusedState &z = c->fz;
-  const unsigned grid = fused_grid(c->np);
-  const bool lim = c->prm.limiter_mode != FVM_LIMITER_SHIPPED;
-  if (z.timing)
-    while ((int)z.ev.size() < 5 * nsteps) {
-      hipEvent_t e;
-      FVM_HIPCHK(c, hipEventCreate(&e));
-      z.ev.push_back(e);
-    }
-  for (int s = 0; s < nsteps; s++) {
-    fvm::FusedArgs<T> A = fused_args<T>(c, dtmax);
-    A.write_w64 = (s == nsteps - 1) ? 1 : 0;
-    hipEvent_t *ev = z.timing ? &z.ev[5 * s] : nullptr;
-    if (ev) FVM_HIPCHK(c, hipEventRecord(ev[0], c->stream));
-    if (lim) hipLaunchKernelGGL((fvm::k_cfl_predict_recon<T, MOVING, true>), dim3(grid), dim3(FB), 0, c->stream, A);
-    else hipLaunchKernelGGL((fvm::k_cfl_predict_recon<T, MOVING, false>), dim3(grid), dim3(FB), 0, c->stream, A);
-    FVM_HIPCHK(c, hipGetLastError());
-    if (ev) FVM_HIPCHK(c, hipEventRecord(ev[1], c->stream));
-    if (lim) {
-      hipLaunchKernelGGL((fvm::k_limit<T>), dim3(grid), dim3(FB), 0, c->stream, A);
-      FVM_HIPCHK(c, hipGetLastError());
-    }
-    if (ev) FVM_HIPCHK(c, hipEventRecord(ev[2], c->stream));
-    hipLaunchKernelGGL((fvm::k_flux_apply<T, MOVING>), dim3(grid), dim3(FB), 0, c->stream, A);
-    FVM_HIPCHK(c, hipGetLastError());
-    if (ev) FVM_HIPCHK(c, hipEventRecord(ev[3], c->stream));
-    hipLaunchKernelGGL(fvm::k_advance, dim3(1), dim3(64), 0, c->stream, (double *)z.tdev.p,
-                       (unsigned long long *)z.dtmin.p, dtmax, (double *)z.hist.p, s);
-    FVM_HIPCHK(c, hipGetLastError());
-    if (ev) FVM_HIPCHK(c, hipEventRecord(ev[4], c->stream));
-    z.cur ^= 1;
-  }
-  if (z.timing && nsteps > 0) {
-    FVM_HIPCHK(c, hipEventSynchronize(z.ev[5 * nsteps - 1]));
-    for (int s = 0; s < nsteps; s++)
-      for (int k = 0; k < 4; k++) {
-        float ms = 0.0f;
-        FVM_HIPCHK(c, hipEventElapsedTime(&ms, z.ev[5 * s + k], z.ev[5 * s + k + 1]));
-        z.ms[k] += ms;
-      }
-    for (int k = 0; k < 4; k++) z.launches[k] += (k == 1 && !lim) ? 0 : nsteps;
-  }
   return FVM_OK;
 }
 
@@ -870,9 +56,7 @@ int fvm_fused_run(fvm_ctx *c, int nsteps, double dtmax, double *t_out) {
   FusedState &z = c->fz;
   int rc = fvm_ensure(c, z.hist, sizeof(double) * (size_t)std::max(nsteps, 1));
   if (rc) return rc;
-  const bool f64 = z.precision == FVM_FUSED_FP64;
-  if (f64) rc = z.moving ? fused_run_t<double, true>(c, nsteps, dtmax) : fused_run_t<double, false>(c, nsteps, dtmax);
-  else rc = z.moving ? fused_run_t<float, true>(c, nsteps, dtmax) : fused_run_t<float, false>(c, nsteps, dtmax);
+  rc = z.precision == FVM_FUSED_FP64 ? fvm_fused_run_f64(c, nsteps, dtmax) : fvm_fused_run_f32(c, nsteps, dtmax);
   if (rc) return rc;
   double t2[2];
   FVM_HIPCHK(c, hipMemcpyAsync(t2, z.tdev.p, sizeof t2, hipMemcpyDeviceToHost, c->stream));
@@ -890,33 +74,11 @@ int fvm_fused_dt_history(fvm_ctx *c, double *dts, int n) {
   return FVM_OK;
 }
 
-// gradients and Wt of the last step into the per-cell fields
-template <typename T, typename S> static int fused_sync_t(fvm_ctx *c) {
-  FusedState &z = c->fz;
-  const int n = c->np;
-  const unsigned nb = (unsigned)((n + 255) / 256);
-  // the last completed step's gradients sit in the records it read (z.cur ^ 1)
-  const T *rec = (const T *)z.rec[z.cur ^ 1].p;
-  const int dstf[4] = {FVM_FIELD_GRAD_X, FVM_FIELD_GRAD_Y, FVM_FIELD_GRAD_Z, FVM_FIELD_DWDT};
-  const int off[4] = {fvm::R_G, fvm::R_G + fvm::NF, fvm::R_G + 2 * fvm::NF, fvm::R_T};
-  for (int q = 0; q < 4; q++) {
-    hipLaunchKernelGGL((fvm::k_rec_to_field<T, S>), dim3(nb), dim3(256), 0, c->stream, rec, off[q],
-                       (S *)c->field[dstf[q]].p, n);
-    FVM_HIPCHK(c, hipGetLastError());
-  }
-  FVM_HIPCHK(c, hipMemcpyAsync(c->field[FVM_FIELD_DT_NEW].p, z.dt_new.p, sizeof(double) * (size_t)n,
-                               hipMemcpyDeviceToDevice, c->stream));
-  FVM_HIPCHK(c, hipStreamSynchronize(c->stream));
-  return FVM_OK;
-}
-
 int fvm_fused_sync(fvm_ctx *c) {
   if (!c) return FVM_EARG;
   if (!c->fz.ready) return fvm_fail(c, FVM_ESTATE, "no fused state");
   (void)hipSetDevice(c->device);
-  const bool shipped = c->prm.storage == FVM_STORAGE_SHIPPED;
-  if (c->fz.precision == FVM_FUSED_FP64) return shipped ? fused_sync_t<double, float>(c) : fused_sync_t<double, double>(c);
-  return shipped ? fused_sync_t<float, float>(c) : fused_sync_t<float, double>(c);
+  return c->fz.precision == FVM_FUSED_FP64 ? fvm_fused_sync_f64(c) : fvm_fused_sync_f32(c);
 }
 
 int fvm_fused_timing(fvm_ctx *c, int enable) {

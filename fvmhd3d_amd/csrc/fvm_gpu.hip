@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(256) k_slope_limiter(Cells<S> C, MeshView M, P
 //   rec[0..9] = flux*dt*A, rec[10..12] = n*(psi*dt*A), rec[13] = Bn*dt*A
 // and swp[a] = 1 when i and j were swapped (s1 on a boundary).
 // ---------------------------------------------------------------------------
-constexpr int NREC = 14;
+// 14 entries per face record (NREC_P = 15 adds the predictor's dV)
 
 template <typename S>
 __global__ void __launch_bounds__(256) k_face_flux(Cells<S> C, MeshView M, Phys P, double *rec, int *swp,

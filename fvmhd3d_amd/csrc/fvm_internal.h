@@ -39,7 +39,10 @@ struct FusedState {
   int moving = 0;
   int E = 0;
   int cur = 0;
-  DBuf fe, gbase, rec[2], mm, dt_new, dtmin, tdev, hist;
+  // internal (window-sorted) cell order: perm[i] = mesh cell of internal cell i
+  DBuf perm, inv, deg, gbase, fe;
+  DBuf U, W64, cvol, mvel;  // FP64 state and geometry in internal order
+  DBuf rec[2], mm, dt_new, dtmin, tdev, hist;
   size_t slots = 0;
   bool timing = false;
   std::vector<hipEvent_t> ev;  // 5 per timed step
@@ -70,6 +73,13 @@ void fvm_release(DBuf &b);
 
 // release of the fused-path buffers of a context (fvm_fast.hip)
 void fvm_fast_release(fvm_ctx *c);
+// fused stepping per face precision (fvm_fused_f32.hip, fvm_fused_f64.hip)
+int fvm_fused_setup_f32(fvm_ctx *c);
+int fvm_fused_setup_f64(fvm_ctx *c);
+int fvm_fused_run_f32(fvm_ctx *c, int nsteps, double dtmax);
+int fvm_fused_run_f64(fvm_ctx *c, int nsteps, double dtmax);
+int fvm_fused_sync_f32(fvm_ctx *c);
+int fvm_fused_sync_f64(fvm_ctx *c);
 
 #define FVM_HIPCHK(c, call)                                                                \
   do {                                                                                     \

@@ -100,27 +100,35 @@ __device__ __forceinline__ double limiter_sl(int mode, double dWi, double mn, do
 
 // limiters in the face precision T: the exact double versions above for the
 // FP64 path, hardware reciprocal for the FP32 face path
-template <typename T> __device__ __forceinline__ T phi_t(int mode, T x) {
-  if (mode == LIM_SHIPPED) return T(1.0);
-  if (mode == LIM_MINMOD) return smin(T(1.0), x);
-  const T fac = T(0.2), eps1 = T(1.0e-16);
-  return fdiv(x * x + T(2.0) * x + fac * eps1, x * x + x + T(2.0) + fac * eps1);
-}
-template <typename T> __device__ __forceinline__ T limiter_rec_t(int mode, T dWi, T mn, T mx) {
-  if constexpr (sizeof(T) == 8) {
-    return limiter_rec(mode, dWi, mn, mx);
+template <int MODE, typename T> __device__ __forceinline__ T phi_c(T x) {
+  if constexpr (MODE == LIM_MINMOD) {
+    return smin(T(1.0), x);
+  } else if constexpr (MODE == LIM_SMOOTH) {
+    const T fac = T(0.2), eps1 = T(1.0e-16);
+    return fdiv(x * x + T(2.0) * x + fac * eps1, x * x + x + T(2.0) + fac * eps1);
   } else {
-    if (dabs(dWi) <= T(1.0e-16) * smax(dabs(mx), dabs(mn))) return T(1.0);
-    const T invWi = frcp(dWi);
-    return phi_t<T>(mode, (dWi < T(0.0) ? mn : mx) * invWi);
+    return T(1.0);
   }
 }
-template <typename T> __device__ __forceinline__ T limiter_sl_t(int mode, T dWi, T mn, T mx) {
+// The mode is a compile-time constant.  FP32: both outcomes are formed and the
+// flat-difference test selects (no divergent branch per variable); the ratio
+// of a flat difference may be inf/NaN and is discarded by the select.
+template <int MODE, typename T> __device__ __forceinline__ T limiter_rec_c(T dWi, T mn, T mx) {
   if constexpr (sizeof(T) == 8) {
-    return limiter_sl(mode, dWi, mn, mx);
+    return limiter_rec(MODE, dWi, mn, mx);
   } else {
-    if (dabs(dWi) <= T(1.0e-16) * smax(dabs(mx), dabs(mn))) return T(1.0);
-    return phi_t<T>(mode, fdiv(dWi < T(0.0) ? mn : mx, dWi));
+    const bool flat = dabs(dWi) <= T(1.0e-16) * smax(dabs(mx), dabs(mn));
+    const T r = phi_c<MODE, T>((dWi < T(0.0) ? mn : mx) * frcp(dWi));
+    return flat ? T(1.0) : r;
+  }
+}
+template <int MODE, typename T> __device__ __forceinline__ T limiter_sl_c(T dWi, T mn, T mx) {
+  if constexpr (sizeof(T) == 8) {
+    return limiter_sl(MODE, dWi, mn, mx);
+  } else {
+    const bool flat = dabs(dWi) <= T(1.0e-16) * smax(dabs(mx), dabs(mn));
+    const T r = phi_c<MODE, T>(fdiv(dWi < T(0.0) ? mn : mx, dWi));
+    return flat ? T(1.0) : r;
   }
 }
 

@@ -86,6 +86,9 @@ def test_substeps_gpu_match_oracle():
         scale = np.max(np.abs(ref), axis=0) + 1e-300
         err = np.max(np.abs(got - ref) / scale)
         print(f"{name}: max rel err {err:.2e}")
-        assert err <= 1e-12, (name, err)
+        # t is the predictor's finite difference (W1 - W0)/(dt/128): a 1-ulp difference
+        # of the device exp/pow against glibc in W1 reaches t as eps*|W|*128/(|t|*dt),
+        # a few 1e-12 here; every other field is held to 1e-12
+        assert err <= (1e-10 if name == "t" else 1e-12), (name, err)
     fr = [h[2] / m.nface for h in Sg.history]
     print("active face fractions", np.round(fr, 3))
