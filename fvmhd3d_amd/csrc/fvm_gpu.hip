@@ -341,12 +341,18 @@ __global__ void __launch_bounds__(256) k_slope_limiter(Cells<S> C, MeshView M, P
 // ---------------------------------------------------------------------------
 // 14 entries per face record (NREC_P = 15 adds the predictor's dV)
 
+// computeFluidUpdate.cpp:99-209: the inputs of computeFlux for active face a
+// (boundary swap, space-time extrapolation of both sides, face velocity)
+struct FaceIn {
+  int i, j, jbnd;
+  bool sw;
+  double dt, area, normal[3], wij[3], Wi[NF], Wj[NF];
+};
+
 template <typename S>
-__global__ void __launch_bounds__(256) k_face_flux(Cells<S> C, MeshView M, Phys P, double *rec, int *swp,
-                                   unsigned long long *diag) {
-  const int a = gid();
-  if (a >= M.nfa) return;
-  const int np = C.np, nfa = M.nfa;
+__device__ __forceinline__ void face_inputs(const Cells<S> &C, const MeshView &M, const Phys &P, int a,
+                                            unsigned long long *diag, FaceIn &q) {
+  const int np = C.np;
   const int f = M.fact[a];
   int i = M.s1[f], j = M.s2[f];
   const double fn[3] = {FLD(M.n, 0, f, M.nface), FLD(M.n, 1, f, M.nface), FLD(M.n, 2, f, M.nface)};
@@ -372,10 +378,10 @@ __global__ void __launch_bounds__(256) k_face_flux(Cells<S> C, MeshView M, Phys 
   const double dtj = dtJ - 0.5 * dt;
   const double dsh = dri[0] * normal[0] + dri[1] * normal[1] + dri[2] * normal[2];
   const double dsl = 2.0 * dsh;
-  if (!(dsl > 0.0)) atomicAdd(&diag[FVM_DIAG_DSL], 1ULL);
+  if (diag && !(dsl > 0.0)) atomicAdd(&diag[FVM_DIAG_DSL], 1ULL);
   const double drj[3] = {dri[0] - normal[0] * dsl, dri[1] - normal[1] * dsl, dri[2] - normal[2] * dsl};
 
-  double Wi[NF], Wj[NF];
+  double *Wi = q.Wi, *Wj = q.Wj;
   for (int k = 0; k < NF; k++) {
     Wi[k] = FLD(C.w, k, i, np);
     Wj[k] = FLD(C.w, k, j, np);
@@ -389,7 +395,7 @@ __global__ void __launch_bounds__(256) k_face_flux(Cells<S> C, MeshView M, Phys 
                (double)FLD(C.z, k, j, np) * drj[2] + (double)FLD(C.t, k, j, np) * dtj;
     }
   }
-  if (!(Wi[DENS] > 0.0) || !(Wi[ETHM] > 0.0) || !(Wj[DENS] > 0.0) || !(Wj[ETHM] > 0.0))
+  if (diag && (!(Wi[DENS] > 0.0) || !(Wi[ETHM] > 0.0) || !(Wj[DENS] > 0.0) || !(Wj[ETHM] > 0.0)))
     atomicAdd(&diag[FVM_DIAG_POSITIVITY], 1ULL);
 
   const double vi[3] = {(double)FLD(C.vel, 0, i, np), (double)FLD(C.vel, 1, i, np), (double)FLD(C.vel, 2, i, np)};
@@ -400,17 +406,54 @@ __global__ void __launch_bounds__(256) k_face_flux(Cells<S> C, MeshView M, Phys 
   const double fdv = dvij[0] * fij[0] + dvij[1] * fij[1] + dvij[2] * fij[2];
   const double rji[3] = {normal[0] * dsl, normal[1] * dsl, normal[2] * dsl};
   const double r = 1.0 / sqr(dsl);
-  const double wij[3] = {vij[0] * 0.5 - rji[0] * fdv * r, vij[1] * 0.5 - rji[1] * fdv * r,
-                         vij[2] * 0.5 - rji[2] * fdv * r};
+  q.wij[0] = vij[0] * 0.5 - rji[0] * fdv * r;
+  q.wij[1] = vij[1] * 0.5 - rji[1] * fdv * r;
+  q.wij[2] = vij[2] * 0.5 - rji[2] * fdv * r;
+  for (int d = 0; d < 3; d++) q.normal[d] = normal[d];
+  q.i = i;
+  q.j = j;
+  q.jbnd = jbnd;
+  q.sw = sw;
+  q.dt = dt;
+  q.area = area;
+}
 
+template <typename S>
+__global__ void __launch_bounds__(256) k_face_flux(Cells<S> C, MeshView M, Phys P, double *rec, int *swp,
+                                   unsigned long long *diag) {
+  const int a = gid();
+  if (a >= M.nfa) return;
+  const int nfa = M.nfa;
+  FaceIn q;
+  face_inputs(C, M, P, a, diag, q);
   double psi, Bn, flux[NF];
-  compute_flux<double>(P.gamma, wij, Wi, Wj, jbnd, normal, psi, Bn, flux);
+  compute_flux<double>(P.gamma, q.wij, q.Wi, q.Wj, q.jbnd, q.normal, psi, Bn, flux);
 
-  const double tarea = dt * area;
+  const double tarea = q.dt * q.area;
   for (int k = 0; k < NF; k++) FLD(rec, k, a, nfa) = flux[k] * tarea;
-  for (int d = 0; d < 3; d++) FLD(rec, NF + d, a, nfa) = normal[d] * (psi * tarea);
+  for (int d = 0; d < 3; d++) FLD(rec, NF + d, a, nfa) = q.normal[d] * (psi * tarea);
   FLD(rec, NF + 3, a, nfa) = Bn * tarea;
-  swp[a] = sw ? 1 : 0;
+  swp[a] = q.sw ? 1 : 0;
+}
+
+// the computeFlux inputs of active faces [first, first + count), in the
+// batch layout of fvm_compute_flux
+template <typename S>
+__global__ void __launch_bounds__(256) k_face_states(Cells<S> C, MeshView M, Phys P, int first, int count, double *wij,
+                                                     double *Wi, double *Wj, int *jbnd, double *normal) {
+  const int q0 = gid();
+  if (q0 >= count) return;
+  FaceIn q;
+  face_inputs(C, M, P, first + q0, nullptr, q);
+  for (int d = 0; d < 3; d++) {
+    wij[3 * (size_t)q0 + d] = q.wij[d];
+    normal[3 * (size_t)q0 + d] = q.normal[d];
+  }
+  for (int k = 0; k < NF; k++) {
+    Wi[(size_t)NF * q0 + k] = q.Wi[k];
+    Wj[(size_t)NF * q0 + k] = q.Wj[k];
+  }
+  jbnd[q0] = q.jbnd;
 }
 
 // computeFluidUpdate.cpp:225-245: per cell, in face_active_list order
@@ -760,7 +803,7 @@ __global__ void __launch_bounds__(256) k_predictor_cell(Cells<S> C, MeshView M, 
 // ---------------------------------------------------------------------------
 // batched System::computeFlux (FP64)
 // ---------------------------------------------------------------------------
-__global__ void k_flux_batch(int n, double gamma, const double *wij, const double *Wi, const double *Wj,
+__global__ void __launch_bounds__(256) k_flux_batch(int n, double gamma, const double *wij, const double *Wi, const double *Wj,
                              const int *jbnd, const double *normal, double *psi, double *Bn, double *flux) {
   const int f = gid();
   if (f >= n) return;
@@ -1259,6 +1302,36 @@ int fvm_compute_fluid_update(fvm_ctx *c) {
   if (!c->have_mesh || c->nfa < 0) return fail(c, FVM_ESTATE, "no mesh or no active face list");
   (void)hipSetDevice(c->device);
   return DISPATCH(c, do_fluid_update, c);
+}
+
+template <typename S>
+static int do_face_states(fvm_ctx *c, int first, int count, double *wij, double *Wi, double *Wj, int *jbnd,
+                          double *normal) {
+  const size_t N = (size_t)count;
+  const size_t nd = (3 + 10 + 10 + 3) * N;
+  int rc = ensure(c, c->stage, sizeof(double) * nd + sizeof(int) * N);
+  if (rc) return rc;
+  double *d_wij = (double *)c->stage.p, *d_Wi = d_wij + 3 * N, *d_Wj = d_Wi + 10 * N, *d_n = d_Wj + 10 * N;
+  int *d_jb = (int *)(d_n + 3 * N);
+  LAUNCH(c, fvm::k_face_states<S>, count, cells<S>(c), meshview(c), phys(c), first, count, d_wij, d_Wi, d_Wj, d_jb,
+         d_n);
+  HIPCHK(c, hipMemcpyAsync(wij, d_wij, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(Wi, d_Wi, sizeof(double) * 10 * N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(Wj, d_Wj, sizeof(double) * 10 * N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(normal, d_n, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(jbnd, d_jb, sizeof(int) * N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FVM_OK;
+}
+int fvm_get_face_states(fvm_ctx *c, int first, int count, double *wij, double *Wi, double *Wj, int *jbnd,
+                        double *normal) {
+  if (!c) return FVM_EARG;
+  if (!c->have_mesh || c->nfa < 0) return fail(c, FVM_ESTATE, "no mesh or no active face list");
+  if (first < 0 || count < 0 || first + count > c->nfa) return fail(c, FVM_EARG, "face range outside the active list");
+  if (count == 0) return FVM_OK;
+  if (!wij || !Wi || !Wj || !jbnd || !normal) return fail(c, FVM_EARG, "NULL array");
+  (void)hipSetDevice(c->device);
+  return DISPATCH(c, do_face_states, c, first, count, wij, Wi, Wj, jbnd, normal);
 }
 
 template <typename S> static int do_primitives(fvm_ctx *c, int flag) {

@@ -76,6 +76,23 @@ SIGNATURES = {
     "fvm_fused_timing": [_P, _I],
     "fvm_fused_kernel_ms": [_P, _P, _P, _I],
     "fvm_measure_copy_bandwidth": [_P, ctypes.c_longlong, _I, ctypes.POINTER(ctypes.c_double)],
+    "fvm_get_face_states": [_P, _I, _I, _P, _P, _P, _P, _P],
+    # learned HLLD surrogate (include/fvm_surrogate.h)
+    "fvm_sg_create": [ctypes.POINTER(ctypes.c_void_p), _I, _D],
+    "fvm_sg_destroy": [_P],
+    "fvm_sg_set_params": [_P, _P, _I],
+    "fvm_sg_get_params": [_P, _P, _I],
+    "fvm_sg_set_gate": [_P, _P],
+    "fvm_sg_add_faces": [_P, _I, _P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_int)],
+    "fvm_sg_clear_samples": [_P],
+    "fvm_sg_num_samples": [_P, ctypes.POINTER(ctypes.c_longlong)],
+    "fvm_sg_get_samples": [_P, ctypes.c_longlong, _I, _P, _P],
+    "fvm_sg_train": [_P, _I, _I, _D, _D, ctypes.c_ulonglong, _P],
+    "fvm_sg_gradients": [_P, _I, _P, _P, _D, _P, _P, ctypes.POINTER(ctypes.c_double)],
+    "fvm_sg_forward": [_P, _I, _P, _P],
+    "fvm_sg_compute_flux": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_int)],
+    "fvm_sg_compute_flux_device": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_int)],
+    "fvm_sg_time_flux": [_P, _I, _P, _P, _P, _P, _P, _I, _P, ctypes.POINTER(ctypes.c_int)],
 }
 FUSED_FP64, FUSED_FP32 = 0, 1
 
@@ -92,8 +109,10 @@ def lib():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = ctypes.c_int
-        L.fvm_last_error.argtypes = [ctypes.c_void_p]
-        L.fvm_last_error.restype = ctypes.c_char_p
+        for name in ("fvm_last_error", "fvm_sg_last_error"):
+            getattr(L, name).argtypes = [ctypes.c_void_p]
+            getattr(L, name).restype = ctypes.c_char_p
+        L.fvm_sg_destroy.restype = None
         _lib = L
     return _lib
 
@@ -208,6 +227,14 @@ class Context:
         self._chk(self.L.fvm_compute_flux(self.h, n, _ptr(wij), _ptr(Wi), _ptr(Wj), _ptr(jbnd), _ptr(normal),
                                           _ptr(psi), _ptr(Bn), _ptr(flux)))
         return psi, Bn, flux
+
+    def face_states(self, first, count):
+        """computeFlux inputs of active faces [first, first+count): (wij, Wi, Wj, jbnd, normal)."""
+        wij, Wi, Wj = np.empty((count, 3)), np.empty((count, 10)), np.empty((count, 10))
+        jbnd, normal = np.empty(count, np.int32), np.empty((count, 3))
+        self._chk(self.L.fvm_get_face_states(self.h, int(first), int(count), _ptr(wij), _ptr(Wi), _ptr(Wj),
+                                             _ptr(jbnd), _ptr(normal)))
+        return wij, Wi, Wj, jbnd, normal
 
     def diagnostics(self):
         out = (ctypes.c_longlong * NDIAG)()

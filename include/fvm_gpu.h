@@ -157,6 +157,12 @@ int fvm_compute_predictor(fvm_ctx *ctx, double dtmax, int min_rung);
  * arrays: wij[n][3], Wi/Wj[n][10], jbnd[n], normal[n][3] -> psi[n], Bn[n], flux[n][10] */
 int fvm_compute_flux(fvm_ctx *ctx, int n, const double *wij, const double *Wi, const double *Wj,
                      const int *jbnd, const double *normal, double *psi, double *Bn, double *flux);
+/* The arguments computeFluidUpdate passes to computeFlux (computeFluidUpdate.cpp:99-211:
+ * boundary swap, space-time extrapolated face states, face velocity) for active
+ * faces [first, first + count) of the active list, in fvm_compute_flux's layout.
+ * Needs the reconstruction (and limiter) of the current step. */
+int fvm_get_face_states(fvm_ctx *ctx, int first, int count, double *wij, double *Wi, double *Wj, int *jbnd,
+                        double *normal);
 
 /* ---- device-resident stepping: static mesh, every cell active, one global
  * timestep (all cells on the smallest rung of the Scheduler, dt = rung of the

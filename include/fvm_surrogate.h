@@ -1,0 +1,100 @@
+/* Learned HLLD surrogate: C ABI.
+ *
+ * No counterpart in the reference.  It stands in for riemannSolver_HLLD
+ * (computeFlux.cpp:178-425) inside computeFlux (computeFlux.cpp:6-148), with
+ * the same batch interface as fvm_compute_flux (fvm_gpu.h) and a gated
+ * fallback to the FP64 HLLD for every face the gate rejects.
+ *
+ * The network is 16 -> 128 -> 128 -> 128 -> 8 (ReLU), bfloat16 operands with
+ * FP32 accumulation on the matrix cores.  Inputs are the HLLD inputs of a face
+ * in the face frame after a Galilean boost (face speed along the normal, mean
+ * tangential velocity) and the MHD scaling (density, fast-speed scale) are
+ * divided out; outputs 0..6 are the normalised moving-frame fluxes (mass, three
+ * momenta, energy, two tangential induction components), output 7 predicts
+ * log10 of the squared error of outputs 0..6 (the confidence gate).  The
+ * normal induction flux -w_n B_n, the Dedner B_n/psi, and the psi and entropy
+ * fluxes are formed exactly as computeFlux forms them.
+ *
+ * Parameters are one FP32 vector, layer by layer: W0[16][128] b0[128]
+ * W1[128][128] b1[128] W2[128][128] b2[128] W3[128][8] b3[8] (W[in][out]).
+ *
+ * All functions return FVM_OK (0) or a negative FVM_E* code (fvm_gpu.h);
+ * fvm_sg_last_error gives the detail.  Host arrays are caller-owned; calls are
+ * synchronous on the surrogate's own stream.
+ */
+#ifndef FVM_SURROGATE_H
+#define FVM_SURROGATE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fvm_sg fvm_sg;
+
+enum { FVM_SG_NIN = 16, FVM_SG_NH = 128, FVM_SG_NOUT = 8, FVM_SG_NPARAM = 36232 };
+
+/* Faces the network may answer; the rest go to the FP64 HLLD.
+ *   max_log_rho_jump, max_log_p_jump  bounds on |ln(rho_R/rho_L)|, |ln(p_R/p_L)|
+ *   max_vn                            bound on |v_n - w_n| / c_s of both sides
+ *   max_log10_err                     bound on the confidence output
+ * Boundary faces (jbnd != FVM_NO_BOUNDARY) and non-positive states always fall back. */
+typedef struct fvm_sg_gate {
+  double max_log_rho_jump;
+  double max_log_p_jump;
+  double max_vn;
+  double max_log10_err;
+} fvm_sg_gate;
+
+int fvm_sg_create(fvm_sg **s, int device, double gamma_gas);
+void fvm_sg_destroy(fvm_sg *s);
+const char *fvm_sg_last_error(const fvm_sg *s);
+
+int fvm_sg_set_params(fvm_sg *s, const float *params, int n);
+int fvm_sg_get_params(fvm_sg *s, float *params, int n);
+int fvm_sg_set_gate(fvm_sg *s, const fvm_sg_gate *gate);
+
+/* Training set: faces in the computeFlux batch layout (wij[n][3], Wi[n][10],
+ * Wj[n][10], jbnd[n], normal[n][3]; host arrays).  Each face that passes the
+ * domain part of the gate becomes one sample (16 features, 7 targets from the
+ * FP64 HLLD); *n_kept receives how many did. */
+int fvm_sg_add_faces(fvm_sg *s, int n, const double *wij, const double *Wi, const double *Wj, const int *jbnd,
+                     const double *normal, int *n_kept);
+int fvm_sg_clear_samples(fvm_sg *s);
+int fvm_sg_num_samples(fvm_sg *s, long long *n);
+int fvm_sg_get_samples(fvm_sg *s, long long first, int n, float *x, float *t);
+
+/* Adam training on the device: `steps` minibatches of `batch` samples drawn by
+ * a counter hash of (seed, step, slot); loss = mean |y[0..6]-t|^2 +
+ * lam * (y[7] - log10(|y[0..6]-t|^2 + 1e-10))^2.  loss_hist[steps] (may be
+ * NULL) receives each step's loss. */
+int fvm_sg_train(fvm_sg *s, int steps, int batch, double lr, double lam, unsigned long long seed, float *loss_hist);
+
+/* One forward and backward pass over the given samples, no update: network
+ * outputs y[n][8], gradients grads[FVM_SG_NPARAM] and the loss. */
+int fvm_sg_gradients(fvm_sg *s, int n, const float *x, const float *t, double lam, float *y, float *grads,
+                     double *loss);
+
+/* The fused inference kernel on raw features x[n][16] -> y[n][8]. */
+int fvm_sg_forward(fvm_sg *s, int n, const float *x, float *y);
+
+/* computeFlux on a batch, surrogate + FP64 fallback.  Host arrays; the device
+ * variant takes device pointers and runs on the surrogate's stream.
+ * *n_fallback (may be NULL) receives the number of faces the FP64 HLLD
+ * answered. */
+int fvm_sg_compute_flux(fvm_sg *s, int n, const double *wij, const double *Wi, const double *Wj, const int *jbnd,
+                        const double *normal, double *psi, double *Bn, double *flux, int *n_fallback);
+int fvm_sg_compute_flux_device(fvm_sg *s, int n, const double *wij, const double *Wi, const double *Wj,
+                               const int *jbnd, const double *normal, double *psi, double *Bn, double *flux,
+                               int *n_fallback);
+
+/* Kernel timing on device-resident copies of one batch: average milliseconds
+ * per launch of the surrogate flux (featurise + network + gate, then the
+ * fallback launch), of the plain FP64 HLLD (computeFlux in double) and of the
+ * FP32 HLLD over the same faces.  ms[3] = {surrogate, hlld_f64, hlld_f32}. */
+int fvm_sg_time_flux(fvm_sg *s, int n, const double *wij, const double *Wi, const double *Wj, const int *jbnd,
+                     const double *normal, int iters, double *ms, int *n_fallback);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
