@@ -77,6 +77,8 @@ SIGNATURES = {
     "fvm_fused_kernel_ms": [_P, _P, _P, _I],
     "fvm_measure_copy_bandwidth": [_P, ctypes.c_longlong, _I, ctypes.POINTER(ctypes.c_double)],
     "fvm_get_face_states": [_P, _I, _I, _P, _P, _P, _P, _P],
+    "fvm_energy": [_P, _P],
+    "fvm_dump_binary": [_P, ctypes.c_char_p, _P, _P, _P],
     # learned HLLD surrogate (include/fvm_surrogate.h)
     "fvm_sg_create": [ctypes.POINTER(ctypes.c_void_p), _I, _D],
     "fvm_sg_destroy": [_P],
@@ -95,6 +97,14 @@ SIGNATURES = {
     "fvm_sg_time_flux": [_P, _I, _P, _P, _P, _P, _P, _I, _P, ctypes.POINTER(ctypes.c_int)],
 }
 FUSED_FP64, FUSED_FP32 = 0, 1
+ENERGY_NAMES = ("MASS", "ETHM", "EPOT", "ETOT", "EKINX", "EKINY", "EKINZ", "EKIN", "EMAGX", "EMAGY", "EMAGZ", "EMAG",
+                "MOMX", "MOMY", "MOMZ", "MOM", "ANMX", "ANMY", "ANMZ", "ANM", "VOLUME")
+
+
+class SnapshotInfo(ctypes.Structure):
+    _fields_ = [("chare_index", ctypes.c_int), ("num_elements", ctypes.c_int), ("global_n", ctypes.c_int),
+                ("iteration", ctypes.c_int), ("tsysU", ctypes.c_ulonglong), ("dt_global", ctypes.c_double),
+                ("rmin", ctypes.c_double * 3), ("rmax", ctypes.c_double * 3)]
 
 
 def lib():
@@ -235,6 +245,21 @@ class Context:
         self._chk(self.L.fvm_get_face_states(self.h, int(first), int(count), _ptr(wij), _ptr(Wi), _ptr(Wj),
                                              _ptr(jbnd), _ptr(normal)))
         return wij, Wi, Wj, jbnd, normal
+
+    def energy(self):
+        """Energy totals (Energy.h:27-79) as a dict in Energy's order."""
+        out = np.empty(len(ENERGY_NAMES))
+        self._chk(self.L.fvm_energy(self.h, _ptr(out)))
+        return dict(zip(ENERGY_NAMES, out))
+
+    def dump_binary(self, path, dt_global, iteration=0, chare_index=0, num_elements=1, global_n=None, tsysU=0,
+                    rmin=(0.0, 0.0, 0.0), rmax=(1.0, 1.0, 1.0), idx=None, dt_cell=None):
+        """System::dump_binary (IO.cpp:7-99) of the active cells into `path`."""
+        info = SnapshotInfo(chare_index, num_elements, self.np if global_n is None else global_n, iteration, tsysU,
+                            dt_global, (ctypes.c_double * 3)(*rmin), (ctypes.c_double * 3)(*rmax))
+        idx = None if idx is None else np.ascontiguousarray(idx, np.int32)
+        dt_cell = None if dt_cell is None else np.ascontiguousarray(dt_cell, np.float64)
+        self._chk(self.L.fvm_dump_binary(self.h, os.fsencode(path), ctypes.byref(info), _ptr(idx), _ptr(dt_cell)))
 
     def diagnostics(self):
         out = (ctypes.c_longlong * NDIAG)()

@@ -191,25 +191,40 @@ def rung_dt(dtmax, dt):
     return dtmax / 2.0 ** 31 * 2.0 ** (31 - rung)
 
 
-def face_states(mesh, W, limiter, t_global=0.0, dtmax=1.0, first=0, count=None):
-    """The computeFlux inputs of every active face for state W on `mesh`, as
+class FaceStates:
+    """The computeFlux inputs of the active faces of state W on `mesh`, as
     computeFluidUpdate forms them after computeTimestep, computePredictor,
-    computeReconstruction and slopeLimiter of one global step."""
-    from .state import global_step_state
-    st = global_step_state(mesh, W, dt=0.0, t_global=t_global)
-    ctx = G.Context(gamma=st.gamma, courant=st.courant, box=st.box, limiter=limiter)
+    computeReconstruction and slopeLimiter of one global step; read in
+    chunks from one prepared reference-order context."""
+
+    def __init__(self, mesh, W, limiter, t_global=0.0, dtmax=1.0):
+        from .state import global_step_state
+        st = global_step_state(mesh, W, dt=0.0, t_global=t_global)
+        self.ctx = G.Context(gamma=st.gamma, courant=st.courant, box=st.box, limiter=limiter)
+        c = self.ctx
+        c.upload_state(st)
+        self.nfa = len(c.get_active_faces())
+        dt = rung_dt(dtmax, c.compute_timestep())
+        c.upload("tend", np.full(st.np, t_global + dt))
+        c.compute_predictor(dtmax, 0)
+        c.compute_reconstruction()
+        c.slope_limiter()
+
+    def read(self, first=0, count=None):
+        count = self.nfa - first if count is None else min(count, self.nfa - first)
+        return self.ctx.face_states(first, count)
+
+    def close(self):
+        self.ctx.close()
+
+
+def face_states(mesh, W, limiter, t_global=0.0, dtmax=1.0, first=0, count=None):
+    """All (or a range of) the active faces' computeFlux inputs; see FaceStates."""
+    fs = FaceStates(mesh, W, limiter, t_global, dtmax)
     try:
-        ctx.upload_state(st)
-        nfa = len(ctx.get_active_faces())
-        dt = rung_dt(dtmax, ctx.compute_timestep())
-        ctx.upload("tend", np.full(st.np, t_global + dt))
-        ctx.compute_predictor(dtmax, 0)
-        ctx.compute_reconstruction()
-        ctx.slope_limiter()
-        count = nfa - first if count is None else min(count, nfa - first)
-        return ctx.face_states(first, count)
+        return fs.read(first, count)
     finally:
-        ctx.close()
+        fs.close()
 
 
 def evolve(mesh, W, limiter, nsteps, precision=G.FUSED_FP64, dtmax=1.0):

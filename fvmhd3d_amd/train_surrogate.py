@@ -1,0 +1,145 @@
+"""Train the learned HLLD surrogate on harvested face states and store its parameters.
+
+    python -m fvmhd3d_amd.train_surrogate [--out fvmhd3d_amd/data/hlld_surrogate.npz]
+
+Training set: the computeFlux inputs of real faces of
+  * the circularly polarised Alfven wave on the jittered 32^3 lattice (every
+    face, initial state and after a few global steps), and
+  * the MHD blast on 2,097,152 uniform-random generators (128^3), a random
+    subset of faces at several times of its first steps,
+each face becoming one sample when it passes the domain part of the gate.
+Held out: another blast snapshot.  The confidence threshold is chosen on the
+held-out faces so that the accepted faces' flux stays within --target-l1 of
+the FP64 HLLD (L1 over faces and components, relative), then surrogate +
+fallback and the FP64 / FP32 HLLD are timed on the held-out faces.  One JSON
+line with the figures goes to stdout.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import gpu as G
+from . import mesh as M
+from . import surrogate as S
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _mesh(kind, n, seed=123):
+    pos = M.uniform_random(n ** 3, seed=seed) if kind == "random" else M.jittered_lattice(n, seed=seed)
+    pos = pos[M.morton_order(pos, (1.0, 1.0, 1.0))]
+    return M.build_periodic(pos)
+
+
+def harvest(sg, m, W0, limiter, steps, per_snapshot, rng, chunk=1 << 20):
+    """Add up to `per_snapshot` faces of each snapshot (after each entry of `steps`
+    global steps), read as randomly chosen chunks of the active list, to the
+    surrogate's training set."""
+    total = 0
+    for k in steps:
+        W, t = S.evolve(m, W0, limiter, k)
+        fs = S.FaceStates(m, W, limiter, t_global=t)
+        try:
+            take = min(per_snapshot, fs.nfa)
+            got = 0
+            for ci in rng.permutation((fs.nfa + chunk - 1) // chunk):
+                if got >= take:
+                    break
+                faces = fs.read(int(ci) * chunk, min(chunk, take - got))
+                total += sg.add_faces(*faces)
+                got += len(faces[1])
+        finally:
+            fs.close()
+        log(f"[train] snapshot after {k} steps: {got} faces read, {total} samples so far")
+    return total
+
+
+def evaluate(sg, faces, target_l1):
+    """Pick the loosest confidence threshold whose accepted faces stay within
+    target_l1 of the FP64 HLLD; return (threshold, l1, fallback fraction)."""
+    ctx = G.Context()
+    try:
+        _, _, F = ctx.compute_flux(*faces)
+    finally:
+        ctx.close()
+    best = None
+    g = sg.gate
+    for thr in (0.0, -1.0, -2.0, -2.5, -3.0, -3.5, -4.0, -4.5, -5.0, -6.0):
+        sg.set_gate(g.max_log_rho_jump, g.max_log_p_jump, g.max_vn, thr)
+        _, _, F2, nfb = sg.compute_flux(*faces)
+        l1 = float(np.sum(np.abs(F2[:, :8] - F[:, :8])) / np.sum(np.abs(F[:, :8])))
+        log(f"[train] threshold {thr:5.1f}: fallback {nfb / len(F):.3f}, L1 rel {l1:.3e}")
+        if l1 <= target_l1:
+            best = (thr, l1, nfb / len(F))
+            break
+    if best is None:
+        best = (-1e30, 0.0, 1.0)
+    sg.set_gate(g.max_log_rho_jump, g.max_log_p_jump, g.max_vn, best[0])
+    return best
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=S.DEFAULT_PARAMS)
+    ap.add_argument("--blast-n", type=int, default=128)
+    ap.add_argument("--alfven-n", type=int, default=32)
+    ap.add_argument("--faces-per-snapshot", type=int, default=2_000_000)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--target-l1", type=float, default=1e-2)
+    ap.add_argument("--seed", type=int, default=11)
+    args = ap.parse_args()
+    rng = np.random.default_rng(args.seed)
+    t0 = time.perf_counter()
+    sg = S.Surrogate(gate=(0.5, 1.0, 3.0, 1e30))
+
+    ma = _mesh("lattice", args.alfven_n)
+    harvest(sg, ma, M.ic_alfven(ma.pos), G.LIMITER_SHIPPED, (0, 4), args.faces_per_snapshot, rng)
+    mb = _mesh("random", args.blast_n)
+    Wb = M.ic_blast(mb.pos)
+    n = harvest(sg, mb, Wb, G.LIMITER_SMOOTH, (1, 5, 12, 20), args.faces_per_snapshot, rng)
+    log(f"[train] {n} samples in {time.perf_counter() - t0:.1f} s")
+
+    t1 = time.perf_counter()
+    hist = []
+    # piecewise-constant learning rate: lr, lr/3, lr/10, lr/30 over the four quarters
+    for q, f in enumerate((1.0, 1 / 3, 0.1, 1 / 30)):
+        h = sg.train(args.steps // 4, batch=args.batch, lr=args.lr * f, seed=args.seed + q)
+        hist.append(h)
+        log(f"[train] quarter {q}: loss {np.mean(h[:20]):.3e} -> {np.mean(h[-20:]):.3e}")
+    hist = np.concatenate(hist)
+    train_s = time.perf_counter() - t1
+
+    # held out: the blast after 16 steps
+    W, t = S.evolve(mb, Wb, G.LIMITER_SMOOTH, 16)
+    fs = S.FaceStates(mb, W, G.LIMITER_SMOOTH, t_global=t)
+    try:
+        faces = fs.read(0, min(fs.nfa, 4_000_000))
+    finally:
+        fs.close()
+    thr, l1, fbf = evaluate(sg, faces, args.target_l1)
+    ms, nfb = sg.time_flux(*faces, iters=10)
+    nf = len(faces[1])
+    sg.save(args.out, loss=hist[-100:], samples=np.int64(n))
+    out = {"samples": n, "train_steps": int(args.steps), "batch": args.batch, "train_s": round(train_s, 1),
+           "loss_first": float(np.mean(hist[:20])), "loss_last": float(np.mean(hist[-20:])),
+           "held_out_faces": nf, "confidence_threshold": thr, "accepted_l1_rel": l1, "fallback_fraction": fbf,
+           "ms": {"surrogate_plus_fallback": ms[0], "hlld_fp64": ms[1], "hlld_fp32": ms[2]},
+           "faces_per_s": {"surrogate_plus_fallback": nf / (ms[0] * 1e-3), "hlld_fp64": nf / (ms[1] * 1e-3),
+                           "hlld_fp32": nf / (ms[2] * 1e-3)},
+           "speedup_vs_fp64": ms[1] / ms[0], "params": os.path.relpath(args.out)}
+    print(json.dumps(out), flush=True)
+    sg.close()
+
+
+if __name__ == "__main__":
+    main()

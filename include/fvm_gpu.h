@@ -164,6 +164,29 @@ int fvm_compute_flux(fvm_ctx *ctx, int n, const double *wij, const double *Wi, c
 int fvm_get_face_states(fvm_ctx *ctx, int first, int count, double *wij, double *Wi, double *Wj, int *jbnd,
                         double *normal);
 
+/* ---- diagnostics and snapshots ------------------------------------------- */
+/* Energy totals over the active cells (Energy.h:27-79), from the U, MESH_VOLUME
+ * and POS fields, in Energy's order: MASS ETHM EPOT ETOT EKINX EKINY EKINZ EKIN
+ * EMAGX EMAGY EMAGZ EMAG MOMX MOMY MOMZ MOM ANMX ANMY ANMZ ANM VOLUME.  EPOT is 0
+ * as in the reference.  Deterministic (fixed summation order). */
+#define FVM_ENERGY_NVAR 21
+int fvm_energy(fvm_ctx *ctx, double *out);
+/* System::dump_binary (IO.cpp:7-99): one per-chare float32 snapshot file, a
+ * 20-word header and one 26-word record per active cell, each wrapped in
+ * 4-byte length markers.  idx (global cell ids) and dt_cell (each cell's rung
+ * dt) may be NULL: ids 0..n-1 and dt_global. */
+typedef struct fvm_snapshot_info {
+  int chare_index;   /* thisIndex */
+  int num_elements;  /* numElements */
+  int global_n;
+  int iteration;
+  unsigned long long tsysU;
+  double dt_global;
+  double rmin[3], rmax[3];
+} fvm_snapshot_info;
+int fvm_dump_binary(fvm_ctx *ctx, const char *path, const fvm_snapshot_info *info, const int *idx,
+                    const double *dt_cell);
+
 /* ---- device-resident stepping: static mesh, every cell active, one global
  * timestep (all cells on the smallest rung of the Scheduler, dt = rung of the
  * minimum CFL step).  No counterpart entry point in the reference: it chains
