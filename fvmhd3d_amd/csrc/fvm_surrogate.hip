@@ -5,7 +5,9 @@
 //   featurisation   the HLLD inputs of a face in the face frame (rotation and
 //                   Dedner B_n/psi exactly as computeFlux, FP64), divided by two
 //                   exact symmetries of the HLLD flux: the Galilean boost
-//                   V = w_n n + u_t and the MHD scaling (rho_s, c_s)
+//                   V = w_n n + u_t and the MHD scaling (rho_s, c_s); the
+//                   network answers the residual over the central flux of the
+//                   two sides per unit normalised jump
 //   training        Adam over bf16-operand MFMA GEMMs (v_mfma_f32_16x16x32_bf16,
 //                   FP32 accumulation), one kernel per GEMM / elementwise step
 //   inference       one fused kernel: each wavefront featurises 64 faces, runs
@@ -121,10 +123,46 @@ __device__ __forceinline__ void face_scale(double gamma, const Frame &s, Scale &
   q.ut2 = 0.5 * (s.vL[2] + s.vR[2]);
 }
 
-// 16 features; returns whether the face is inside the domain part of the gate
-__device__ __forceinline__ bool features(const Frame &s, const Scale &q, int jbnd, const Gate &G, float *x) {
+// ideal-MHD flux through a face at rest, normalised units: mass, mom n, mom t1,
+// mom t2, energy, B t1, B t2 (e = thermal energy)
+__device__ __forceinline__ void physical_flux(double r, double u, double v1, double v2, double p, double e, double bn,
+                                              double b1, double b2, double *F) {
+  const double B2 = bn * bn + b1 * b1 + b2 * b2;
+  const double PT = p + 0.5 * B2;
+  const double E = e + 0.5 * r * (u * u + v1 * v1 + v2 * v2) + 0.5 * B2;
+  const double vB = u * bn + v1 * b1 + v2 * b2;
+  F[0] = r * u;
+  F[1] = r * u * u + PT - bn * bn;
+  F[2] = r * u * v1 - bn * b1;
+  F[3] = r * u * v2 - bn * b2;
+  F[4] = (E + PT) * u - bn * vB;
+  F[5] = u * b1 - bn * v1;
+  F[6] = u * b2 - bn * v2;
+}
+constexpr double J_EPS = 1e-6;
+
+// 16 features (15: the size J of the normalised jump), the central flux Fc of
+// the normalised moving-frame states; returns whether the face is inside the
+// domain part of the gate
+__device__ __forceinline__ bool features(double gamma, const Frame &s, const Scale &q, int jbnd, const Gate &G,
+                                         float *x, double *Fc, double &J) {
   const double ic = 1.0 / q.cs, iP = 1.0 / q.Ps, iB = 1.0 / q.Bs;
   const double vnL = (s.vL[0] - s.wn) * ic, vnR = (s.vR[0] - s.wn) * ic;
+  {
+    const double eL = gamma > 1.0 ? s.eL * iP : s.pL * iP, eR = gamma > 1.0 ? s.eR * iP : s.pR * iP;
+    const double L[9] = {s.rL / q.rs, vnL, (s.vL[1] - q.ut1) * ic, (s.vL[2] - q.ut2) * ic, s.pL * iP, eL,
+                         s.Bn * iB, s.BL[1] * iB, s.BL[2] * iB};
+    const double R[9] = {s.rR / q.rs, vnR, (s.vR[1] - q.ut1) * ic, (s.vR[2] - q.ut2) * ic, s.pR * iP, eR,
+                         s.Bn * iB, s.BR[1] * iB, s.BR[2] * iB};
+    double FL[7], FR[7];
+    physical_flux(L[0], L[1], L[2], L[3], L[4], L[5], L[6], L[7], L[8], FL);
+    physical_flux(R[0], R[1], R[2], R[3], R[4], R[5], R[6], R[7], R[8], FR);
+    for (int k = 0; k < 7; k++) Fc[k] = 0.5 * (FL[k] + FR[k]);
+    const int jk[7] = {0, 1, 2, 3, 4, 7, 8};
+    double j2 = 0.0;
+    for (int k = 0; k < 7; k++) j2 += sqr(R[jk[k]] - L[jk[k]]);
+    J = dsqrt(j2) + J_EPS;
+  }
   x[0] = (float)(s.rL / q.rs - 1.0);
   x[1] = (float)(s.pL * iP);
   x[2] = (float)(s.pR * iP);
@@ -141,7 +179,7 @@ __device__ __forceinline__ bool features(const Frame &s, const Scale &q, int jbn
   x[12] = (float)lr;
   x[13] = (float)lp;
   x[14] = (float)(vnR - vnL);
-  x[15] = 0.0f;
+  x[15] = (float)J;
   const bool pos = s.rL > 0.0 && s.rR > 0.0 && s.pL > 0.0 && s.pR > 0.0;
   bool ok = jbnd == NO_BOUNDARY && pos;
   ok = ok && dabs(lr) <= (double)G.max_log_rho && dabs(lp) <= (double)G.max_log_p;
@@ -178,7 +216,8 @@ __global__ void __launch_bounds__(256) k_samples(int n, double gamma, const doub
     to_frame(gamma, w3, L, R, n3, s);
     Scale q;
     face_scale(gamma, s, q);
-    keep = features(s, q, jbnd[f], G, x);
+    double Fc[7], J;
+    keep = features(gamma, s, q, jbnd[f], G, x, Fc, J);
     if (keep) {
       double F[NF];
       riemann_hlld<double>(gamma, F, s.Bn, s.wn, s.rL, s.pL, s.eL, s.vL[0], s.vL[1], s.vL[2], s.BL[1], s.BL[2], s.rR,
@@ -190,13 +229,10 @@ __global__ void __launch_bounds__(256) k_samples(int n, double gamma, const doub
       const double GE = F[ENER] - (V[0] * Gp[0] + V[1] * Gp[1] + V[2] * Gp[2]) -
                         0.5 * (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) * Gm;
       const double Gb1 = F[WBY] + s.Bn * V[1], Gb2 = F[WBZ] + s.Bn * V[2];
-      t[0] = (float)(Gm / (q.rs * q.cs));
-      t[1] = (float)(Gp[0] / q.Ps);
-      t[2] = (float)(Gp[1] / q.Ps);
-      t[3] = (float)(Gp[2] / q.Ps);
-      t[4] = (float)(GE / (q.Ps * q.cs));
-      t[5] = (float)(Gb1 / (q.Bs * q.cs));
-      t[6] = (float)(Gb2 / (q.Bs * q.cs));
+      // residual over the central flux, per unit jump
+      const double g[7] = {Gm / (q.rs * q.cs), Gp[0] / q.Ps, Gp[1] / q.Ps, Gp[2] / q.Ps, GE / (q.Ps * q.cs),
+                           Gb1 / (q.Bs * q.cs), Gb2 / (q.Bs * q.cs)};
+      for (int k = 0; k < 7; k++) t[k] = (float)((g[k] - Fc[k]) / J);
       t[7] = 0.0f;
       for (int k = 0; k < 7; k++) keep = keep && __builtin_isfinite(t[k]);
     }
@@ -296,20 +332,23 @@ __global__ void __launch_bounds__(256) k_colsum(const float *D, int rows, int co
   __syncthreads();
   if (ty == 0 && c < cols) out[c] = (part[0][tx] + part[1][tx]) + (part[2][tx] + part[3][tx]);
 }
-// loss terms and dL/dy for a batch (outputs 0..6 fluxes, 7 confidence)
-__global__ void k_loss(const float *Y, const float *T, int B, float lam, float *dY, float *loss_acc) {
+// loss terms and dL/dy for a batch (outputs 0..6 fluxes weighted by w, 7 confidence)
+struct LossW {
+  float w[7];
+};
+__global__ void k_loss(const float *Y, const float *T, int B, float lam, LossW lw, float *dY, float *loss_acc) {
   const int i = gid();
   float l = 0.0f;
   if (i < B) {
     float e2 = 0.0f, d[7];
     for (int k = 0; k < 7; k++) {
       d[k] = Y[(size_t)i * NOUT + k] - T[(size_t)i * NOUT + k];
-      e2 += d[k] * d[k];
+      e2 += lw.w[k] * d[k] * d[k];
     }
     const float c = log10f(e2 + 1e-10f);
     const float dc = Y[(size_t)i * NOUT + 7] - c;
     const float iB = 1.0f / (float)B;
-    for (int k = 0; k < 7; k++) dY[(size_t)i * NOUT + k] = 2.0f * d[k] * iB;
+    for (int k = 0; k < 7; k++) dY[(size_t)i * NOUT + k] = 2.0f * lw.w[k] * d[k] * iB;
     dY[(size_t)i * NOUT + 7] = 2.0f * lam * dc * iB;
     l = (e2 + lam * dc * dc) * iB;
   }
@@ -361,13 +400,14 @@ constexpr int OUT_B = 64 * NOUT * 4;
 struct FaceCtx {
   float nrm[3];
   float wn, ut1, ut2, rs, cs, psiL, psiR, entrL, entrR;
+  float Fc[7], J;
   float pad[2];
   double Bn, psi;
 };
 constexpr int CTX_B = 64 * (int)sizeof(FaceCtx);
 constexpr int WAVE_B = (FEAT_B > OUT_B ? FEAT_B : OUT_B) + CTX_B;
 constexpr int LDS_B = P_BYTES + (IB / 64) * WAVE_B;
-static_assert(sizeof(FaceCtx) == 72, "FaceCtx layout");
+static_assert(sizeof(FaceCtx) == 104, "FaceCtx layout");
 static_assert(LDS_B <= 160 * 1024, "LDS budget");
 
 __device__ __forceinline__ int kfeat(int c, int g, int e) { return 32 * c + (e < 4 ? 4 * g + e : 16 + 4 * g + e - 4); }
@@ -528,8 +568,11 @@ __global__ void __launch_bounds__(IB) k_infer(int n, double gamma, const double 
         to_frame(gamma, w3, L, R, n3, s);
         Scale q;
         face_scale(gamma, s, q);
-        ok = features(s, q, jbnd[f], G, x);
+        double Fc[7], J;
+        ok = features(gamma, s, q, jbnd[f], G, x, Fc, J);
         FaceCtx &cx = ctx[lane];
+        for (int k = 0; k < 7; k++) cx.Fc[k] = (float)Fc[k];
+        cx.J = (float)J;
         for (int d = 0; d < 3; d++) cx.nrm[d] = (float)n3[d];
         cx.wn = (float)s.wn;
         cx.ut1 = (float)q.ut1;
@@ -570,13 +613,15 @@ __global__ void __launch_bounds__(IB) k_infer(int n, double gamma, const double 
         const FaceCtx &cx = ctx[lane];
         const float rs = cx.rs, cs = cx.cs, Ps = rs * cs * cs, Bs = __builtin_sqrtf(Ps);
         const float V[3] = {cx.wn, cx.ut1, cx.ut2};
-        const float Gm = y[0] * rs * cs;
-        const float Gp[3] = {y[1] * Ps, y[2] * Ps, y[3] * Ps};
-        const float GE = y[4] * Ps * cs + (V[0] * Gp[0] + V[1] * Gp[1] + V[2] * Gp[2]) +
+        float g[7];
+        for (int k = 0; k < 7; k++) g[k] = cx.Fc[k] + cx.J * y[k];
+        const float Gm = g[0] * rs * cs;
+        const float Gp[3] = {g[1] * Ps, g[2] * Ps, g[3] * Ps};
+        const float GE = g[4] * Ps * cs + (V[0] * Gp[0] + V[1] * Gp[1] + V[2] * Gp[2]) +
                          0.5f * (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) * Gm;
         const float Bn = (float)cx.Bn;
         const float Gmom[3] = {Gp[0] + V[0] * Gm, Gp[1] + V[1] * Gm, Gp[2] + V[2] * Gm};
-        const float GB[3] = {-cx.wn * Bn, y[5] * Bs * cs - Bn * V[1], y[6] * Bs * cs - Bn * V[2]};
+        const float GB[3] = {-cx.wn * Bn, g[5] * Bs * cs - Bn * V[1], g[6] * Bs * cs - Bn * V[2]};
         double A[3][3];
         const double n3[3] = {(double)cx.nrm[0], (double)cx.nrm[1], (double)cx.nrm[2]};
         frame_rows(n3, A);
@@ -659,6 +704,7 @@ struct fvm_sg {
   double gamma = 5.0 / 3.0;
   std::string err;
   fvm::sg::Gate gate{0.5f, 1.0f, 3.0f, -3.0f};
+  fvm::sg::LossW lossw{{1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f}};
   Buf params, grads, am, av, pfrag, pbias;
   Buf X, T;
   long long nsamp = 0, cap = 0;
@@ -738,7 +784,7 @@ static int fwd_bwd(fvm_sg *s, int B, float lam, int slot) {
   hipLaunchKernelGGL(k_bias_act, dim3(nblk(bh)), dim3(256), 0, s->stream, H2, P + OB2, B, NH, 1);
   if ((rc = gemm(s, B, NOUT, NH, H2, NH, 1, P + OW3, NOUT, 1, Y, NOUT, 0))) return rc;
   hipLaunchKernelGGL(k_bias_act, dim3(nblk((long long)B * NOUT)), dim3(256), 0, s->stream, Y, P + OB3, B, NOUT, 0);
-  hipLaunchKernelGGL(k_loss, dim3(nblk(B)), dim3(256), 0, s->stream, Y, (const float *)s->tb.p, B, lam, dY,
+  hipLaunchKernelGGL(k_loss, dim3(nblk(B)), dim3(256), 0, s->stream, Y, (const float *)s->tb.p, B, lam, s->lossw, dY,
                      (float *)s->loss.p + slot);
   // backward
   if ((rc = gemm(s, NH, NOUT, B, H2, 1, NH, dY, NOUT, 1, G + OW3, NOUT, 0))) return rc;
@@ -848,6 +894,15 @@ int fvm_sg_get_params(fvm_sg *s, float *params, int n) {
   (void)hipSetDevice(s->device);
   SGCHK(s, hipMemcpyAsync(params, s->params.p, 4 * (size_t)NPARAM, hipMemcpyDeviceToHost, s->stream));
   SGCHK(s, hipStreamSynchronize(s->stream));
+  return FVM_OK;
+}
+
+int fvm_sg_set_loss_weights(fvm_sg *s, const double *w, int n) {
+  if (!s || !w || n != 7) return sg_fail(s, FVM_EARG, "set_loss_weights: need 7 weights");
+  for (int k = 0; k < 7; k++) {
+    if (!(w[k] > 0.0)) return sg_fail(s, FVM_EARG, "set_loss_weights: weights must be positive");
+    s->lossw.w[k] = (float)w[k];
+  }
   return FVM_OK;
 }
 

@@ -85,6 +85,7 @@ SIGNATURES = {
     "fvm_sg_set_params": [_P, _P, _I],
     "fvm_sg_get_params": [_P, _P, _I],
     "fvm_sg_set_gate": [_P, _P],
+    "fvm_sg_set_loss_weights": [_P, _P, _I],
     "fvm_sg_add_faces": [_P, _I, _P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_int)],
     "fvm_sg_clear_samples": [_P],
     "fvm_sg_num_samples": [_P, ctypes.POINTER(ctypes.c_longlong)],

@@ -142,6 +142,22 @@ class Surrogate:
         self._chk(self.L.fvm_sg_get_samples(self.h, int(first), int(n), _p(x), _p(t)))
         return x, t
 
+    def set_loss_weights(self, w):
+        w = np.ascontiguousarray(w, np.float64)
+        self._chk(self.L.fvm_sg_set_loss_weights(self.h, _p(w), 7))
+        self.loss_w = w
+
+    def standardize(self, max_rows=1_000_000):
+        """Weight each flux output by the inverse variance of its target over the
+        training set (mean weight 1), so every output is learned to a similar
+        relative precision."""
+        n = min(self.num_samples(), max_rows)
+        _, t = self.samples(0, n)
+        var = np.var(t[:, :7].astype(np.float64), axis=0) + 1e-12
+        w = 1.0 / var
+        self.set_loss_weights(w / w.mean())
+        return self.loss_w
+
     def train(self, steps, batch=8192, lr=1e-3, lam=0.05, seed=7):
         hist = np.empty(max(steps, 1), np.float32)
         self._chk(self.L.fvm_sg_train(self.h, int(steps), int(batch), float(lr), float(lam), int(seed), _p(hist)))

@@ -76,7 +76,9 @@ def evaluate(sg, faces, target_l1):
         sg.set_gate(g.max_log_rho_jump, g.max_log_p_jump, g.max_vn, thr)
         _, _, F2, nfb = sg.compute_flux(*faces)
         l1 = float(np.sum(np.abs(F2[:, :8] - F[:, :8])) / np.sum(np.abs(F[:, :8])))
-        log(f"[train] threshold {thr:5.1f}: fallback {nfb / len(F):.3f}, L1 rel {l1:.3e}")
+        per = np.sum(np.abs(F2[:, :8] - F[:, :8]), 0) / np.sum(np.abs(F[:, :8]), 0)
+        log(f"[train] threshold {thr:5.1f}: fallback {nfb / len(F):.3f}, L1 rel {l1:.3e}, per component "
+            + " ".join(f"{v:.1e}" for v in per))
         if l1 <= target_l1:
             best = (thr, l1, nfb / len(F))
             break
@@ -109,6 +111,8 @@ def main():
     n = harvest(sg, mb, Wb, G.LIMITER_SMOOTH, (1, 5, 12, 20), args.faces_per_snapshot, rng)
     log(f"[train] {n} samples in {time.perf_counter() - t0:.1f} s")
 
+    w = sg.standardize()
+    log(f"[train] loss weights {np.round(w, 3)}")
     t1 = time.perf_counter()
     hist = []
     # piecewise-constant learning rate: lr, lr/3, lr/10, lr/30 over the four quarters
@@ -129,7 +133,7 @@ def main():
     thr, l1, fbf = evaluate(sg, faces, args.target_l1)
     ms, nfb = sg.time_flux(*faces, iters=10)
     nf = len(faces[1])
-    sg.save(args.out, loss=hist[-100:], samples=np.int64(n))
+    sg.save(args.out, loss=hist[-100:], samples=np.int64(n), loss_weights=w)
     out = {"samples": n, "train_steps": int(args.steps), "batch": args.batch, "train_s": round(train_s, 1),
            "loss_first": float(np.mean(hist[:20])), "loss_last": float(np.mean(hist[-20:])),
            "held_out_faces": nf, "confidence_threshold": thr, "accepted_l1_rel": l1, "fallback_fraction": fbf,

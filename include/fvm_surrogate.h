@@ -9,9 +9,12 @@
  * FP32 accumulation on the matrix cores.  Inputs are the HLLD inputs of a face
  * in the face frame after a Galilean boost (face speed along the normal, mean
  * tangential velocity) and the MHD scaling (density, fast-speed scale) are
- * divided out; outputs 0..6 are the normalised moving-frame fluxes (mass, three
- * momenta, energy, two tangential induction components), output 7 predicts
- * log10 of the squared error of outputs 0..6 (the confidence gate).  The
+ * divided out, plus the size J of the normalised jump between the sides.
+ * Outputs 0..6 are (G' - Fc)/J for the normalised moving-frame fluxes G' (mass,
+ * three momenta, energy, two tangential induction components), Fc the central
+ * flux of the two sides: the flux error scales with the jump.  Output 7
+ * predicts log10 of the weighted squared error of outputs 0..6 (the
+ * confidence gate).  The
  * normal induction flux -w_n B_n, the Dedner B_n/psi, and the psi and entropy
  * fluxes are formed exactly as computeFlux forms them.
  *
@@ -64,9 +67,11 @@ int fvm_sg_num_samples(fvm_sg *s, long long *n);
 int fvm_sg_get_samples(fvm_sg *s, long long first, int n, float *x, float *t);
 
 /* Adam training on the device: `steps` minibatches of `batch` samples drawn by
- * a counter hash of (seed, step, slot); loss = mean |y[0..6]-t|^2 +
- * lam * (y[7] - log10(|y[0..6]-t|^2 + 1e-10))^2.  loss_hist[steps] (may be
- * NULL) receives each step's loss. */
+ * a counter hash of (seed, step, slot); with e2 = sum_k w_k (y_k - t_k)^2 over
+ * outputs 0..6, loss = mean e2 + lam * (y[7] - log10(e2 + 1e-10))^2.
+ * loss_hist[steps] (may be NULL) receives each step's loss.  The weights w
+ * (default 1) are set by fvm_sg_set_loss_weights. */
+int fvm_sg_set_loss_weights(fvm_sg *s, const double *w, int n);
 int fvm_sg_train(fvm_sg *s, int steps, int batch, double lr, double lam, unsigned long long seed, float *loss_hist);
 
 /* One forward and backward pass over the given samples, no update: network

@@ -20,6 +20,12 @@ flux have been divided out:
 
 Both are pinned in tests/test_surrogate.py against the oracle's HLLD.
 
+Targets are residuals: with the normalised moving-frame states of both sides,
+their central flux Fc = (F(L) + F(R))/2 and the size J of the normalised jump,
+the network learns (G' - Fc)/J (G' the normalised moving-frame HLLD flux), so
+its error in the flux scales with the jump and vanishes for smooth flow; J is
+also feature 15.
+
 Network: 16 -> 128 -> 128 -> 128 -> 8, ReLU, weights W[in][out]; every GEMM
 operand is rounded to bfloat16 (round-to-nearest-even) and products are
 accumulated in higher precision, as the device kernels do (FP32 there, FP64
@@ -32,6 +38,7 @@ from __future__ import annotations
 import numpy as np
 
 N_IN, N_H, N_OUT = 16, 128, 8
+J_EPS = 1e-6
 LAYERS = ((N_IN, N_H), (N_H, N_H), (N_H, N_H), (N_H, N_OUT))
 CONF_EPS = 1e-10
 
@@ -102,9 +109,31 @@ def featurize(gamma, wij, Wi, Wj, normal):
     x[:, 12] = np.log(rR / rL)
     x[:, 13] = np.log(pR / pL)
     x[:, 14] = vnR - vnL
+    # normalised moving-frame states and their central flux
+    eL = Wi[:, 4] / Ps if gamma > 1.0 else pL / Ps
+    eR = Wj[:, 4] / Ps if gamma > 1.0 else pR / Ps
+    sideL = (rL / rs, vnL, (vL[:, 1] - ut[:, 0]) / cs, (vL[:, 2] - ut[:, 1]) / cs, pL / Ps, eL,
+             Bn / Bs, BL[:, 1] / Bs, BL[:, 2] / Bs)
+    sideR = (rR / rs, vnR, (vR[:, 1] - ut[:, 0]) / cs, (vR[:, 2] - ut[:, 1]) / cs, pR / Ps, eR,
+             Bn / Bs, BR[:, 1] / Bs, BR[:, 2] / Bs)
+    Fc = 0.5 * (physical_flux(*sideL) + physical_flux(*sideR))
+    dl = np.stack([sideR[k] - sideL[k] for k in (0, 1, 2, 3, 4, 7, 8)], 1)
+    J = np.sqrt((dl * dl).sum(1)) + J_EPS
+    x[:, 15] = J
     ctx = dict(A=A, wn=wn, ut=ut, rs=rs, cs=cs, Ps=Ps, Bs=Bs, Bn=Bn, psi=psi, psiL=psiL, psiR=psiR,
-               entrL=Wi[:, 9], entrR=Wj[:, 9])
+               entrL=Wi[:, 9], entrR=Wj[:, 9], Fc=Fc, J=J)
     return x, ctx
+
+
+def physical_flux(r, u, v1, v2, p, e, bn, b1, b2):
+    """Ideal-MHD flux through a face at rest with normal n, normalised units
+    (mass, mom n, mom t1, mom t2, energy, B t1, B t2); e is the thermal energy."""
+    B2 = bn * bn + b1 * b1 + b2 * b2
+    PT = p + 0.5 * B2
+    E = e + 0.5 * r * (u * u + v1 * v1 + v2 * v2) + 0.5 * B2
+    vB = u * bn + v1 * b1 + v2 * b2
+    return np.stack([r * u, r * u * u + PT - bn * bn, r * u * v1 - bn * b1, r * u * v2 - bn * b2,
+                     (E + PT) * u - bn * vB, u * b1 - bn * v1, u * b2 - bn * v2], 1)
 
 
 def targets(flux_lab, ctx):
@@ -121,22 +150,23 @@ def targets(flux_lab, ctx):
     GpE = GE - (V * Gpmom).sum(1) - 0.5 * (V * V).sum(1) * Gpm
     GpBt = GB[:, 1:] + ctx["Bn"][:, None] * V[:, 1:]
     rs, cs, Ps, Bs = ctx["rs"], ctx["cs"], ctx["Ps"], ctx["Bs"]
-    y = np.empty((len(Gm), 7))
-    y[:, 0] = Gpm / (rs * cs)
-    y[:, 1:4] = Gpmom / Ps[:, None]
-    y[:, 4] = GpE / (Ps * cs)
-    y[:, 5:7] = GpBt / (Bs * cs)[:, None]
-    return y
+    g = np.empty((len(Gm), 7))
+    g[:, 0] = Gpm / (rs * cs)
+    g[:, 1:4] = Gpmom / Ps[:, None]
+    g[:, 4] = GpE / (Ps * cs)
+    g[:, 5:7] = GpBt / (Bs * cs)[:, None]
+    return (g - ctx["Fc"]) / ctx["J"][:, None]
 
 
 def denormalize(y, ctx):
     """Network outputs 0..6 -> (psi, Bn, flux[n,10]) exactly as computeFlux assembles them."""
     rs, cs, Ps, Bs = ctx["rs"], ctx["cs"], ctx["Ps"], ctx["Bs"]
     V = np.concatenate([ctx["wn"][:, None], ctx["ut"]], 1)
-    Gpm = y[:, 0] * rs * cs
-    Gpmom = y[:, 1:4] * Ps[:, None]
-    GpE = y[:, 4] * Ps * cs
-    GpBt = y[:, 5:7] * (Bs * cs)[:, None]
+    g = ctx["Fc"] + ctx["J"][:, None] * y
+    Gpm = g[:, 0] * rs * cs
+    Gpmom = g[:, 1:4] * Ps[:, None]
+    GpE = g[:, 4] * Ps * cs
+    GpBt = g[:, 5:7] * (Bs * cs)[:, None]
     Gmom = Gpmom + V * Gpm[:, None]
     GE = GpE + (V * Gpmom).sum(1) + 0.5 * (V * V).sum(1) * Gpm
     GB = np.empty((len(y), 3))
@@ -181,17 +211,19 @@ def forward(P, x, round_bf16=True):
     return h, cache
 
 
-def loss_and_dy(y, t, lam):
-    """Mean over the batch of |y[:7]-t|^2 + lam (y[7] - log10(|y[:7]-t|^2 + eps))^2,
-    the confidence target held fixed.  Returns (loss, dL/dy)."""
+def loss_and_dy(y, t, lam, w=None):
+    """Mean over the batch of e2 + lam (y[7] - log10(e2 + eps))^2 with
+    e2 = sum_k w_k (y[k] - t[k])^2 over outputs 0..6, the confidence target held
+    fixed.  Returns (loss, dL/dy)."""
     B = len(y)
+    w = np.ones(7) if w is None else np.asarray(w, np.float64)
     d = y[:, :7] - t
-    e2 = (d * d).sum(1)
+    e2 = (w * d * d).sum(1)
     c = np.log10(e2 + CONF_EPS)
     dc = y[:, 7] - c
     loss = float((e2 + lam * dc * dc).sum() / B)
     dy = np.zeros_like(y)
-    dy[:, :7] = 2.0 * d / B
+    dy[:, :7] = 2.0 * w * d / B
     dy[:, 7] = 2.0 * lam * dc / B
     return loss, dy
 
