@@ -88,6 +88,39 @@ def cpu_baseline(st, lim, budget_s=30.0):
                       f"oracle/fvm_oracle.c FP64, {dt:.1f} s"}
 
 
+def surrogate_bench(iters, n=128, nsteps=16, max_faces=4_000_000):
+    """Surrogate + FP64 fallback against the FP64 and FP32 HLLD on the computeFlux
+    inputs of the 128^3 MHD blast after 16 steps (held out from training), with
+    the trained parameters fvmhd3d_amd/data/hlld_surrogate.npz."""
+    from fvmhd3d_amd import gpu as G
+    from fvmhd3d_amd import mesh as M
+    from fvmhd3d_amd import surrogate as S
+    pos = M.uniform_random(n ** 3, seed=123)
+    m = M.build_periodic(pos[M.morton_order(pos, (1.0, 1.0, 1.0))])
+    W, t = S.evolve(m, M.ic_blast(m.pos), G.LIMITER_SMOOTH, nsteps)
+    faces = S.face_states(m, W, G.LIMITER_SMOOTH, t_global=t, count=max_faces)
+    nf = len(faces[1])
+    sg = S.Surrogate.load()
+    ctx = G.Context()
+    _, _, F = ctx.compute_flux(*faces)
+    ctx.close()
+    _, _, F2, nfb = sg.compute_flux(*faces)
+    ans = np.any(F2 != F, axis=1)
+    l1 = float(np.sum(np.abs(F2[:, :8] - F[:, :8])) / np.sum(np.abs(F[:, :8])))
+    per = (np.sum(np.abs(F2[:, :8] - F[:, :8]), 0) / np.sum(np.abs(F[:, :8]), 0)).tolist()
+    ms, _ = sg.time_flux(*faces, iters=max(iters, 1))
+    sg.close()
+    bytes_face = (3 + 10 + 10 + 3) * 8 + 4 + 12 * 8
+    return {"metric": "computeFlux faces/s: learned surrogate + FP64 fallback vs FP64 HLLD", "unit": "faces/s",
+            "value": nf / (ms[0] * 1e-3), "faces": nf, "fallback_fraction": nfb / nf,
+            "answered_by_network": int(ans.sum()), "l1_rel_vs_fp64": l1, "l1_rel_per_component": per,
+            "ms": {"surrogate_plus_fallback": ms[0], "hlld_fp64": ms[1], "hlld_fp32": ms[2]},
+            "faces_per_s": {"hlld_fp64": nf / (ms[1] * 1e-3), "hlld_fp32": nf / (ms[2] * 1e-3)},
+            "speedup_vs_fp64": ms[1] / ms[0],
+            "hbm_GBps_fp64_hlld": nf * bytes_face / (ms[1] * 1e-3) / 1e9,
+            "workload": f"mhd_blast_{n}^3 after {nsteps} steps, first {nf} active faces"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +130,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--copy-only", action="store_true", help="only the stream-copy probe (counter calibration)")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="the learned HLLD surrogate against the FP64/FP32 HLLD on harvested blast faces")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -104,6 +139,9 @@ def main():
         sys.exit(2)
 
     from fvmhd3d_amd import gpu as G
+    if args.surrogate:
+        print(json.dumps(surrogate_bench(args.steps)), flush=True)
+        return
     if args.copy_only:
         c = G.Context()
         gb = c.copy_bandwidth(1 << 30, 10)
@@ -148,7 +186,7 @@ def main():
     k2_b = k2_bytes(m.ncell, nfe, 4)
     achieved = k2_b / (k2_ms * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_flux_apply_{args.workload.replace('^', '')}.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_flux_apply_{args.workload.replace('^3', '')}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")
