@@ -74,17 +74,24 @@ enum { R_W = 0, R_T = 10, R_G = 20 };
 // orientation s1 -> s2; FP64 also carries the reconstruction centroid of the
 // own side as the reference forms it.
 template <typename T> struct FaceEndRec;
+// The unit normal and the area are formed once, in FP64, when the records are
+// built (area = |face.n|, n = face.n / area as the reference forms them): the
+// kernels skip a square root and a reciprocal per face-end, and the FP32 path
+// never squares an area-vector component (faces of 1e-20 would underflow).
 template <> struct alignas(16) FaceEndRec<float> {
   int nbr;
-  float fn[3];
+  float nrm[3];
   float drL[3];
-  float pad;
+  float area;
 };
+// FP64: area carries the sign of cen . face.n (the orientation of the
+// reconstruction normal, computeReconstruction.cpp:113-123)
 template <> struct alignas(16) FaceEndRec<double> {
   int nbr, pad;
-  double fn[3];
+  double nrm[3];
   double drL[3];
   double cen[3];
+  double area;
 };
 
 template <typename T> struct FusedArgs {
@@ -158,35 +165,37 @@ template <typename T> __device__ __forceinline__ T ntl(const T *p) { return __bu
 template <typename T> __device__ __forceinline__ void nts(T v, T *p) { __builtin_nontemporal_store(v, p); }
 
 template <typename T>
-__device__ __forceinline__ void load_fe(const FusedArgs<T> &A, int e, FE<T> &g) {
-  const FaceEndRec<T> r = A.fe[e];
+__device__ __forceinline__ void decode_fe(const FaceEndRec<T> &r, FE<T> &g) {
   g.isL = r.nbr >= 0;
   g.j = g.isL ? r.nbr : ~r.nbr;
-  const T fx = r.fn[0], fy = r.fn[1], fz = r.fn[2];
-  g.area = fsqrt(fx * fx + fy * fy + fz * fz);
-  const T ia = frcp(g.area);
-  g.nrm[0] = fx * ia;
-  g.nrm[1] = fy * ia;
-  g.nrm[2] = fz * ia;
-  for (int d = 0; d < 3; d++) g.drL[d] = r.drL[d];
+  for (int d = 0; d < 3; d++) {
+    g.nrm[d] = r.nrm[d];
+    g.drL[d] = r.drL[d];
+  }
   const T dsh = g.drL[0] * g.nrm[0] + g.drL[1] * g.nrm[1] + g.drL[2] * g.nrm[2];
   const T dsl = T(2.0) * dsh;
   for (int d = 0; d < 3; d++) g.drR[d] = g.drL[d] - g.nrm[d] * dsl;
   if constexpr (std::is_same<T, double>::value) {
     // the reference's reconstruction geometry (computeReconstruction.cpp:97-123)
-    for (int d = 0; d < 3; d++) g.cen[d] = r.cen[d];
-    const T cn = g.cen[0] * fx + g.cen[1] * fy + g.cen[2] * fz;
-    const T sc = (cn < 0.0) ? (-1.0 / g.area) : (1.0 / g.area);
-    g.ncen[0] = fx * sc;
-    g.ncen[1] = fy * sc;
-    g.ncen[2] = fz * sc;
+    g.area = dabs(r.area);
+    for (int d = 0; d < 3; d++) {
+      g.cen[d] = r.cen[d];
+      g.ncen[d] = r.area < 0.0 ? -g.nrm[d] : g.nrm[d];
+    }
   } else {
+    g.area = r.area;
     for (int d = 0; d < 3; d++) {
       g.cen[d] = g.isL ? g.drL[d] : g.drR[d];
       g.ncen[d] = g.isL ? g.nrm[d] : -g.nrm[d];
     }
   }
 }
+
+template <typename T>
+__device__ __forceinline__ void load_fe(const FusedArgs<T> &A, int e, FE<T> &g) {
+  decode_fe(A.fe[e], g);
+}
+__device__ __forceinline__ int fe_nbr(int nbr) { return nbr >= 0 ? nbr : ~nbr; }
 
 // face velocity with Springel's correction (computeFluidUpdate.cpp:194-207), canonical orientation
 template <typename T, bool MOVING>
@@ -285,12 +294,18 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
 
     const int deg = A.deg[c];
     const int e0 = A.gbase[c >> 6] + (c & 63);
+    // the next face-end record is loaded one iteration ahead, so the neighbour
+    // gather of an iteration issues without waiting for its own record
+    FaceEndRec<T> rn;
+    if (deg > 0) rn = A.fe[e0];
     for (int q = 0, e = e0; q < deg; q++, e += 64) {
-      FE<T> fe;
-      load_fe(A, e, fe);
-      const int j = fe.j;
+      const FaceEndRec<T> rc0 = rn;
+      if (q + 1 < deg) rn = A.fe[e + 64];
+      const int j = fe_nbr(rc0.nbr);
       T Wj[NF];
       ldv<NF>(A.rec + (size_t)j * REC + R_W, Wj);
+      FE<T> fe;
+      decode_fe(rc0, fe);
 
       // --- reconstruction sums (computeReconstruction.cpp:108-149)
       {
@@ -461,9 +476,14 @@ __global__ void __launch_bounds__(FB) k_limit(FusedArgs<T> A) {
   for (int k = 0; k < NF; k++) lm[k] = T(1.0);
   const T dtt = (T)(dtI - dth);
   const int deg = A.deg[c];
-  for (int q = 0, e = A.gbase[c >> 6] + (c & 63); q < deg; q++, e += 64) {
+  const int e0 = A.gbase[c >> 6] + (c & 63);
+  FaceEndRec<T> rn;
+  if (deg > 0) rn = A.fe[e0];
+  for (int q = 0, e = e0; q < deg; q++, e += 64) {
+    const FaceEndRec<T> rc0 = rn;
+    if (q + 1 < deg) rn = A.fe[e + 64];
     FE<T> fe;
-    load_fe(A, e, fe);
+    decode_fe(rc0, fe);
     const T *dr = fe.isL ? fe.drL : fe.drR;
     for (int k = 0; k < NF; k++) {
       const T dW = gg[k] * dr[0] + gg[NF + k] * dr[1] + gg[2 * NF + k] * dr[2] + tv[k] * dtt;
@@ -511,18 +531,23 @@ __global__ void __launch_bounds__(FB) k_flux_apply(FusedArgs<T> A) {
   bool bad = false;
 
   const int deg = A.deg[c];
-  for (int q = 0, e = A.gbase[c >> 6] + (c & 63); q < deg; q++, e += 64) {
-    FE<T> fe;
-    load_fe(A, e, fe);
-    const int j = fe.j;
+  const int e0 = A.gbase[c >> 6] + (c & 63);
+  FaceEndRec<T> rn;
+  if (deg > 0) rn = A.fe[e0];
+  for (int q = 0, e = e0; q < deg; q++, e += 64) {
+    const FaceEndRec<T> rc0 = rn;
+    if (q + 1 < deg) rn = A.fe[e + 64];
+    const int j = fe_nbr(rc0.nbr);
     // neighbour face state, extrapolated as soon as it is loaded
     T Wn[NF];
+    FE<T> fe;
     {
       const T *rj = A.rec + (size_t)j * REC;
       T Wj[NF], tj[NF], gj[3 * NF];
       ldv<NF>(rj + R_W, Wj);
       ldv<NF>(rj + R_T, tj);
       ldv<3 * NF>(rj + R_G, gj);
+      decode_fe(rc0, fe);
       const T *dr = fe.isL ? fe.drR : fe.drL;
       for (int k = 0; k < NF; k++) Wn[k] = Wj[k] + (gj[k] * dr[0] + gj[NF + k] * dr[1] + gj[2 * NF + k] * dr[2] + tj[k] * dti);
     }
@@ -656,9 +681,14 @@ __global__ void k_build_faceends(int n, const int *perm, const int *inv, const i
     const bool isL = a == o;
     FaceEndRec<T> r;
     r.nbr = isL ? inv[b] : ~inv[a];
+    const double fx = FLD(fnv, 0, f, nface), fy = FLD(fnv, 1, f, nface), fz = FLD(fnv, 2, f, nface);
+    const double area = dsqrt(fx * fx + fy * fy + fz * fz);
+    const double ia = 1.0 / area;
+    const double fn[3] = {fx, fy, fz};
+    double cn = 0.0;
     for (int d = 0; d < 3; d++) {
       const double cf = FLD(fc, d, f, nface);
-      r.fn[d] = (T)FLD(fnv, d, f, nface);
+      r.nrm[d] = (T)(fn[d] * ia);
       r.drL[d] = (T)(cf - FLD(pos, d, a, n));
       if constexpr (std::is_same<T, double>::value) {
         double v = cf - FLD(pos, d, o, n);
@@ -667,7 +697,13 @@ __global__ void k_build_faceends(int n, const int *perm, const int *inv, const i
         r.cen[d] = v;
       }
     }
-    r.pad = 0;
+    if constexpr (std::is_same<T, double>::value) {
+      r.pad = 0;
+      cn = r.cen[0] * fx + r.cen[1] * fy + r.cen[2] * fz;
+      r.area = cn < 0.0 ? -area : area;
+    } else {
+      r.area = (float)area;
+    }
     fe[slot] = r;
   }
 }
