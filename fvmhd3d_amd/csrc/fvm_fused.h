@@ -19,9 +19,9 @@
 // the reference's) or T = float (FP32 face path); accumulation into conserved
 // variables, pressure recovery and the CFL reduction are FP64 in both.
 //
-// Layout (SELL-C-sigma, C = 64 = one wavefront, sigma = 512 cells): the mesh's
+// Layout (SELL-C-sigma, C = 64 = one wavefront, sigma = 256 cells): the mesh's
 // cells, expected in a space-filling-curve order, are renumbered inside each
-// window of 512 consecutive cells by decreasing face count, so the 64 cells of
+// window of 256 consecutive cells by decreasing face count, so the 64 cells of
 // a wavefront have nearly equal face counts and the per-face-end loop wastes
 // few lanes.  Face-end k of the 64 cells of a wavefront is stored as 64
 // consecutive records, so every record load of a wavefront is one contiguous
@@ -741,7 +741,7 @@ __global__ void k_rec_to_field(const T *rec, int off, S *dst, const int *perm, i
 // ===========================================================================
 // host side
 // ===========================================================================
-constexpr int SIGMA = 512;  // cells per sorting window (8 wavefronts)
+constexpr int SIGMA = 256;  // cells per sorting window (4 wavefronts)
 
 unsigned fused_grid(int n) {
   unsigned nb = (unsigned)((n + FB - 1) / FB);
