@@ -108,29 +108,15 @@ __device__ __forceinline__ void to_frame(double gamma, const double *wij, const 
   s.psi = (s.psiL * chR + s.psiR * chL + chR * chL * (s.BL[0] - s.BR[0])) / (chL + chR);
 }
 
-// scales and boost of a face
-struct Scale {
-  double rs, cs, Ps, Bs, ut1, ut2;
-};
-__device__ __forceinline__ void face_scale(double gamma, const Frame &s, Scale &q) {
-  q.rs = 0.5 * (s.rL + s.rR);
-  const double B2 = 0.5 * ((s.BL[0] * s.BL[0] + s.BL[1] * s.BL[1] + s.BL[2] * s.BL[2]) +
-                           (s.BR[0] * s.BR[0] + s.BR[1] * s.BR[1] + s.BR[2] * s.BR[2]));
-  q.cs = dsqrt((gamma * 0.5 * (s.pL + s.pR) + B2) / q.rs);
-  q.Ps = q.rs * q.cs * q.cs;
-  q.Bs = dsqrt(q.Ps);
-  q.ut1 = 0.5 * (s.vL[1] + s.vR[1]);
-  q.ut2 = 0.5 * (s.vL[2] + s.vR[2]);
-}
-
 // ideal-MHD flux through a face at rest, normalised units: mass, mom n, mom t1,
 // mom t2, energy, B t1, B t2 (e = thermal energy)
-__device__ __forceinline__ void physical_flux(double r, double u, double v1, double v2, double p, double e, double bn,
-                                              double b1, double b2, double *F) {
-  const double B2 = bn * bn + b1 * b1 + b2 * b2;
-  const double PT = p + 0.5 * B2;
-  const double E = e + 0.5 * r * (u * u + v1 * v1 + v2 * v2) + 0.5 * B2;
-  const double vB = u * bn + v1 * b1 + v2 * b2;
+template <typename TC>
+__device__ __forceinline__ void physical_flux(TC r, TC u, TC v1, TC v2, TC p, TC e, TC bn, TC b1, TC b2, TC *F) {
+  const TC h = TC(0.5);
+  const TC B2 = bn * bn + b1 * b1 + b2 * b2;
+  const TC PT = p + h * B2;
+  const TC E = e + h * r * (u * u + v1 * v1 + v2 * v2) + h * B2;
+  const TC vB = u * bn + v1 * b1 + v2 * b2;
   F[0] = r * u;
   F[1] = r * u * u + PT - bn * bn;
   F[2] = r * u * v1 - bn * b1;
@@ -140,51 +126,65 @@ __device__ __forceinline__ void physical_flux(double r, double u, double v1, dou
   F[6] = u * b2 - bn * v2;
 }
 constexpr double J_EPS = 1e-6;
+__device__ __forceinline__ float tlog(float x) { return __logf(x); }
+__device__ __forceinline__ double tlog(double x) { return log(x); }
 
-// 16 features (15: the size J of the normalised jump), the central flux Fc of
-// the normalised moving-frame states; returns whether the face is inside the
-// domain part of the gate
-__device__ __forceinline__ bool features(double gamma, const Frame &s, const Scale &q, int jbnd, const Gate &G,
-                                         float *x, double *Fc, double &J) {
-  const double ic = 1.0 / q.cs, iP = 1.0 / q.Ps, iB = 1.0 / q.Bs;
-  const double vnL = (s.vL[0] - s.wn) * ic, vnR = (s.vR[0] - s.wn) * ic;
-  {
-    const double eL = gamma > 1.0 ? s.eL * iP : s.pL * iP, eR = gamma > 1.0 ? s.eR * iP : s.pR * iP;
-    const double L[9] = {s.rL / q.rs, vnL, (s.vL[1] - q.ut1) * ic, (s.vL[2] - q.ut2) * ic, s.pL * iP, eL,
-                         s.Bn * iB, s.BL[1] * iB, s.BL[2] * iB};
-    const double R[9] = {s.rR / q.rs, vnR, (s.vR[1] - q.ut1) * ic, (s.vR[2] - q.ut2) * ic, s.pR * iP, eR,
-                         s.Bn * iB, s.BR[1] * iB, s.BR[2] * iB};
-    double FL[7], FR[7];
-    physical_flux(L[0], L[1], L[2], L[3], L[4], L[5], L[6], L[7], L[8], FL);
-    physical_flux(R[0], R[1], R[2], R[3], R[4], R[5], R[6], R[7], R[8], FR);
-    for (int k = 0; k < 7; k++) Fc[k] = 0.5 * (FL[k] + FR[k]);
-    const int jk[7] = {0, 1, 2, 3, 4, 7, 8};
-    double j2 = 0.0;
-    for (int k = 0; k < 7; k++) j2 += sqr(R[jk[k]] - L[jk[k]]);
-    J = dsqrt(j2) + J_EPS;
-  }
-  x[0] = (float)(s.rL / q.rs - 1.0);
-  x[1] = (float)(s.pL * iP);
-  x[2] = (float)(s.pR * iP);
+// Scales, boost and the 16 features (15: the size J of the normalised jump) of
+// a face, and the central flux Fc of its normalised moving-frame states, in
+// precision TC (float for inference: the features are rounded to bf16 anyway;
+// double for training targets, which divide by J); returns whether the face is
+// inside the domain part of the gate.  sc = {rs, cs, ut1, ut2} for the inverse map.
+template <typename TC>
+__device__ __forceinline__ bool features(double gamma64, const Frame &s, int jbnd, const Gate &G, float *x, TC *Fc,
+                                         TC &J, TC *sc) {
+  const TC gamma = (TC)gamma64, h = TC(0.5);
+  const TC rL = (TC)s.rL, rR = (TC)s.rR, pL = (TC)s.pL, pR = (TC)s.pR, wn = (TC)s.wn, Bn = (TC)s.Bn;
+  const TC vL[3] = {(TC)s.vL[0], (TC)s.vL[1], (TC)s.vL[2]}, vR[3] = {(TC)s.vR[0], (TC)s.vR[1], (TC)s.vR[2]};
+  const TC BL[3] = {(TC)s.BL[0], (TC)s.BL[1], (TC)s.BL[2]}, BR[3] = {(TC)s.BR[0], (TC)s.BR[1], (TC)s.BR[2]};
+  const TC rs = h * (rL + rR);
+  const TC B2 = h * ((BL[0] * BL[0] + BL[1] * BL[1] + BL[2] * BL[2]) + (BR[0] * BR[0] + BR[1] * BR[1] + BR[2] * BR[2]));
+  const TC cs = dsqrt(fdiv(gamma * h * (pL + pR) + B2, rs));
+  const TC Ps = rs * cs * cs, Bs = dsqrt(Ps);
+  const TC ut1 = h * (vL[1] + vR[1]), ut2 = h * (vL[2] + vR[2]);
+  sc[0] = rs;
+  sc[1] = cs;
+  sc[2] = ut1;
+  sc[3] = ut2;
+  const TC ic = frcp(cs), iP = frcp(Ps), iB = frcp(Bs), irs = frcp(rs);
+  const TC vnL = (vL[0] - wn) * ic, vnR = (vR[0] - wn) * ic;
+  const TC eL = gamma64 > 1.0 ? (TC)s.eL * iP : pL * iP, eR = gamma64 > 1.0 ? (TC)s.eR * iP : pR * iP;
+  const TC L[9] = {rL * irs, vnL, (vL[1] - ut1) * ic, (vL[2] - ut2) * ic, pL * iP, eL, Bn * iB, BL[1] * iB, BL[2] * iB};
+  const TC R[9] = {rR * irs, vnR, (vR[1] - ut1) * ic, (vR[2] - ut2) * ic, pR * iP, eR, Bn * iB, BR[1] * iB, BR[2] * iB};
+  TC FL[7], FR[7];
+  physical_flux<TC>(L[0], L[1], L[2], L[3], L[4], L[5], L[6], L[7], L[8], FL);
+  physical_flux<TC>(R[0], R[1], R[2], R[3], R[4], R[5], R[6], R[7], R[8], FR);
+  for (int k = 0; k < 7; k++) Fc[k] = h * (FL[k] + FR[k]);
+  const int jk[7] = {0, 1, 2, 3, 4, 7, 8};
+  TC j2 = TC(0);
+  for (int k = 0; k < 7; k++) j2 += sqr(R[jk[k]] - L[jk[k]]);
+  J = dsqrt(j2) + (TC)J_EPS;
+  x[0] = (float)(L[0] - TC(1));
+  x[1] = (float)L[4];
+  x[2] = (float)R[4];
   x[3] = (float)vnL;
   x[4] = (float)vnR;
-  x[5] = (float)((s.vL[1] - q.ut1) * ic);
-  x[6] = (float)((s.vL[2] - q.ut2) * ic);
-  x[7] = (float)(s.Bn * iB);
-  x[8] = (float)(s.BL[1] * iB);
-  x[9] = (float)(s.BR[1] * iB);
-  x[10] = (float)(s.BL[2] * iB);
-  x[11] = (float)(s.BR[2] * iB);
-  const double lr = log(s.rR / s.rL), lp = log(s.pR / s.pL);
+  x[5] = (float)L[2];
+  x[6] = (float)L[3];
+  x[7] = (float)L[6];
+  x[8] = (float)L[7];
+  x[9] = (float)R[7];
+  x[10] = (float)L[8];
+  x[11] = (float)R[8];
+  const TC lr = tlog(fdiv(rR, rL)), lp = tlog(fdiv(pR, pL));
   x[12] = (float)lr;
   x[13] = (float)lp;
   x[14] = (float)(vnR - vnL);
   x[15] = (float)J;
   const bool pos = s.rL > 0.0 && s.rR > 0.0 && s.pL > 0.0 && s.pR > 0.0;
   bool ok = jbnd == NO_BOUNDARY && pos;
-  ok = ok && dabs(lr) <= (double)G.max_log_rho && dabs(lp) <= (double)G.max_log_p;
-  ok = ok && dabs(vnL) <= (double)G.max_vn && dabs(vnR) <= (double)G.max_vn;
-  for (int k = 0; k < 15; k++) ok = ok && __builtin_isfinite(x[k]);
+  ok = ok && dabs(lr) <= (TC)G.max_log_rho && dabs(lp) <= (TC)G.max_log_p;
+  ok = ok && dabs(vnL) <= (TC)G.max_vn && dabs(vnR) <= (TC)G.max_vn;
+  for (int k = 0; k < 16; k++) ok = ok && __builtin_isfinite(x[k]);
   return ok;
 }
 
@@ -214,24 +214,23 @@ __global__ void __launch_bounds__(256) k_samples(int n, double gamma, const doub
     load_face(f, wij, Wi, Wj, nrm, w3, L, R, n3);
     Frame s;
     to_frame(gamma, w3, L, R, n3, s);
-    Scale q;
-    face_scale(gamma, s, q);
-    double Fc[7], J;
-    keep = features(gamma, s, q, jbnd[f], G, x, Fc, J);
+    double Fc[7], J, sc[4];
+    keep = features<double>(gamma, s, jbnd[f], G, x, Fc, J, sc);
     if (keep) {
+      const double rs = sc[0], cs = sc[1], Ps = rs * cs * cs, Bs = dsqrt(Ps);
       double F[NF];
       riemann_hlld<double>(gamma, F, s.Bn, s.wn, s.rL, s.pL, s.eL, s.vL[0], s.vL[1], s.vL[2], s.BL[1], s.BL[2], s.rR,
                            s.pR, s.eR, s.vR[0], s.vR[1], s.vR[2], s.BR[1], s.BR[2]);
       // moving-frame flux: G' = G - boost(V), V = (w_n, u_t1, u_t2)
-      const double V[3] = {s.wn, q.ut1, q.ut2};
+      const double V[3] = {s.wn, sc[2], sc[3]};
       const double Gm = F[MASS];
       const double Gp[3] = {F[MOMX] - V[0] * Gm, F[MOMY] - V[1] * Gm, F[MOMZ] - V[2] * Gm};
       const double GE = F[ENER] - (V[0] * Gp[0] + V[1] * Gp[1] + V[2] * Gp[2]) -
                         0.5 * (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) * Gm;
       const double Gb1 = F[WBY] + s.Bn * V[1], Gb2 = F[WBZ] + s.Bn * V[2];
       // residual over the central flux, per unit jump
-      const double g[7] = {Gm / (q.rs * q.cs), Gp[0] / q.Ps, Gp[1] / q.Ps, Gp[2] / q.Ps, GE / (q.Ps * q.cs),
-                           Gb1 / (q.Bs * q.cs), Gb2 / (q.Bs * q.cs)};
+      const double g[7] = {Gm / (rs * cs), Gp[0] / Ps, Gp[1] / Ps, Gp[2] / Ps, GE / (Ps * cs), Gb1 / (Bs * cs),
+                           Gb2 / (Bs * cs)};
       for (int k = 0; k < 7; k++) t[k] = (float)((g[k] - Fc[k]) / J);
       t[7] = 0.0f;
       for (int k = 0; k < 7; k++) keep = keep && __builtin_isfinite(t[k]);
@@ -566,19 +565,17 @@ __global__ void __launch_bounds__(IB) k_infer(int n, double gamma, const double 
         load_face(f, wij, Wi, Wj, nrm, w3, L, R, n3);
         Frame s;
         to_frame(gamma, w3, L, R, n3, s);
-        Scale q;
-        face_scale(gamma, s, q);
-        double Fc[7], J;
-        ok = features(gamma, s, q, jbnd[f], G, x, Fc, J);
+        float Fc[7], J, sc[4];
+        ok = features<float>(gamma, s, jbnd[f], G, x, Fc, J, sc);
         FaceCtx &cx = ctx[lane];
-        for (int k = 0; k < 7; k++) cx.Fc[k] = (float)Fc[k];
-        cx.J = (float)J;
+        for (int k = 0; k < 7; k++) cx.Fc[k] = Fc[k];
+        cx.J = J;
         for (int d = 0; d < 3; d++) cx.nrm[d] = (float)n3[d];
         cx.wn = (float)s.wn;
-        cx.ut1 = (float)q.ut1;
-        cx.ut2 = (float)q.ut2;
-        cx.rs = (float)q.rs;
-        cx.cs = (float)q.cs;
+        cx.ut1 = sc[2];
+        cx.ut2 = sc[3];
+        cx.rs = sc[0];
+        cx.cs = sc[1];
         cx.psiL = (float)s.psiL;
         cx.psiR = (float)s.psiR;
         cx.entrL = (float)s.entrL;

@@ -234,9 +234,10 @@ def test_gpu_samples_match_oracle_featurisation():
     yo = R.targets(o.compute_flux(GAMMA, wij, Wi, Wj, jb, nrm)[2], ctx)
     # the device appends in wave order: match rows by nearest features
     from scipy.spatial import cKDTree
-    d, k = cKDTree(xo[:, :15]).query(x[:, :15].astype(np.float64))
+    d, k = cKDTree(xo).query(x.astype(np.float64))
     assert len(np.unique(k)) == len(k)
-    ex = np.max(np.abs(x[:, :15] - xo[k, :15]) / (np.abs(xo[k, :15]) + 1e-6))
+    # features are formed in FP32 (they are rounded to bf16 for the network); targets in FP64
+    ex = np.max(np.abs(x - xo[k]) / (np.abs(xo[k]) + 1e-2))
     et = np.max(np.abs(t[:, :7] - yo[k]) / (_scale(yo)))
     print("features", ex, "targets", et)
     assert ex < 1e-5 and et < 1e-6
