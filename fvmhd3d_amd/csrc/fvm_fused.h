@@ -504,82 +504,41 @@ __global__ void __launch_bounds__(FB) k_limit(FusedArgs<T> A) {
 // K2: computeFluidUpdate (extrapolation + HLLD + FP64 scatter sums) and
 // computePrimitives (FP64) for the own cell
 // ---------------------------------------------------------------------------
+// one face-end, given both extrapolated face states (Wo own side, Wn
+// neighbour side): HLLD in canonical orientation, FP64 sums of flux * dt * A
+// into the own cell (computeFluidUpdate.cpp:209-307)
 template <typename T, bool MOVING>
-__global__ void __launch_bounds__(FB) k_flux_apply(FusedArgs<T> A) {
-  const int c = xcd_cell(A.n);
-  if (c >= A.n) return;
-  const int n = A.n;
-  const T g = (T)A.gamma;
-  const double g64 = A.gamma;
-  const double t0 = A.tdev[0];
-  const double dt_glob = rung_dt0(A.dtmax, __longlong_as_double((long long)*A.dtmin_bits));
-  const double tend = t0 + dt_glob;
-  // every cell has tend = t0 + dt: dtI = dtJ, dt = min(dtI, dtJ), dti = dtI - dt/2
-  const double dtI = tend - t0;
-  const double dtf = smin(dtI, dtI);
-  const T dti = (T)(dtI - 0.5 * dtf);
-
-  const T *rc = A.rec + (size_t)c * REC;
-  T Wc[NF], tc[NF], gc[3 * NF];
-  ldv<NF>(rc + R_W, Wc);
-  ldv<NF>(rc + R_T, tc);
-  ldv<3 * NF>(rc + R_G, gc);
-
-  double dU[NF], gp[3], dv = 0.0;
-  for (int k = 0; k < NF; k++) dU[k] = 0.0;
-  gp[0] = gp[1] = gp[2] = 0.0;
-  bool bad = false;
-
-  const int deg = A.deg[c];
-  const int e0 = A.gbase[c >> 6] + (c & 63);
-  FaceEndRec<T> rn;
-  if (deg > 0) rn = A.fe[e0];
-  for (int q = 0, e = e0; q < deg; q++, e += 64) {
-    const FaceEndRec<T> rc0 = rn;
-    if (q + 1 < deg) rn = A.fe[e + 64];
-    const int j = fe_nbr(rc0.nbr);
-    // neighbour face state, extrapolated as soon as it is loaded
-    T Wn[NF];
-    FE<T> fe;
-    {
-      const T *rj = A.rec + (size_t)j * REC;
-      T Wj[NF], tj[NF], gj[3 * NF];
-      ldv<NF>(rj + R_W, Wj);
-      ldv<NF>(rj + R_T, tj);
-      ldv<3 * NF>(rj + R_G, gj);
-      decode_fe(rc0, fe);
-      const T *dr = fe.isL ? fe.drR : fe.drL;
-      for (int k = 0; k < NF; k++) Wn[k] = Wj[k] + (gj[k] * dr[0] + gj[NF + k] * dr[1] + gj[2 * NF + k] * dr[2] + tj[k] * dti);
-    }
-    T Wo[NF];
-    {
-      const T *dr = fe.isL ? fe.drL : fe.drR;
-      for (int k = 0; k < NF; k++) Wo[k] = Wc[k] + (gc[k] * dr[0] + gc[NF + k] * dr[1] + gc[2 * NF + k] * dr[2] + tc[k] * dti);
-    }
-    T WL[NF], WR[NF];
-    for (int k = 0; k < NF; k++) {
-      WL[k] = fe.isL ? Wo[k] : Wn[k];
-      WR[k] = fe.isL ? Wn[k] : Wo[k];
-    }
-    bad = bad || !(WL[DENS] > T(0.0)) || !(WL[ETHM] > T(0.0)) || !(WR[DENS] > T(0.0)) || !(WR[ETHM] > T(0.0));
-    T wij[3], dV;
-    face_velocity<T, MOVING>(A, fe.isL ? c : j, fe.isL ? j : c, fe, wij, dV);
-    T psi, Bn, flux[NF];
-    compute_flux<T>(g, wij, WL, WR, NO_BOUNDARY, fe.nrm, psi, Bn, flux);
-    const double tarea = dtf * (double)fe.area;
-    if (fe.isL) {
-      for (int k = 0; k < NF; k++) dU[k] = dU[k] - (double)flux[k] * tarea;
-      for (int d = 0; d < 3; d++) gp[d] = gp[d] + (double)fe.nrm[d] * ((double)psi * tarea);
-      dv = dv + (double)Bn * tarea;
-    } else {
-      for (int k = 0; k < NF; k++) dU[k] = dU[k] + (double)flux[k] * tarea;
-      for (int d = 0; d < 3; d++) gp[d] = gp[d] - (double)fe.nrm[d] * ((double)psi * tarea);
-      dv = dv - (double)Bn * tarea;
-    }
+__device__ __forceinline__ void face_update(const FusedArgs<T> &A, int c, const FE<T> &fe, const T *Wn, const T *Wo,
+                                            double dtf, double *dU, double *gp, double &dv, bool &bad) {
+  T WL[NF], WR[NF];
+  for (int k = 0; k < NF; k++) {
+    WL[k] = fe.isL ? Wo[k] : Wn[k];
+    WR[k] = fe.isL ? Wn[k] : Wo[k];
   }
-  wave_count(&A.diag[FVM_DIAG_POSITIVITY], bad);
+  bad = bad || !(WL[DENS] > T(0.0)) || !(WL[ETHM] > T(0.0)) || !(WR[DENS] > T(0.0)) || !(WR[ETHM] > T(0.0));
+  T wij[3], dV;
+  face_velocity<T, MOVING>(A, fe.isL ? c : fe.j, fe.isL ? fe.j : c, fe, wij, dV);
+  T psi, Bn, flux[NF];
+  compute_flux<T>((T)A.gamma, wij, WL, WR, NO_BOUNDARY, fe.nrm, psi, Bn, flux);
+  const double tarea = dtf * (double)fe.area;
+  if (fe.isL) {
+    for (int k = 0; k < NF; k++) dU[k] = dU[k] - (double)flux[k] * tarea;
+    for (int d = 0; d < 3; d++) gp[d] = gp[d] + (double)fe.nrm[d] * ((double)psi * tarea);
+    dv = dv + (double)Bn * tarea;
+  } else {
+    for (int k = 0; k < NF; k++) dU[k] = dU[k] + (double)flux[k] * tarea;
+    for (int d = 0; d < 3; d++) gp[d] = gp[d] - (double)fe.nrm[d] * ((double)psi * tarea);
+    dv = dv - (double)Bn * tarea;
+  }
+}
 
-  // computePrimitives at t1 = t0 + dt (computeFluidUpdate.cpp:367-505)
+// computePrimitives at t1 = t0 + dt (computeFluidUpdate.cpp:367-505) for the
+// own cell, from the summed face terms
+template <typename T>
+__device__ __forceinline__ void cell_update(const FusedArgs<T> &A, int c, double t0, double dt_glob, const double *dU,
+                                            const double *gp, double dv, const T *tc) {
+  const int n = A.n;
+  const double g64 = A.gamma;
   const double t1 = t0 + dt_glob;
   const double dt = t1 - t0;
   const double V = ntl(A.cvol + c);
@@ -647,6 +606,210 @@ __global__ void __launch_bounds__(FB) k_flux_apply(FusedArgs<T> A) {
     Wn[k] = (T)W1[k];
   }
   stv<NF>(A.rec_next + (size_t)c * REC + R_W, Wn);
+}
+
+// the step's dt and the extrapolation time of the face states: every cell has
+// tend = t0 + dt, so dtI = dtJ, dt = min(dtI, dtJ), dti = dtI - dt/2
+struct StepTimes {
+  double t0, dt_glob, dtf, dti;
+};
+template <typename T> __device__ __forceinline__ StepTimes step_times(const FusedArgs<T> &A) {
+  StepTimes s;
+  s.t0 = A.tdev[0];
+  s.dt_glob = rung_dt0(A.dtmax, __longlong_as_double((long long)*A.dtmin_bits));
+  const double tend = s.t0 + s.dt_glob;
+  const double dtI = tend - s.t0;
+  s.dtf = smin(dtI, dtI);
+  s.dti = dtI - 0.5 * s.dtf;
+  return s;
+}
+
+// generic K2: each lane gathers its own neighbour records
+template <typename T, bool MOVING>
+__global__ void __launch_bounds__(FB) k_flux_apply(FusedArgs<T> A) {
+  const int c = xcd_cell(A.n);
+  if (c >= A.n) return;
+  const StepTimes st = step_times(A);
+  const T dti = (T)st.dti;
+
+  const T *rc = A.rec + (size_t)c * REC;
+  T Wc[NF], tc[NF], gc[3 * NF];
+  ldv<NF>(rc + R_W, Wc);
+  ldv<NF>(rc + R_T, tc);
+  ldv<3 * NF>(rc + R_G, gc);
+
+  double dU[NF], gp[3], dv = 0.0;
+  for (int k = 0; k < NF; k++) dU[k] = 0.0;
+  gp[0] = gp[1] = gp[2] = 0.0;
+  bool bad = false;
+
+  const int deg = A.deg[c];
+  const int e0 = A.gbase[c >> 6] + (c & 63);
+  FaceEndRec<T> rn;
+  if (deg > 0) rn = A.fe[e0];
+  for (int q = 0, e = e0; q < deg; q++, e += 64) {
+    const FaceEndRec<T> rc0 = rn;
+    if (q + 1 < deg) rn = A.fe[e + 64];
+    const int j = fe_nbr(rc0.nbr);
+    // neighbour face state, extrapolated as soon as it is loaded
+    T Wn[NF];
+    FE<T> fe;
+    {
+      const T *rj = A.rec + (size_t)j * REC;
+      T Wj[NF], tj[NF], gj[3 * NF];
+      ldv<NF>(rj + R_W, Wj);
+      ldv<NF>(rj + R_T, tj);
+      ldv<3 * NF>(rj + R_G, gj);
+      decode_fe(rc0, fe);
+      const T *dr = fe.isL ? fe.drR : fe.drL;
+      for (int k = 0; k < NF; k++) Wn[k] = Wj[k] + (gj[k] * dr[0] + gj[NF + k] * dr[1] + gj[2 * NF + k] * dr[2] + tj[k] * dti);
+    }
+    T Wo[NF];
+    {
+      const T *dr = fe.isL ? fe.drL : fe.drR;
+      for (int k = 0; k < NF; k++) Wo[k] = Wc[k] + (gc[k] * dr[0] + gc[NF + k] * dr[1] + gc[2 * NF + k] * dr[2] + tc[k] * dti);
+    }
+    face_update<T, MOVING>(A, c, fe, Wn, Wo, st.dtf, dU, gp, dv, bad);
+  }
+  wave_count(&A.diag[FVM_DIAG_POSITIVITY], bad);
+  cell_update<T>(A, c, st.t0, st.dt_glob, dU, gp, dv, tc);
+}
+
+// FP32 K2 with a cooperative, asynchronous neighbour gather.  At face-end
+// step q the wavefront needs 64 neighbour records.  Lane-per-record loads
+// touch 64 cache lines per instruction and saturate the vector-memory address
+// unit; here 16 lanes fetch one whole 256-byte record (lane l of load k: a
+// 16-byte slot of record 4k + l / 16), so each load instruction reads 4
+// aligned records.  The loads go straight to LDS (global_load_lds_dwordx4:
+// lane l of load k lands at byte 1024k + 16l of the wavefront's buffer, i.e.
+// record r at 256r) and are issued for step q+1 as soon as every lane has read
+// its step-q record, so the gather latency overlaps the HLLD of step q.  Slot
+// s of record r is stored at position (s + r) % 16 of the record's 256 bytes:
+// the 16 lanes served by one LDS cycle then read distinct banks.
+constexpr int GQ = 13;  // float4 of a record that the flux needs (floats 0..51)
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glb_void_t;
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// s_waitcnt immediates (gfx9 encoding: vmcnt [3:0]+[15:14], expcnt [6:4], lgkmcnt [11:8])
+constexpr int WAIT_ALL = 0;           // vmcnt(0) expcnt(0) lgkmcnt(0)
+constexpr int WAIT_LGKM = 0xC07F;     // lgkmcnt(0) only
+
+// gather of one face-end step: lane l asks for the record of neighbour jsel
+// (-1: none; record 0 is read instead so that every lane issues the same
+// loads).  buf is the wavefront's buffer (uniform).
+__device__ __forceinline__ void gather_async(const float *__restrict__ rec, int jsel, float4 *buf, int lane) {
+  const int rr = lane >> 4, sl = lane & 15;
+  int jr[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) jr[k] = __builtin_amdgcn_ds_bpermute((4 * k + rr) << 2, jsel);
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const int r = 4 * k + rr;
+    const float *src = rec + (size_t)(jr[k] < 0 ? 0 : jr[k]) * REC + 4 * ((sl - r) & 15);
+    __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)(buf + 64 * k), 16, 0, 0);
+  }
+}
+
+// face state of one side: (W + t dti) + grad . dr with explicit FMAs.  A cell
+// forms its own side and its neighbour's with the same operations, so the two
+// cells of a face build bit-identical states and fluxes.
+__device__ __forceinline__ float half_step32(float W, float t, float dti) { return __builtin_fmaf(t, dti, W); }
+__device__ __forceinline__ float face_state32(float Wh, float g0, float g1, float g2, const float *dr) {
+  return Wh + __builtin_fmaf(g2, dr[2], __builtin_fmaf(g1, dr[1], g0 * dr[0]));
+}
+
+template <bool MOVING>
+__global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
+  using T = float;
+  __shared__ float4 sbuf[FB / 64][64 * 16];
+  float4 *buf = sbuf[__builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6)];
+  const int lane = (int)__lane_id();
+  const int c = xcd_cell(A.n);
+  const bool live = c < A.n;
+  // a wavefront is 64 consecutive cells of one 64-cell group; lanes past the
+  // last cell keep loading for the others but own no face-ends
+  const int deg = live ? A.deg[c] : 0;
+  int maxdeg = deg;
+  for (int o = 32; o > 0; o >>= 1) maxdeg = max(maxdeg, __shfl_xor(maxdeg, o));
+  maxdeg = __builtin_amdgcn_readfirstlane(maxdeg);
+  if (maxdeg == 0 && !live) return;  // whole wavefront past the last cell (uniform)
+
+  // step 0's gather; every later one is issued by the loop, which waits for
+  // each before it ends, so no load into LDS is in flight when the wave exits
+  const int e0 = live ? A.gbase[c >> 6] + (c & 63) : 0;
+  FaceEndRec<T> rcur, rn;
+  if (deg > 0) rcur = A.fe[e0];
+  if (maxdeg > 0) gather_async(A.rec, deg > 0 ? fe_nbr(rcur.nbr) : -1, buf, lane);
+  if (deg > 1) rn = A.fe[e0 + 64];
+
+  const StepTimes st = step_times(A);
+  const T dti = (T)st.dti;
+  // own state at the half step, extrapolated per face-end along drL / drR
+  T Wc[NF], gc[3 * NF];
+  if (live) {
+    const T *rc = A.rec + (size_t)c * REC;
+    T tc[NF];
+    ldv<NF>(rc + R_W, Wc);
+    ldv<NF>(rc + R_T, tc);
+    ldv<3 * NF>(rc + R_G, gc);
+    for (int k = 0; k < NF; k++) Wc[k] = half_step32(Wc[k], tc[k], dti);
+  }
+  double dU[NF], gp[3], dv = 0.0;
+  for (int k = 0; k < NF; k++) dU[k] = 0.0;
+  gp[0] = gp[1] = gp[2] = 0.0;
+  bool bad = false;
+
+  for (int q = 0, e = e0; q < maxdeg; q++, e += 64) {
+    __builtin_amdgcn_s_waitcnt(WAIT_ALL);  // step q's records are in LDS
+    wave_lds_sync();
+    const bool act = q < deg;
+    FE<T> fe;
+    T Wn[NF];
+    if (act) {
+      decode_fe(rcur, fe);
+      const float4 *my = buf + 16 * lane;
+      T x[4 * GQ];
+#pragma unroll
+      for (int k = 0; k < GQ; k++) {
+        const float4 w = my[(k + lane) & 15];
+        x[4 * k] = w.x; x[4 * k + 1] = w.y; x[4 * k + 2] = w.z; x[4 * k + 3] = w.w;
+      }
+      const T *dr = fe.isL ? fe.drR : fe.drL;
+      for (int k = 0; k < NF; k++)
+        Wn[k] = face_state32(half_step32(x[R_W + k], x[R_T + k], dti), x[R_G + k], x[R_G + NF + k], x[R_G + 2 * NF + k], dr);
+    }
+    __builtin_amdgcn_s_waitcnt(WAIT_LGKM);  // every lane's reads of buf are done
+    wave_lds_sync();
+    if (q + 1 < maxdeg) {
+      const FaceEndRec<T> rnext = rn;
+      if (q + 2 < deg) rn = A.fe[e + 128];
+      gather_async(A.rec, q + 1 < deg ? fe_nbr(rnext.nbr) : -1, buf, lane);
+      if (act) {
+        T Wo[NF];
+        const T *dr = fe.isL ? fe.drL : fe.drR;
+        for (int k = 0; k < NF; k++) Wo[k] = face_state32(Wc[k], gc[k], gc[NF + k], gc[2 * NF + k], dr);
+        face_update<T, MOVING>(A, c, fe, Wn, Wo, st.dtf, dU, gp, dv, bad);
+      }
+      rcur = rnext;
+    } else if (act) {
+      T Wo[NF];
+      const T *dr = fe.isL ? fe.drL : fe.drR;
+      for (int k = 0; k < NF; k++) Wo[k] = face_state32(Wc[k], gc[k], gc[NF + k], gc[2 * NF + k], dr);
+      face_update<T, MOVING>(A, c, fe, Wn, Wo, st.dtf, dU, gp, dv, bad);
+    }
+  }
+  wave_count(&A.diag[FVM_DIAG_POSITIVITY], bad);
+  if (live) {
+    T tc[NF];  // reloaded rather than held through the face loop
+    ldv<NF>(A.rec + (size_t)c * REC + R_T, tc);
+    cell_update<T>(A, c, st.t0, st.dt_glob, dU, gp, dv, tc);
+  }
 }
 
 __global__ void k_advance(double *tdev, unsigned long long *dtmin_bits, double dtmax, double *hist, int step) {
@@ -866,7 +1029,10 @@ template <typename T, bool MOVING, int LIM> void launch_step(fvm_ctx *c, const F
   if (ev) (void)hipEventRecord(ev[1], c->stream);
   if constexpr (LIM != LIM_SHIPPED) hipLaunchKernelGGL((k_limit<T, LIM>), dim3(grid), dim3(FB), 0, c->stream, A);
   if (ev) (void)hipEventRecord(ev[2], c->stream);
-  hipLaunchKernelGGL((k_flux_apply<T, MOVING>), dim3(grid), dim3(FB), 0, c->stream, A);
+  if constexpr (std::is_same<T, float>::value)
+    hipLaunchKernelGGL((k_flux_apply_coop<MOVING>), dim3(grid), dim3(FB), 0, c->stream, A);
+  else
+    hipLaunchKernelGGL((k_flux_apply<T, MOVING>), dim3(grid), dim3(FB), 0, c->stream, A);
   if (ev) (void)hipEventRecord(ev[3], c->stream);
 }
 
