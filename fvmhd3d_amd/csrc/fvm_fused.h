@@ -291,6 +291,7 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
     double pdV = 0.0;
     for (int k = 0; k < NF; k++) gx[k] = gy[k] = gz[k] = dmn[k] = dmx[k] = pU[k] = T(0.0);
     pgp[0] = pgp[1] = pgp[2] = pdv = T(0.0);
+    T gw[3] = {T(0.0), T(0.0), T(0.0)};  // FP32 reconstruction: geometric weight of the own state
 
     const int deg = A.deg[c];
     const int e0 = A.gbase[c >> 6] + (c & 63);
@@ -312,21 +313,41 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
         const T dsh = fe.cen[0] * fe.ncen[0] + fe.cen[1] * fe.ncen[1] + fe.cen[2] * fe.ncen[2];
         const T ids = (dsh != T(0.0)) ? fdiv(T(0.5), dsh) : T(0.0);
         const T idsA = fe.area * ids;
-        T drh[3], fij[3];
-        for (int d = 0; d < 3; d++) {
-          drh[d] = fe.ncen[d] * dsh;
-          fij[d] = fe.cen[d] - drh[d];
-        }
-        for (int k = 0; k < NF; k++) {
-          const T sum = Wj[k] + Wc[k];
-          const T diff = Wj[k] - Wc[k];
-          if constexpr (LIMIT) {
-            dmn[k] = smin(dmn[k], diff);
-            dmx[k] = smax(dmx[k], diff);
+        if constexpr (std::is_same<T, float>::value) {
+          // FP32: (Wj + Wc) drh + (Wj - Wc) fij = Wj cen + Wc (2 drh - cen); the
+          // Wc part is summed as geometry once per face and applied per cell
+          T a[3];
+          for (int d = 0; d < 3; d++) {
+            a[d] = idsA * fe.cen[d];
+            gw[d] += idsA * (T(2.0) * fe.ncen[d] * dsh - fe.cen[d]);
           }
-          gx[k] += (sum * drh[0] + diff * fij[0]) * idsA;
-          gy[k] += (sum * drh[1] + diff * fij[1]) * idsA;
-          gz[k] += (sum * drh[2] + diff * fij[2]) * idsA;
+          for (int k = 0; k < NF; k++) {
+            if constexpr (LIMIT) {
+              const T diff = Wj[k] - Wc[k];
+              dmn[k] = smin(dmn[k], diff);
+              dmx[k] = smax(dmx[k], diff);
+            }
+            gx[k] += a[0] * Wj[k];
+            gy[k] += a[1] * Wj[k];
+            gz[k] += a[2] * Wj[k];
+          }
+        } else {
+          T drh[3], fij[3];
+          for (int d = 0; d < 3; d++) {
+            drh[d] = fe.ncen[d] * dsh;
+            fij[d] = fe.cen[d] - drh[d];
+          }
+          for (int k = 0; k < NF; k++) {
+            const T sum = Wj[k] + Wc[k];
+            const T diff = Wj[k] - Wc[k];
+            if constexpr (LIMIT) {
+              dmn[k] = smin(dmn[k], diff);
+              dmx[k] = smax(dmx[k], diff);
+            }
+            gx[k] += (sum * drh[0] + diff * fij[0]) * idsA;
+            gy[k] += (sum * drh[1] + diff * fij[1]) * idsA;
+            gz[k] += (sum * drh[2] + diff * fij[2]) * idsA;
+          }
         }
       }
       // --- CFL neighbour term (computeTimestep.cpp:37-54)
@@ -406,6 +427,12 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
     // --- reconstruction finish (computeReconstruction.cpp:162-239)
     {
       const double invV = 1.0 / V;
+      if constexpr (std::is_same<T, float>::value)
+        for (int k = 0; k < NF; k++) {
+          gx[k] += gw[0] * Wc[k];
+          gy[k] += gw[1] * Wc[k];
+          gz[k] += gw[2] * Wc[k];
+        }
       for (int k = 0; k < NF; k++) {
         gx[k] = (T)((double)gx[k] * invV);
         gy[k] = (T)((double)gy[k] * invV);
