@@ -438,7 +438,14 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
         gy[k] = (T)((double)gy[k] * invV);
         gz[k] = (T)((double)gz[k] * invV);
       }
-      if constexpr (LIMIT) {
+      if constexpr (LIMIT && std::is_same<T, float>::value) {
+        // FP32: k_limit evaluates this limiter together with the slope limiter
+        // from the raw differences and the unlimited gradients
+        for (int k = 0; k < NF; k++) {
+          SF(A.mm, k, c, n) = dmn[k];
+          SF(A.mm, NF + k, c, n) = dmx[k];
+        }
+      } else if constexpr (LIMIT) {
         T lm[NF];
         for (int k = 0; k < NF; k++) lm[k] = T(1.0);
         for (int q = 0, e = e0; q < deg; q++, e += 64) {
@@ -477,10 +484,86 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
 // K1b (opt-in limiters): slopeLimiter compute part + computeFluidUpdateI for
 // the global step, per cell over its own face-ends (own data only)
 // ---------------------------------------------------------------------------
+// FP32 limiters from one pass.  phi (minmod min(1, x); smooth
+// (x^2 + 2x + e)/(x^2 + x + 2 + e)) rises on [0, X1] to phi(X1) = 1 and stays
+// >= 1 beyond (X1 = 1 minmod, 2 smooth), so a limiter min(1, min_f phi(x_f))
+// with x_f = bound / d_f >= 0 equals min(1, phi(min_f x_f)), and min_f x_f
+// = min(mx / max_f d_f, mn / min_f d_f).  Faces with |d_f| <= tau = 1e-16
+// max(|mn|, |mx|) count as 1 in the reference; they never set the minimum
+// while a face beyond tau exists, and a side whose extreme is within tau is
+// ignored, as the reference ignores it.  The reconstruction limiter takes d_f
+// = grad . cen_f (computeReconstruction.cpp:214-239); the slope limiter takes
+// the reconstruction-limited lr (grad . dr_f + t dtt) (slopeLimiter.cpp:20-77);
+// in the FP32 records cen_f = dr_f.
+template <int LIM> __device__ __forceinline__ float lim_from_ratio(float x) {
+  constexpr float X1 = LIM == LIM_MINMOD ? 1.0f : 2.0f;
+  return x >= X1 ? 1.0f : phi_c<LIM, float>(x);
+}
+// min_f bound / (s d_f) over the faces beyond the flat threshold
+__device__ __forceinline__ float min_ratio(float mn, float mx, float dmin, float dmax, float s) {
+  const float inf = __builtin_huge_valf();
+  const float tau = 1.0e-16f * smax(dabs(mx), dabs(mn));
+  const float hi = dmax * s, lo = dmin * s;
+  const float a = hi > tau ? (mx > 0.0f ? fdiv(mx, hi) : 0.0f) : inf;
+  const float b = lo < -tau ? (mn < 0.0f ? fdiv(mn, lo) : 0.0f) : inf;
+  return smin(a, b);
+}
+
+template <int LIM>
+__device__ __forceinline__ void limit_fp32(const FusedArgs<float> &A, int c) {
+  const int n = A.n;
+  const double t0 = A.tdev[0];
+  const double dt_glob = rung_dt0(A.dtmax, __longlong_as_double((long long)*A.dtmin_bits));
+  const double tend = t0 + dt_glob;
+  const double dtI = tend - t0;
+  const double dth = 0.5 * smin(dtI, dtI);
+  const float dtt = (float)(dtI - dth);
+  float *rc = A.rec_w + (size_t)c * REC;
+  float tv[NF], gg[3 * NF];
+  ldv<NF>(rc + R_T, tv);
+  ldv<3 * NF>(rc + R_G, gg);
+  float dlo[NF], dhi[NF], ylo[NF], yhi[NF];
+  for (int k = 0; k < NF; k++) dlo[k] = dhi[k] = ylo[k] = yhi[k] = 0.0f;
+  const int deg = A.deg[c];
+  const int e0 = A.gbase[c >> 6] + (c & 63);
+  FaceEndRec<float> rn;
+  if (deg > 0) rn = A.fe[e0];
+  for (int q = 0, e = e0; q < deg; q++, e += 64) {
+    const FaceEndRec<float> rc0 = rn;
+    if (q + 1 < deg) rn = A.fe[e + 64];
+    FE<float> fe;
+    decode_fe(rc0, fe);
+    const float *dr = fe.isL ? fe.drL : fe.drR;
+    for (int k = 0; k < NF; k++) {
+      const float d = gg[k] * dr[0] + gg[NF + k] * dr[1] + gg[2 * NF + k] * dr[2];
+      const float y = d + tv[k] * dtt;
+      dlo[k] = smin(dlo[k], d);
+      dhi[k] = smax(dhi[k], d);
+      ylo[k] = smin(ylo[k], y);
+      yhi[k] = smax(yhi[k], y);
+    }
+  }
+  for (int k = 0; k < NF; k++) {
+    const float mn = SF(A.mm, k, c, n), mx = SF(A.mm, NF + k, c, n);
+    const float lr = lim_from_ratio<LIM>(min_ratio(mn, mx, dlo[k], dhi[k], 1.0f));
+    const float lm = lr * lim_from_ratio<LIM>(min_ratio(mn, mx, ylo[k], yhi[k], lr));
+    tv[k] *= lm;
+    gg[k] *= lm;
+    gg[NF + k] *= lm;
+    gg[2 * NF + k] *= lm;
+  }
+  stv<NF>(rc + R_T, tv);
+  stv<3 * NF>(rc + R_G, gg);
+}
+
 template <typename T, int LIM>
 __global__ void __launch_bounds__(FB) k_limit(FusedArgs<T> A) {
   const int c = xcd_cell(A.n);
   if (c >= A.n) return;
+  if constexpr (std::is_same<T, float>::value) {
+    limit_fp32<LIM>(A, c);
+    return;
+  }
   const int n = A.n;
   const double t0 = A.tdev[0];
   const double dt_glob = rung_dt0(A.dtmax, __longlong_as_double((long long)*A.dtmin_bits));
