@@ -74,6 +74,14 @@ def k1_bytes(ncell, nfe, prec_bytes, limiter):
     return ncell * per_cell + nfe * per_fe
 
 
+def limit_bytes(ncell, nfe):
+    """FP32 limiter launch: per cell read Wt + gradients (40 floats) and the neighbour min/max
+    differences (20 floats), face count and group base, write Wt + gradients back; per
+    face-end the 12-byte own-side centroid offset."""
+    per_cell = 40 * 4 + 20 * 4 + 4 + 4 + 40 * 4
+    return ncell * per_cell + nfe * 12
+
+
 def cpu_baseline(st, lim, budget_s=30.0):
     """The oracle (CPU restatement, one core) on one global step of the same mesh."""
     from oracle.oracle import Oracle
@@ -197,11 +205,13 @@ def main():
     k1_ms = ms[0] / max(nl[0], 1)
     k1_b = k1_bytes(m.ncell, nfe, 4, lim != 0)
 
+    kl_ms = ms[1] / max(nl[1], 1)
+    kl_b = limit_bytes(m.ncell, nfe) if lim != 0 else 0
     extra = {"kernels": {
         "k_cfl_predict_recon": {"ms": round(k1_ms, 4), "GB/s": round(k1_b / (k1_ms * 1e-3) / 1e9, 1)},
-        "k_limit": {"ms": round(ms[1] / max(nl[1], 1), 4)},
+        "k_limit": {"ms": round(kl_ms, 4), "GB/s": round(kl_b / (kl_ms * 1e-3) / 1e9, 1) if kl_b else None},
         "k_flux_apply": {"ms": round(k2_ms, 4), "GB/s": round(achieved, 1)}},
-        "step_GB_alg": round((k1_b + k2_b) / 1e9, 3), "stream_copy_GBps": round(copy_gbps, 1)}
+        "step_GB_alg": round((k1_b + kl_b + k2_b) / 1e9, 3), "stream_copy_GBps": round(copy_gbps, 1)}
 
     if not args.no_verify:
         c64 = fresh(G.FUSED_FP64)

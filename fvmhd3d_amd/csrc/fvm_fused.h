@@ -99,6 +99,7 @@ template <typename T> struct FusedArgs {
   const int *deg;           // [n] face count of each cell
   const int *gbase;         // [n/64] first slot of each 64-cell group in fe
   const FaceEndRec<T> *fe;  // SELL: face-end k of cell c at gbase[c/64] + 64*k + c%64
+  const float *fdr;         // FP32: own-side centroid offset of each face-end slot, [slot][3]
   const double *cvol;       // [n]
   const double *mvel;       // [3][n] generator velocity (moving mesh only)
   const T *rec;             // [n][REC] records of this step
@@ -509,6 +510,10 @@ __device__ __forceinline__ float min_ratio(float mn, float mx, float dmin, float
   return smin(a, b);
 }
 
+struct Dr3 {
+  float x, y, z;
+};
+
 template <int LIM>
 __device__ __forceinline__ void limit_fp32(const FusedArgs<float> &A, int c) {
   const int n = A.n;
@@ -526,14 +531,14 @@ __device__ __forceinline__ void limit_fp32(const FusedArgs<float> &A, int c) {
   for (int k = 0; k < NF; k++) dlo[k] = dhi[k] = ylo[k] = yhi[k] = 0.0f;
   const int deg = A.deg[c];
   const int e0 = A.gbase[c >> 6] + (c & 63);
-  FaceEndRec<float> rn;
-  if (deg > 0) rn = A.fe[e0];
+  // the own-side offsets only (12 bytes per face-end, not the 32-byte record)
+  const Dr3 *fdr = reinterpret_cast<const Dr3 *>(A.fdr);
+  Dr3 rn;
+  if (deg > 0) rn = fdr[e0];
   for (int q = 0, e = e0; q < deg; q++, e += 64) {
-    const FaceEndRec<float> rc0 = rn;
-    if (q + 1 < deg) rn = A.fe[e + 64];
-    FE<float> fe;
-    decode_fe(rc0, fe);
-    const float *dr = fe.isL ? fe.drL : fe.drR;
+    const Dr3 r = rn;
+    if (q + 1 < deg) rn = fdr[e + 64];
+    const float dr[3] = {r.x, r.y, r.z};
     for (int k = 0; k < NF; k++) {
       const float d = gg[k] * dr[0] + gg[NF + k] * dr[1] + gg[2 * NF + k] * dr[2];
       const float y = d + tv[k] * dtt;
@@ -981,6 +986,20 @@ __global__ void k_build_faceends(int n, const int *perm, const int *inv, const i
   }
 }
 
+// FP32: own-side centroid offset of every face-end slot (drL or drR as the
+// kernels decode them); unused slots of short cells stay unwritten
+__global__ void k_build_dr(int n, const int *deg, const int *gbase, const FaceEndRec<float> *fe, float *fdr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int q = 0; q < deg[i]; q++) {
+    const int slot = gbase[i >> 6] + 64 * q + (i & 63);
+    FE<float> g;
+    decode_fe(fe[slot], g);
+    const float *dr = g.isL ? g.drL : g.drR;
+    for (int d = 0; d < 3; d++) fdr[3 * (size_t)slot + d] = dr[d];
+  }
+}
+
 // SoA fields between the mesh order and the internal order
 __global__ void k_gather_soa(const double *src, double *dst, const int *perm, int n, int nk) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1028,6 +1047,7 @@ template <typename T> FusedArgs<T> fused_args(fvm_ctx *c, double dtmax) {
   A.deg = (const int *)z.deg.p;
   A.gbase = (const int *)z.gbase.p;
   A.fe = (const FaceEndRec<T> *)z.fe.p;
+  A.fdr = (const float *)z.fdr.p;
   A.cvol = (const double *)z.cvol.p;
   A.mvel = (const double *)z.mvel.p;
   A.rec = (const T *)z.rec[z.cur].p;
@@ -1106,6 +1126,12 @@ template <typename T> int fused_setup_t(fvm_ctx *c) {
                      (const double *)c->field[FVM_FIELD_POS].p, c->prm.domain_size[0], c->prm.domain_size[1],
                      c->prm.domain_size[2], (const int *)z.gbase.p, (FaceEndRec<T> *)z.fe.p);
   FVM_HIPCHK(c, hipGetLastError());
+  if constexpr (std::is_same<T, float>::value) {
+    if ((rc = fvm_ensure(c, z.fdr, 3 * sizeof(float) * z.slots))) return rc;
+    hipLaunchKernelGGL(k_build_dr, dim3(nb), dim3(256), 0, c->stream, n, (const int *)z.deg.p,
+                       (const int *)z.gbase.p, (const FaceEndRec<float> *)z.fe.p, (float *)z.fdr.p);
+    FVM_HIPCHK(c, hipGetLastError());
+  }
   struct {
     const void *src;
     DBuf *dst;

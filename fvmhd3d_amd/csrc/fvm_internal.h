@@ -41,6 +41,7 @@ struct FusedState {
   int cur = 0;
   // internal (window-sorted) cell order: perm[i] = mesh cell of internal cell i
   DBuf perm, inv, deg, gbase, fe;
+  DBuf fdr;  // FP32 path: own-side centroid offset per face-end slot (3 floats)
   DBuf U, W64, cvol, mvel;  // FP64 state and geometry in internal order
   DBuf rec[2], mm, dt_new, dtmin, tdev, hist;
   size_t slots = 0;
