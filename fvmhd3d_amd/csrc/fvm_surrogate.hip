@@ -32,6 +32,7 @@
 
 #include "../../include/fvm_gpu.h"
 #include "../../include/fvm_surrogate.h"
+#include "fvm_internal.h"
 #include "fvm_physics.h"
 
 namespace fvm {
@@ -204,8 +205,7 @@ __device__ __forceinline__ void load_face(int f, const double *wij, const double
 // training samples: features + normalised moving-frame HLLD fluxes (FP64 HLLD)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_samples(int n, double gamma, const double *wij, const double *Wi, const double *Wj, const int *jbnd,
-                          const double *nrm, Gate G, float *X, float *T, long long base, long long cap,
-                          unsigned long long *count) {
+                          const double *nrm, Gate G, float *Xs, float *Ts, int *flag) {
   const int f = gid();
   bool keep = false;
   float x[NIN], t[NOUT];
@@ -236,18 +236,25 @@ __global__ void __launch_bounds__(256) k_samples(int n, double gamma, const doub
       for (int k = 0; k < 7; k++) keep = keep && __builtin_isfinite(t[k]);
     }
   }
-  const unsigned long long m = __ballot(keep);
-  unsigned long long b0 = 0;
-  if (__lane_id() == 0 && m) b0 = atomicAdd(count, (unsigned long long)__popcll(m));
-  b0 = __shfl(b0, 0);
-  if (keep) {
-    const unsigned long long below = m & ((1ULL << __lane_id()) - 1ULL);
-    const long long i = base + (long long)(b0 + (unsigned long long)__popcll(below));
-    if (i < cap) {
-      for (int k = 0; k < NIN; k++) X[(size_t)i * NIN + k] = x[k];
-      for (int k = 0; k < NOUT; k++) T[(size_t)i * NOUT + k] = t[k];
+  // staged at the face's own index; k_pack_samples keeps the face order, so
+  // the training set (and the minibatches drawn from it) is the same every run
+  if (f < n) {
+    flag[f] = keep ? 1 : 0;
+    if (keep) {
+      for (int k = 0; k < NIN; k++) Xs[(size_t)f * NIN + k] = x[k];
+      for (int k = 0; k < NOUT; k++) Ts[(size_t)f * NOUT + k] = t[k];
     }
   }
+}
+
+// kept samples, in face order, appended at base (pos: exclusive scan of flag)
+__global__ void k_pack_samples(int n, const int *flag, const int *pos, const float *Xs, const float *Ts, float *X,
+                               float *T, long long base) {
+  const int f = gid();
+  if (f >= n || !flag[f]) return;
+  const size_t i = (size_t)(base + pos[f]);
+  for (int k = 0; k < NIN; k++) X[i * NIN + k] = Xs[(size_t)f * NIN + k];
+  for (int k = 0; k < NOUT; k++) T[i * NOUT + k] = Ts[(size_t)f * NOUT + k];
 }
 
 // ---------------------------------------------------------------------------
@@ -708,6 +715,8 @@ struct fvm_sg {
   Buf xb, tb, h[3], y, dy, d[2], loss;
   int bcap = 0;
   Buf stage, fbl, fbc;
+  Buf sx, st, sflag, spos;  // sample staging of one add_faces call
+  fvm::ScanScratch scan;
   int adam_t = 0;
   bool packed = false;
 };
@@ -866,8 +875,10 @@ void fvm_sg_destroy(fvm_sg *s) {
   if (!s) return;
   (void)hipSetDevice(s->device);
   for (Buf *b : {&s->params, &s->grads, &s->am, &s->av, &s->pfrag, &s->pbias, &s->X, &s->T, &s->xb, &s->tb, &s->h[0],
-                 &s->h[1], &s->h[2], &s->y, &s->dy, &s->d[0], &s->d[1], &s->loss, &s->stage, &s->fbl, &s->fbc})
+                 &s->h[1], &s->h[2], &s->y, &s->dy, &s->d[0], &s->d[1], &s->loss, &s->stage, &s->fbl, &s->fbc,
+                 &s->sx, &s->st, &s->sflag, &s->spos})
     sg_release(*b);
+  fvm::scan_release(s->scan);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -951,14 +962,20 @@ int fvm_sg_add_faces(fvm_sg *s, int n, const double *wij, const double *Wi, cons
   }
   double *d[8];
   if ((rc = stage_faces(s, n, wij, Wi, Wj, jbnd, normal, false, d))) return rc;
-  if ((rc = sg_ensure(s, s->fbc, sizeof(unsigned long long)))) return rc;
-  SGCHK(s, hipMemsetAsync(s->fbc.p, 0, sizeof(unsigned long long), s->stream));
+  if ((rc = sg_ensure(s, s->sx, sizeof(float) * NIN * (size_t)n)) ||
+      (rc = sg_ensure(s, s->st, sizeof(float) * NOUT * (size_t)n)) || (rc = sg_ensure(s, s->sflag, sizeof(int) * (size_t)n)) ||
+      (rc = sg_ensure(s, s->spos, sizeof(int) * (size_t)n)))
+    return rc;
   hipLaunchKernelGGL(k_samples, dim3(nblk(n)), dim3(256), 0, s->stream, n, s->gamma, d[0], d[1], d[2],
-                     (const int *)d[7], d[3], s->gate, (float *)s->X.p, (float *)s->T.p, s->nsamp, s->cap,
-                     (unsigned long long *)s->fbc.p);
+                     (const int *)d[7], d[3], s->gate, (float *)s->sx.p, (float *)s->st.p, (int *)s->sflag.p);
   SGCHK(s, hipGetLastError());
-  unsigned long long kept = 0;
-  SGCHK(s, hipMemcpyAsync(&kept, s->fbc.p, sizeof kept, hipMemcpyDeviceToHost, s->stream));
+  int kept = 0;
+  if (fvm::exclusive_scan_flags(s->scan, (const int *)s->sflag.p, (int *)s->spos.p, n, &kept, s->stream))
+    return sg_fail(s, FVM_EHIP, "add_faces: sample scan failed");
+  hipLaunchKernelGGL(k_pack_samples, dim3(nblk(n)), dim3(256), 0, s->stream, n, (const int *)s->sflag.p,
+                     (const int *)s->spos.p, (const float *)s->sx.p, (const float *)s->st.p, (float *)s->X.p,
+                     (float *)s->T.p, s->nsamp);
+  SGCHK(s, hipGetLastError());
   SGCHK(s, hipStreamSynchronize(s->stream));
   s->nsamp += (long long)kept;
   if (n_kept) *n_kept = (int)kept;

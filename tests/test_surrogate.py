@@ -172,6 +172,27 @@ def _harvest(n=14, steps=3):
 
 
 @pytest.mark.gpu
+def test_gpu_samples_and_training_are_deterministic():
+    """Samples are kept in face order and minibatches come from a counter hash,
+    so two surrogates fed the same faces hold the same training set and train
+    to the same parameters bit for bit."""
+    from fvmhd3d_amd.surrogate import Surrogate, flatten
+    faces = _harvest(n=12, steps=2)
+    out = []
+    for _ in range(2):
+        sg = Surrogate(gate=(0.5, 1.0, 3.0, 1e30))
+        sg.add_faces(*faces)
+        x, t = sg.samples()
+        sg.standardize()
+        sg.train(40, batch=2048, lr=2e-3)
+        out.append((x, t, flatten(sg.params())))
+        sg.close()
+    assert out[0][0].shape[0] > 0
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
 def test_gpu_gradients_match_oracle():
     from fvmhd3d_amd.surrogate import Surrogate
     rng = np.random.default_rng(8)
