@@ -40,6 +40,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -799,8 +800,10 @@ __global__ void __launch_bounds__(FB) k_flux_apply(FusedArgs<T> A) {
 // lane l of load k lands at byte 1024k + 16l of the wavefront's buffer, i.e.
 // record r at 256r) and are issued for step q+1 as soon as every lane has read
 // its step-q record, so the gather latency overlaps the HLLD of step q.  Slot
-// s of record r is stored at position (s + r) % 16 of the record's 256 bytes:
-// the 16 lanes served by one LDS cycle then read distinct banks.
+// s of record r is stored at position s ^ (r % 16) of the record's 256 bytes:
+// the 16 lanes served by one LDS cycle then read distinct banks, and with the
+// records 256-byte aligned a lane addresses slot s as (its base | 16 (r % 16))
+// ^ 16 s, one instruction per read.
 constexpr int GQ = 13;  // float4 of a record that the flux needs (floats 0..51)
 
 typedef __attribute__((address_space(3))) void lds_void_t;
@@ -816,8 +819,12 @@ constexpr int WAIT_ALL = 0;           // vmcnt(0) expcnt(0) lgkmcnt(0)
 constexpr int WAIT_LGKM = 0xC07F;     // lgkmcnt(0) only
 
 // gather of one face-end step: lane l asks for the record of neighbour jsel
-// (-1: none; record 0 is read instead so that every lane issues the same
-// loads).  buf is the wavefront's buffer (uniform).
+// (>= 0; lanes without a face-end ask for record 0, so that every lane issues
+// the same loads).  buf is the wavefront's buffer (uniform).  Record r = 4k +
+// l / 16 of load k has r % 16 = 4 (k % 4) + l / 16.
+// OFF32: the records span at most 4 GiB (n <= 2^24 cells), so a load
+// addresses them as the uniform base plus a 32-bit byte offset.
+template <bool OFF32>
 __device__ __forceinline__ void gather_async(const float *__restrict__ rec, int jsel, float4 *buf, int lane) {
   const int rr = lane >> 4, sl = lane & 15;
   int jr[16];
@@ -825,11 +832,19 @@ __device__ __forceinline__ void gather_async(const float *__restrict__ rec, int 
   for (int k = 0; k < 16; k++) jr[k] = __builtin_amdgcn_ds_bpermute((4 * k + rr) << 2, jsel);
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    const int r = 4 * k + rr;
-    const float *src = rec + (size_t)(jr[k] < 0 ? 0 : jr[k]) * REC + 4 * ((sl - r) & 15);
+    const unsigned slot = (unsigned)(sl ^ (4 * (k & 3) + rr));
+    const float *src;
+    if constexpr (OFF32)
+      src = reinterpret_cast<const float *>(reinterpret_cast<const char *>(rec) +
+                                            ((unsigned)jr[k] * (unsigned)(REC * sizeof(float)) + 16u * slot));
+    else
+      src = rec + 4 * slot + (size_t)(unsigned)jr[k] * REC;
     __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)(buf + 64 * k), 16, 0, 0);
   }
 }
+
+typedef float f4v_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) const f4v_t lds_cf4_t;
 
 // face state of one side: (W + t dti) + grad . dr with explicit FMAs.  A cell
 // forms its own side and its neighbour's with the same operations, so the two
@@ -839,12 +854,14 @@ __device__ __forceinline__ float face_state32(float Wh, float g0, float g1, floa
   return Wh + __builtin_fmaf(g2, dr[2], __builtin_fmaf(g1, dr[1], g0 * dr[0]));
 }
 
-template <bool MOVING>
+template <bool MOVING, bool OFF32>
 __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
   using T = float;
-  __shared__ float4 sbuf[FB / 64][64 * 16];
+  __shared__ __attribute__((aligned(256))) float4 sbuf[FB / 64][64 * 16];
   float4 *buf = sbuf[__builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6)];
   const int lane = (int)__lane_id();
+  // LDS byte address of this lane's record with its swizzle folded in
+  const unsigned myb = (unsigned)(size_t)(lds_cf4_t *)(buf + 16 * lane) | ((unsigned)(lane & 15) << 4);
   const int c = xcd_cell(A.n);
   const bool live = c < A.n;
   // a wavefront is 64 consecutive cells of one 64-cell group; lanes past the
@@ -860,7 +877,7 @@ __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
   const int e0 = live ? A.gbase[c >> 6] + (c & 63) : 0;
   FaceEndRec<T> rcur, rn;
   if (deg > 0) rcur = A.fe[e0];
-  if (maxdeg > 0) gather_async(A.rec, deg > 0 ? fe_nbr(rcur.nbr) : -1, buf, lane);
+  if (maxdeg > 0) gather_async<OFF32>(A.rec, deg > 0 ? fe_nbr(rcur.nbr) : 0, buf, lane);
   if (deg > 1) rn = A.fe[e0 + 64];
 
   const StepTimes st = step_times(A);
@@ -888,11 +905,10 @@ __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
     T Wn[NF];
     if (act) {
       decode_fe(rcur, fe);
-      const float4 *my = buf + 16 * lane;
       T x[4 * GQ];
 #pragma unroll
       for (int k = 0; k < GQ; k++) {
-        const float4 w = my[(k + lane) & 15];
+        const f4v_t w = *(lds_cf4_t *)(size_t)(myb ^ (16u * (unsigned)k));
         x[4 * k] = w.x; x[4 * k + 1] = w.y; x[4 * k + 2] = w.z; x[4 * k + 3] = w.w;
       }
       const T *dr = fe.isL ? fe.drR : fe.drL;
@@ -904,15 +920,10 @@ __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
     if (q + 1 < maxdeg) {
       const FaceEndRec<T> rnext = rn;
       if (q + 2 < deg) rn = A.fe[e + 128];
-      gather_async(A.rec, q + 1 < deg ? fe_nbr(rnext.nbr) : -1, buf, lane);
-      if (act) {
-        T Wo[NF];
-        const T *dr = fe.isL ? fe.drL : fe.drR;
-        for (int k = 0; k < NF; k++) Wo[k] = face_state32(Wc[k], gc[k], gc[NF + k], gc[2 * NF + k], dr);
-        face_update<T, MOVING>(A, c, fe, Wn, Wo, st.dtf, dU, gp, dv, bad);
-      }
+      gather_async<OFF32>(A.rec, q + 1 < deg ? fe_nbr(rnext.nbr) : 0, buf, lane);
       rcur = rnext;
-    } else if (act) {
+    }
+    if (act) {
       T Wo[NF];
       const T *dr = fe.isL ? fe.drL : fe.drR;
       for (int k = 0; k < NF; k++) Wo[k] = face_state32(Wc[k], gc[k], gc[NF + k], gc[2 * NF + k], dr);
@@ -1165,8 +1176,15 @@ template <typename T, bool MOVING, int LIM> void launch_step(fvm_ctx *c, const F
   if (ev) (void)hipEventRecord(ev[1], c->stream);
   if constexpr (LIM != LIM_SHIPPED) hipLaunchKernelGGL((k_limit<T, LIM>), dim3(grid), dim3(FB), 0, c->stream, A);
   if (ev) (void)hipEventRecord(ev[2], c->stream);
-  if constexpr (std::is_same<T, float>::value)
-    hipLaunchKernelGGL((k_flux_apply_coop<MOVING>), dim3(grid), dim3(FB), 0, c->stream, A);
+  if constexpr (std::is_same<T, float>::value) {
+    // FVM_FUSED_ADDR64 set: always the 64-bit addressing (it serves meshes past
+    // 2^24 cells; the switch lets small meshes exercise it)
+    const bool addr64 = std::getenv("FVM_FUSED_ADDR64") != nullptr;
+    if (!addr64 && (size_t)c->np * REC * sizeof(float) <= (size_t(1) << 32))
+      hipLaunchKernelGGL((k_flux_apply_coop<MOVING, true>), dim3(grid), dim3(FB), 0, c->stream, A);
+    else
+      hipLaunchKernelGGL((k_flux_apply_coop<MOVING, false>), dim3(grid), dim3(FB), 0, c->stream, A);
+  }
   else
     hipLaunchKernelGGL((k_flux_apply<T, MOVING>), dim3(grid), dim3(FB), 0, c->stream, A);
   if (ev) (void)hipEventRecord(ev[3], c->stream);

@@ -90,3 +90,16 @@ def test_fused_conservation_and_determinism():
     # Powell/Dedner source terms act on B and psi only
     for k in (0, 1, 2, 3, 4):
         assert abs(tot1[k] - tot0[k]) <= 1e-12 * mag[k] + 1e-15, (k, tot1[k] - tot0[k])
+
+
+def test_fused_fp32_addressing_variants_agree(monkeypatch):
+    """The FP32 flux kernel addresses the neighbour records with 32-bit offsets
+    up to 2^24 cells and with 64-bit addresses beyond; FVM_FUSED_ADDR64 (read
+    at every launch) forces the latter.  Both must give the same bits."""
+    m, st, lim = _case("blast", 12)
+    monkeypatch.delenv("FVM_FUSED_ADDR64", raising=False)
+    a = _run_gpu(st.copy(), lim, G.FUSED_FP32, 4, 1.0)
+    monkeypatch.setenv("FVM_FUSED_ADDR64", "1")
+    b = _run_gpu(st.copy(), lim, G.FUSED_FP32, 4, 1.0)
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
