@@ -200,7 +200,7 @@ def main():
             traffic = json.load(f).get("bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_flux_apply<float> (flux + FP64 apply)", "bytes_per_launch": k2_b,
+                "kernel": "k_flux_apply_coop<float> (flux + FP64 apply)", "bytes_per_launch": k2_b,
                 "ms_per_launch": round(k2_ms, 4)}
     k1_ms = ms[0] / max(nl[0], 1)
     k1_b = k1_bytes(m.ncell, nfe, 4, lim != 0)
