@@ -212,6 +212,10 @@ def main():
         "k_limit": {"ms": round(kl_ms, 4), "GB/s": round(kl_b / (kl_ms * 1e-3) / 1e9, 1) if kl_b else None},
         "k_flux_apply": {"ms": round(k2_ms, 4), "GB/s": round(achieved, 1)}},
         "step_GB_alg": round((k1_b + kl_b + k2_b) / 1e9, 3), "stream_copy_GBps": round(copy_gbps, 1)}
+    # the whole step against the same peak: every kernel's algorithmic bytes over the step time
+    step_gbs = (k1_b + kl_b + k2_b) / (elapsed / args.steps) / 1e9
+    extra["pipeline_roofline"] = {"achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
 
     if not args.no_verify:
         c64 = fresh(G.FUSED_FP64)
