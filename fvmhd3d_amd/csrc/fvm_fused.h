@@ -102,6 +102,7 @@ template <typename T> struct FusedArgs {
   const FaceEndRec<T> *fe;  // SELL: face-end k of cell c at gbase[c/64] + 64*k + c%64
   const float *fdr;         // FP32: own-side centroid offset of each face-end slot, [slot][3]
   const double *cvol;       // [n]
+  const double *lmin;       // [n] pow(V / (4 pi / 3), 1 / 3), formed once at setup
   const double *mvel;       // [3][n] generator velocity (moving mesh only)
   const T *rec;             // [n][REC] records of this step
   T *rec_w;                 // records written by this launch (K1: rec itself; K2: next step's)
@@ -273,7 +274,7 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
       double U0[NF], W0[NF];
       for (int k = 0; k < NF; k++) U0[k] = ntl(&SF(A.U, k, c, n));
       to_primitive(U0, V, W0);
-      h = pow(V / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
+      h = A.lmin[c];  // pow(V / (4 pi / 3), 1 / 3)
       const double Pi = pressure(g64, W0[ETHM]) * g64;
       const double B2i = W0[BX] * W0[BX] + W0[BY] * W0[BY] + W0[BZ] * W0[BZ];
       csi = dsqrt((Pi + B2i) / W0[DENS]);
@@ -409,7 +410,7 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
       Uc[WBZ] -= (double)pgp[2] * dt;
       Uc[MPSI] -= dcs2 * divB * dt;
       const double cr = 0.5;
-      const double l_min = pow(V / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
+      const double l_min = A.lmin[c];
       Uc[MPSI] *= exp(-cr * ch_sig / l_min * dt);
       double W1[NF];
       to_primitive(Uc, V * exp(pdV * dt / V), W1);
@@ -691,7 +692,7 @@ __device__ __forceinline__ void cell_update(const FusedArgs<T> &A, int c, double
                   dt * 0.5;
     }
     const double cr = 0.5;
-    const double l_min = pow(V / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
+    const double l_min = A.lmin[c];
     Uc[MPSI] *= exp(-cr * ch_sig / l_min * dt);
     to_primitive(Uc, V, W1);
     ef = W1[ETHM] <= 0.0;
@@ -1004,6 +1005,15 @@ __global__ void k_build_dr(int n, const int *deg, const int *gbase, const FaceEn
   }
 }
 
+// the cell length scale of the CFL step and the Dedner damping,
+// pow(V / (4 pi / 3), 1 / 3) (computeTimestep.cpp:17, computePredictor.cpp:319,
+// computeFluidUpdate.cpp:451): the volume is fixed on this path, so it is
+// formed once
+__global__ void k_cell_lmin(const double *cvol, double *lmin, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) lmin[i] = pow(cvol[i] / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
+}
+
 // SoA fields between the mesh order and the internal order
 __global__ void k_gather_soa(const double *src, double *dst, const int *perm, int n, int nk) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1053,6 +1063,7 @@ template <typename T> FusedArgs<T> fused_args(fvm_ctx *c, double dtmax) {
   A.fe = (const FaceEndRec<T> *)z.fe.p;
   A.fdr = (const float *)z.fdr.p;
   A.cvol = (const double *)z.cvol.p;
+  A.lmin = (const double *)z.lmin.p;
   A.mvel = (const double *)z.mvel.p;
   A.rec = (const T *)z.rec[z.cur].p;
   A.rec_w = (T *)z.rec[z.cur].p;
@@ -1149,6 +1160,9 @@ template <typename T> int fused_setup_t(fvm_ctx *c) {
                        n, q.nk);
     FVM_HIPCHK(c, hipGetLastError());
   }
+  if ((rc = fvm_ensure(c, z.lmin, nd))) return rc;
+  hipLaunchKernelGGL(k_cell_lmin, dim3(nb), dim3(256), 0, c->stream, (const double *)z.cvol.p, (double *)z.lmin.p, n);
+  FVM_HIPCHK(c, hipGetLastError());
   for (int q = 0; q < 2; q++) {
     hipLaunchKernelGGL(k_rec_from_W<T>, dim3(nb), dim3(256), 0, c->stream, (const double *)z.W64.p, (T *)z.rec[q].p, n);
     FVM_HIPCHK(c, hipGetLastError());

@@ -43,6 +43,7 @@ struct FusedState {
   DBuf perm, inv, deg, gbase, fe;
   DBuf fdr;  // FP32 path: own-side centroid offset per face-end slot (3 floats)
   DBuf U, W64, cvol, mvel;  // FP64 state and geometry in internal order
+  DBuf lmin;                // (3V/4pi)^(1/3) per cell (the volume is fixed on this path)
   DBuf rec[2], mm, dt_new, dtmin, tdev, hist;
   size_t slots = 0;
   bool timing = false;
