@@ -219,7 +219,12 @@ def main():
 
     if not args.no_verify:
         c64 = fresh(G.FUSED_FP64)
-        c64.fused_run(args.warmup + args.steps, 1.0)
+        c64.fused_run(args.warmup, 1.0)
+        c64.L.fvm_synchronize(c64.h)
+        t64 = time.perf_counter()
+        c64.fused_run(args.steps, 1.0)
+        c64.L.fvm_synchronize(c64.h)
+        extra["fp64_path_ms_per_step"] = round((time.perf_counter() - t64) * 1e3 / args.steps, 4)
         W64 = c64.download("w")
         U64 = c64.download("U")
         c64.close()

@@ -7,6 +7,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace fvm {
 
 enum { NF = 10 };
@@ -200,10 +202,20 @@ __device__ __forceinline__ void riemann_hlld(T gamma_gas, T *flux, const T Bx, c
   const T etot_R_s =
       ((S_R - velx_R) * etot_R - pT_R * velx_R + pT_s * S_M + Bx * (vB_R - vB_R_s)) * iSRmSM;
   const T dens_L_ss = dens_L_s, dens_R_ss = dens_R_s;
-  const T sDens_L_s = fsqrt(dens_L_s);
-  const T sDens_R_s = fsqrt(dens_R_s);
-  const T S_L_s = S_M - dabs(fdiv(Bx, sDens_L_s));
-  const T S_R_s = S_M + dabs(fdiv(Bx, sDens_R_s));
+  T sDens_L_s, sDens_R_s, S_L_s, S_R_s;
+  if constexpr (std::is_same<T, float>::value) {
+    // FP32: one reciprocal square root gives both sqrt(rho*) and Bx / sqrt(rho*)
+    const T rL = __builtin_amdgcn_rsqf(dens_L_s), rR = __builtin_amdgcn_rsqf(dens_R_s);
+    sDens_L_s = dens_L_s * rL;
+    sDens_R_s = dens_R_s * rR;
+    S_L_s = S_M - dabs(Bx * rL);
+    S_R_s = S_M + dabs(Bx * rR);
+  } else {
+    sDens_L_s = fsqrt(dens_L_s);
+    sDens_R_s = fsqrt(dens_R_s);
+    S_L_s = S_M - dabs(fdiv(Bx, sDens_L_s));
+    S_R_s = S_M + dabs(fdiv(Bx, sDens_R_s));
+  }
   const T idsqroot = frcp(sDens_L_s + sDens_R_s);
   const T vely_ss =
       idsqroot * (sDens_L_s * vely_L_s + sDens_R_s * vely_R_s + (By_R_s - By_L_s) * signBx);
@@ -296,8 +308,19 @@ __device__ __forceinline__ void compute_flux(const T gamma_gas, const T *wij, co
   const T ex = normal[0];
   const T ey = normal[1];
   const T ez = normal[2];
-  const T ds = fsqrt(ex * ex + ey * ey);
-  const T ids = (ds != c0) ? frcp(ds) : c0;
+  T ds, ids;
+  if constexpr (std::is_same<T, float>::value) {
+    // FP32: sin(theta) and its reciprocal from one reciprocal square root; a
+    // normal within 1e-15 of the z axis takes the axis frame (no subnormal input)
+    const T s2 = ex * ex + ey * ey;
+    const bool axis = !(s2 > T(1.0e-30));
+    const T r = __builtin_amdgcn_rsqf(axis ? c1 : s2);
+    ds = axis ? c0 : s2 * r;
+    ids = axis ? c0 : r;
+  } else {
+    ds = fsqrt(ex * ex + ey * ey);
+    ids = (ds != c0) ? frcp(ds) : c0;
+  }
   const T cosph = (ids == c0) ? c1 : ex * ids;
   const T sinph = (ids == c0) ? c0 : ey * ids;
   const T costh = ez;
@@ -367,8 +390,15 @@ __device__ __forceinline__ void compute_flux(const T gamma_gas, const T *wij, co
 
   const T cfl2 = fdiv(gamma_gas * pres_L + sqr(Bx_L) + sqr(By_L) + sqr(Bz_L), dens_L);
   const T cfr2 = fdiv(gamma_gas * pres_R + sqr(Bx_R) + sqr(By_R) + sqr(Bz_R), dens_R);
-  const T chL = fsqrt(cfl2 + c0 * sqr(velx_L - wn_ij));
-  const T chR = fsqrt(cfr2 + c0 * sqr(velx_R - wn_ij));
+  T chL, chR;
+  if constexpr (std::is_same<T, float>::value) {
+    // FP32: the reference's 0 * (v_n - w)^2 term is exactly 0 for finite states
+    chL = fsqrt(cfl2);
+    chR = fsqrt(cfr2);
+  } else {
+    chL = fsqrt(cfl2 + c0 * sqr(velx_L - wn_ij));
+    chR = fsqrt(cfr2 + c0 * sqr(velx_R - wn_ij));
+  }
 
   Bn_ij = fdiv(Bx_L * chL + Bx_R * chR + (psiL - psiR), chL + chR);
   psi_ij = fdiv(psiL * chR + psiR * chL + chR * chL * (Bx_L - Bx_R), chL + chR);
