@@ -24,6 +24,10 @@ def _case(kind, n):
     elif kind == "ot":
         pos = M.jittered_lattice(n)
         lim = G.LIMITER_SMOOTH
+    elif kind == "blast_minmod":
+        pos = M.uniform_random(n ** 3, seed=123)
+        lim = G.LIMITER_MINMOD
+        kind = "blast"
     else:
         pos = M.uniform_random(n ** 3, seed=123)
         lim = G.LIMITER_SMOOTH
@@ -46,7 +50,7 @@ def _run_gpu(st, lim, precision, nsteps, dtmax):
     return W, U, dts, t, dg
 
 
-@pytest.mark.parametrize("kind,n,nsteps", [("alfven", 12, 5), ("ot", 12, 5), ("blast", 14, 4)])
+@pytest.mark.parametrize("kind,n,nsteps", [("alfven", 12, 5), ("ot", 12, 5), ("blast", 14, 4), ("blast_minmod", 12, 4)])
 def test_fused_fp64_matches_oracle(kind, n, nsteps):
     m, st, lim = _case(kind, n)
     ref = st.copy()
@@ -63,7 +67,7 @@ def test_fused_fp64_matches_oracle(kind, n, nsteps):
     assert dg["positivity"] == o.diag["positivity"]
 
 
-@pytest.mark.parametrize("kind,n,nsteps", [("alfven", 16, 10), ("blast", 16, 10)])
+@pytest.mark.parametrize("kind,n,nsteps", [("alfven", 16, 10), ("blast", 16, 10), ("blast_minmod", 16, 10)])
 def test_fused_fp32_error_vs_fp64(kind, n, nsteps):
     m, st, lim = _case(kind, n)
     W64, U64, dts64, _, _ = _run_gpu(st.copy(), lim, G.FUSED_FP64, nsteps, 1.0)
