@@ -4,12 +4,16 @@
 //
 //   K1 k_cfl_predict_recon  one thread per cell, one pass over its face-ends:
 //        computeTimestep (CFL, FP64), computePredictor (first-order HLLD sweep
-//        + FP64 cell update -> Wt), computeReconstruction (gradients, min/max)
-//   K1b k_limit             opt-in limiters only (slopeLimiter + apply)
-//   K2 k_flux_apply         one thread per cell, one pass over its face-ends:
-//        space-time extrapolation + HLLD (computeFluidUpdate), FP64 sums of
-//        flux*dt*A in the cell's face order, then computePrimitives (FP64
-//        pressure recovery, Dedner/Powell terms, fallbacks)
+//        + FP64 cell update -> Wt), computeReconstruction (gradients, min/max;
+//        the FP64 path also applies the reconstruction limiter here)
+//   K1b k_limit             opt-in limiters only: FP64 slopeLimiter + apply;
+//        FP32 both limiters from one pass over 12-byte own-side offsets
+//   K2 k_flux_apply (FP64) / k_flux_apply_coop (FP32, neighbour records
+//        gathered cooperatively into LDS one face-end step ahead), one
+//        thread per cell, one pass over its face-ends: space-time
+//        extrapolation + HLLD (computeFluidUpdate), FP64 sums of flux*dt*A in
+//        the cell's face order, then computePrimitives (FP64 pressure recovery,
+//        Dedner/Powell terms, fallbacks)
 //   K3 k_advance            t += dt, reset the CFL minimum
 //
 // Each face is evaluated by both of its cells in the same canonical
