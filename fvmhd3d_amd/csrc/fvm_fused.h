@@ -273,25 +273,34 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
     ldv<NF>(rc + R_W, Wc);
 
     // CFL (computeTimestep.cpp:13-31), FP64 from U
-    double v0[3], csi, h;
-    {
-      double U0[NF], W0[NF];
-      for (int k = 0; k < NF; k++) U0[k] = ntl(&SF(A.U, k, c, n));
-      to_primitive(U0, V, W0);
+    double pv[3] = {0.0, 0.0, 0.0};
+    if constexpr (MOVING)
+      for (int d = 0; d < 3; d++) pv[d] = SF(A.mvel, d, c, n);
+    double v0[3], csi = 0.0, h = 0.0, dti = 0.0;
+    // the own-cell terms, from the FP64 primitives
+    auto cfl_cell = [&](const double *W0) {
       h = A.lmin[c];  // pow(V / (4 pi / 3), 1 / 3)
       const double Pi = pressure(g64, W0[ETHM]) * g64;
       const double B2i = W0[BX] * W0[BX] + W0[BY] * W0[BY] + W0[BZ] * W0[BZ];
       csi = dsqrt((Pi + B2i) / W0[DENS]);
+      const double dvr[3] = {W0[VELX] - pv[0], W0[VELY] - pv[1], W0[VELZ] - pv[2]};
+      const double vrel = dsqrt(dvr[0] * dvr[0] + dvr[1] * dvr[1] + dvr[2] * dvr[2]);
+      dti = h / (csi + vrel + 1.0 / REF_HUGE);
+    };
+    if constexpr (std::is_same<T, float>::value) {
+      // FP32 face path: the neighbour speed differences take the own velocity
+      // from the FP32 record, as the neighbours' come from theirs; U is read
+      // once, after the face loop, for the own-cell terms and the predictor
+      for (int d = 0; d < 3; d++) v0[d] = (double)Wc[VELX + d];
+    } else {
+      double U0[NF], W0[NF];
+      for (int k = 0; k < NF; k++) U0[k] = ntl(&SF(A.U, k, c, n));
+      to_primitive(U0, V, W0);
+      cfl_cell(W0);
       v0[0] = W0[VELX];
       v0[1] = W0[VELY];
       v0[2] = W0[VELZ];
     }
-    double pv[3] = {0.0, 0.0, 0.0};
-    if constexpr (MOVING)
-      for (int d = 0; d < 3; d++) pv[d] = SF(A.mvel, d, c, n);
-    const double dvr[3] = {v0[0] - pv[0], v0[1] - pv[1], v0[2] - pv[2]};
-    const double vrel = dsqrt(dvr[0] * dvr[0] + dvr[1] * dvr[1] + dvr[2] * dvr[2]);
-    double dti = h / (csi + vrel + 1.0 / REF_HUGE);
     double arg_max = -1.0;  // min_j h/(cs + sqrt(a_j)) = h/(cs + sqrt(max_j a_j)), monotone roundings
 
     T gx[NF], gy[NF], gz[NF], dmn[NF], dmx[NF], pU[NF], pgp[3], pdv;
@@ -387,16 +396,16 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
         pdV += fe.isL ? (double)dV : -(double)dV;
       }
     }
-    if (arg_max >= 0.0) dti = smin(dti, h / (csi + dsqrt(arg_max)));
-    dti = smin(dti, REF_HUGE);
-    dtn = dti * A.courant;
-
     // --- predictor cell update (computePredictor.cpp:141-379), FP64
     T tv[NF];
     {
       double U0[NF], W0[NF];
-      for (int k = 0; k < NF; k++) U0[k] = SF(A.U, k, c, n);
+      for (int k = 0; k < NF; k++) U0[k] = ntl(&SF(A.U, k, c, n));
       to_primitive(U0, V, W0);
+      if constexpr (std::is_same<T, float>::value) cfl_cell(W0);
+      if (arg_max >= 0.0) dti = smin(dti, h / (csi + dsqrt(arg_max)));
+      dti = smin(dti, REF_HUGE);
+      dtn = dti * A.courant;
       while (U0[MASS] + (double)pU[MASS] * dtn <= 0.0) dtn *= 0.5;
       const double dt = rung_dt0(A.dtmax, dtn) / 128;
       const double divB = (double)pdv;
