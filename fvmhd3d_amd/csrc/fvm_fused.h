@@ -257,8 +257,35 @@ __device__ __forceinline__ double entropy_from_ethm_d(double g, const double *W)
 // ---------------------------------------------------------------------------
 // K1: computeTimestep + computePredictor + computeReconstruction
 // ---------------------------------------------------------------------------
+// FP32 predictor kernel: a lane's reconstruction accumulators (gradient sums
+// x, y, z; neighbour min/max differences) live in its LDS slice between face-end
+// steps, so they do not hold registers through the HLLD and the kernel fits
+// three wavefronts per SIMD.  13 float4 per lane at a 208-byte stride: the 16
+// lanes of one LDS cycle hit distinct banks; 53 KiB per block, three blocks in
+// the 160 KiB of a CU.
+constexpr int K1ACC = 13;
+__device__ __forceinline__ float4 *k1_acc_slice() {
+  __shared__ float4 k1acc[FB * K1ACC];
+  return k1acc + threadIdx.x * K1ACC;
+}
+template <int NQ> __device__ __forceinline__ void acc_ld(const float4 *p, float *x) {
+#pragma unroll
+  for (int i = 0; i < NQ; i++) {
+    const float4 v = p[i];
+    x[4 * i] = v.x; x[4 * i + 1] = v.y; x[4 * i + 2] = v.z; x[4 * i + 3] = v.w;
+  }
+}
+template <int NQ> __device__ __forceinline__ void acc_st(float4 *p, const float *x) {
+#pragma unroll
+  for (int i = 0; i < NQ; i++) p[i] = make_float4(x[4 * i], x[4 * i + 1], x[4 * i + 2], x[4 * i + 3]);
+}
+template <typename T> struct K1Waves {
+  static constexpr int value = std::is_same<T, float>::value ? 3 : 1;
+};
+
 template <typename T, bool MOVING, int LIM>
-__global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
+__global__ void __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(K1Waves<T>::value)))
+k_cfl_predict_recon(FusedArgs<T> A) {
   constexpr bool LIMIT = LIM != LIM_SHIPPED;
   const int c = xcd_cell(A.n);
   const bool live = c < A.n;
@@ -308,6 +335,13 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
     for (int k = 0; k < NF; k++) gx[k] = gy[k] = gz[k] = dmn[k] = dmx[k] = pU[k] = T(0.0);
     pgp[0] = pgp[1] = pgp[2] = pdv = T(0.0);
     T gw[3] = {T(0.0), T(0.0), T(0.0)};  // FP32 reconstruction: geometric weight of the own state
+    // FP32: accumulators x[0,30) gradient sums, x[30,50) min/max, in LDS
+    constexpr int K1NQ = LIMIT ? K1ACC : 8;
+    if constexpr (std::is_same<T, float>::value) {
+      float z[4 * K1ACC];
+      for (int k = 0; k < 4 * K1ACC; k++) z[k] = 0.0f;
+      acc_st<K1NQ>(k1_acc_slice(), z);
+    }
 
     const int deg = A.deg[c];
     const int e0 = A.gbase[c >> 6] + (c & 63);
@@ -337,16 +371,23 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
             a[d] = idsA * fe.cen[d];
             gw[d] += idsA * (T(2.0) * fe.ncen[d] * dsh - fe.cen[d]);
           }
+          // accumulators from and back to the lane's LDS slice; the barrier
+          // keeps the compiler from carrying them in registers across steps
+          __asm__ __volatile__("" ::: "memory");
+          float4 *acc = k1_acc_slice();
+          float x[4 * K1ACC];
+          acc_ld<K1NQ>(acc, x);
           for (int k = 0; k < NF; k++) {
             if constexpr (LIMIT) {
               const T diff = Wj[k] - Wc[k];
-              dmn[k] = smin(dmn[k], diff);
-              dmx[k] = smax(dmx[k], diff);
+              x[3 * NF + k] = smin(x[3 * NF + k], diff);
+              x[4 * NF + k] = smax(x[4 * NF + k], diff);
             }
-            gx[k] += a[0] * Wj[k];
-            gy[k] += a[1] * Wj[k];
-            gz[k] += a[2] * Wj[k];
+            x[k] += a[0] * Wj[k];
+            x[NF + k] += a[1] * Wj[k];
+            x[2 * NF + k] += a[2] * Wj[k];
           }
+          acc_st<K1NQ>(acc, x);
         } else {
           T drh[3], fij[3];
           for (int d = 0; d < 3; d++) {
@@ -403,7 +444,21 @@ __global__ void __launch_bounds__(FB) k_cfl_predict_recon(FusedArgs<T> A) {
       for (int k = 0; k < NF; k++) U0[k] = ntl(&SF(A.U, k, c, n));
       to_primitive(U0, V, W0);
       if constexpr (std::is_same<T, float>::value) cfl_cell(W0);
-      if (arg_max >= 0.0) dti = smin(dti, h / (csi + dsqrt(arg_max)));
+      if constexpr (std::is_same<T, float>::value) {
+      __asm__ __volatile__("" ::: "memory");
+      float x[4 * K1ACC];
+      acc_ld<K1NQ>(k1_acc_slice(), x);
+      for (int k = 0; k < NF; k++) {
+        gx[k] = x[k];
+        gy[k] = x[NF + k];
+        gz[k] = x[2 * NF + k];
+        if constexpr (LIMIT) {
+          dmn[k] = x[3 * NF + k];
+          dmx[k] = x[4 * NF + k];
+        }
+      }
+    }
+    if (arg_max >= 0.0) dti = smin(dti, h / (csi + dsqrt(arg_max)));
       dti = smin(dti, REF_HUGE);
       dtn = dti * A.courant;
       while (U0[MASS] + (double)pU[MASS] * dtn <= 0.0) dtn *= 0.5;
