@@ -592,8 +592,14 @@ __device__ __forceinline__ void limit_fp32(const FusedArgs<float> &A, int c) {
   float tv[NF], gg[3 * NF];
   ldv<NF>(rc + R_T, tv);
   ldv<3 * NF>(rc + R_G, gg);
-  float dlo[NF], dhi[NF], ylo[NF], yhi[NF];
-  for (int k = 0; k < NF; k++) dlo[k] = dhi[k] = ylo[k] = yhi[k] = 0.0f;
+  // extremes of d_f = grad . dr_f over the faces; the slope limiter's
+  // y_f = d_f + t dtt differs by a face-independent term, and rounding is
+  // monotone, so max_f fl(d_f + t dtt) = fl(max_f d_f + t dtt)
+  float dmin[NF], dmax[NF];
+  for (int k = 0; k < NF; k++) {
+    dmin[k] = __builtin_huge_valf();
+    dmax[k] = -__builtin_huge_valf();
+  }
   const int deg = A.deg[c];
   const int e0 = A.gbase[c >> 6] + (c & 63);
   // the own-side offsets only (12 bytes per face-end, not the 32-byte record)
@@ -606,17 +612,18 @@ __device__ __forceinline__ void limit_fp32(const FusedArgs<float> &A, int c) {
     const float dr[3] = {r.x, r.y, r.z};
     for (int k = 0; k < NF; k++) {
       const float d = gg[k] * dr[0] + gg[NF + k] * dr[1] + gg[2 * NF + k] * dr[2];
-      const float y = d + tv[k] * dtt;
-      dlo[k] = smin(dlo[k], d);
-      dhi[k] = smax(dhi[k], d);
-      ylo[k] = smin(ylo[k], y);
-      yhi[k] = smax(yhi[k], y);
+      dmin[k] = smin(dmin[k], d);
+      dmax[k] = smax(dmax[k], d);
     }
   }
   for (int k = 0; k < NF; k++) {
     const float mn = SF(A.mm, k, c, n), mx = SF(A.mm, NF + k, c, n);
-    const float lr = lim_from_ratio<LIM>(min_ratio(mn, mx, dlo[k], dhi[k], 1.0f));
-    const float lm = lr * lim_from_ratio<LIM>(min_ratio(mn, mx, ylo[k], yhi[k], lr));
+    // the extremes start at 0 (a cell without faces gives 0, 0)
+    const float dlo = smin(0.0f, dmin[k]), dhi = smax(0.0f, dmax[k]);
+    const float ylo = smin(0.0f, __builtin_fmaf(tv[k], dtt, dmin[k]));
+    const float yhi = smax(0.0f, __builtin_fmaf(tv[k], dtt, dmax[k]));
+    const float lr = lim_from_ratio<LIM>(min_ratio(mn, mx, dlo, dhi, 1.0f));
+    const float lm = lr * lim_from_ratio<LIM>(min_ratio(mn, mx, ylo, yhi, lr));
     tv[k] *= lm;
     gg[k] *= lm;
     gg[NF + k] *= lm;
