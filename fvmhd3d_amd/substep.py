@@ -49,6 +49,18 @@ class RungClock:
     def get_dt(self, rung: int) -> float:
         return self.dt_tick * float(1 << (RUNGMAX - 1 - rung))
 
+    def rungs_of(self, dt: np.ndarray, nrung: int) -> np.ndarray:
+        """get_rung over an array, clamped to [min_rung, nrung - 1] as the
+        driver clamps (Iterate.cpp:146-168); the exponent is read from the bits."""
+        e = (np.ascontiguousarray(dt, dtype=np.float64).view(np.uint64) >> np.uint64(52)) & np.uint64(0xFFF)
+        r = np.maximum(exp_of(self.dtmax) - e.astype(np.int64), self.min_rung)
+        assert np.all(r < RUNGMAX)
+        return np.maximum(np.minimum(r, nrung - 1), self.min_rung)
+
+    def dts_of(self, rung: np.ndarray) -> np.ndarray:
+        """get_dt over an array: dt_tick * 2^(RUNGMAX-1-rung), exact scalings."""
+        return np.ldexp(self.dt_tick, (RUNGMAX - 1 - np.asarray(rung, dtype=np.int64)).astype(np.int32))
+
     def tsys(self) -> float:
         return self.dt_tick * self.tsysU
 
@@ -156,9 +168,8 @@ class Substepper:
         backend.fetch(st, ["dt_new"])
         dtn = np.empty(self.mesh.ncell)
         dtn[perm] = st.dt_new
-        for i in range(self.mesh.ncell):
-            self.rung[i] = min(self.clock.get_rung(dtn[i]), self.nrung - 1)
-        self.tend = self.tbeg + np.array([self.clock.get_dt(int(r)) for r in self.rung])
+        self.rung[:] = self.clock.rungs_of(dtn, self.nrung)
+        self.tend = self.tbeg + self.clock.dts_of(self.rung)
 
     def _state(self, active):
         perm, nact, nimp, faces, coff, cface, cvol, is_ngb = substep_layout(self.mesh, active, self.bnd)
@@ -204,12 +215,10 @@ class Substepper:
         backend.compute_predictor(st, self.clock.dtmax, self.clock.min_rung)
         backend.fetch(st, ["dt_new"])
         # new rungs of the active cells (Iterate.cpp:146-168)
-        for i in range(nact):
-            r = min(self.clock.get_rung(st.dt_new[i]), self.nrung - 1)
-            r = max(r, self.clock.min_rung)
-            self.rung[perm[i]] = r
-            st.tbeg[i] = self.t_global
-            st.tend[i] = self.t_global + self.clock.get_dt(r)
+        r = self.clock.rungs_of(st.dt_new[:nact], self.nrung)
+        self.rung[perm[:nact]] = r
+        st.tbeg[:nact] = self.t_global
+        st.tend[:nact] = self.t_global + self.clock.dts_of(r)
         backend.put(st, ["tbeg", "tend"])
         backend.compute_reconstruction(st)
         backend.slope_limiter(st)

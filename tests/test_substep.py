@@ -92,3 +92,20 @@ def test_substeps_gpu_match_oracle():
         assert err <= (1e-10 if name == "t" else 1e-12), (name, err)
     fr = [h[2] / m.nface for h in Sg.history]
     print("active face fractions", np.round(fr, 3))
+
+
+def test_rung_arrays_match_scalar_scheduler():
+    """RungClock.rungs_of / dts_of (the driver's array form) against get_rung /
+    get_dt cell by cell (Scheduler.h:94-111), with the driver's clamp to
+    [min_rung, nrung - 1]; dts are compared bit for bit."""
+    c = RungClock(1.0 / 64)
+    rng = np.random.default_rng(3)
+    dt = np.exp(rng.uniform(np.log(1e-7), np.log(1.0 / 64), 20000))
+    dt[:4] = [1.0 / 64, 1.0 / 128, np.nextafter(1.0 / 128, 0.0), 3e-7]
+    for min_rung in (0, 2):
+        c.min_rung = min_rung
+        r_ref = np.array([max(min(c.get_rung(x), 5), min_rung) for x in dt])
+        d_ref = np.array([c.get_dt(int(r)) for r in r_ref])
+        r = c.rungs_of(dt, 6)
+        assert np.array_equal(r, r_ref)
+        assert np.array_equal(c.dts_of(r).view(np.uint64), d_ref.view(np.uint64))
