@@ -94,31 +94,35 @@ def substep_layout(mesh, active: np.ndarray, bnd: np.ndarray):
     starts, ends = mesh.cell_off[A], mesh.cell_off[A + 1]
     lens = ends - starts
     idx = np.repeat(starts, lens) + (np.arange(lens.sum()) - np.repeat(np.cumsum(lens) - lens, lens))
-    fl = np.unique(mesh.cell_face[idx])  # faces of active cells, ascending
-    f = mesh.faces[fl]
-    s1 = f["s1"].astype(np.int64)
-    s2 = f["s2"].astype(np.int64)
-    touched = np.unique(np.concatenate([s1, s2]))
-    inact = touched[~active[touched]]
+    # faces of active cells, ascending, and the inactive cells they touch
+    # (boolean marks: linear in the mesh size, no sort)
+    fmark = np.zeros(mesh.nface, dtype=bool)
+    fmark[mesh.cell_face[idx]] = True
+    fl = np.nonzero(fmark)[0]
+    # the face records as raw rows (a structured gather is several times slower)
+    rows = mesh.faces.view(np.uint8).reshape(mesh.nface, mesh.faces.dtype.itemsize)
+    faces = rows[fl].view(mesh.faces.dtype).reshape(len(fl))
+    s1 = faces["s1"].astype(np.int64)
+    s2 = faces["s2"].astype(np.int64)
+    cmark = np.zeros(n, dtype=bool)
+    cmark[s1] = True
+    cmark[s2] = True
+    inact = np.nonzero(cmark & ~active)[0]
     perm = np.concatenate([A, inact])
     g2l = np.full(n, -1, dtype=np.int64)
     g2l[perm] = np.arange(len(perm))
     # orient: s1 must be active; swap when only s2 is
     swap = ~active[s1] & active[s2]
-    c = f["centroid"].copy()
-    nv = f["n"].copy()
     if np.any(swap):
+        sw = np.nonzero(swap)[0]
         box = np.asarray(mesh.box)
-        p_new = mesh.pos[s2[swap]]
-        d = c[swap] - p_new
+        p_new = mesh.pos[s2[sw]]
+        d = faces["centroid"][sw] - p_new
         d -= np.round(d / box) * box
-        c[swap] = p_new + d
-        nv[swap] = -nv[swap]
-    faces = np.empty(len(fl), dtype=mesh.faces.dtype)
+        faces["centroid"][sw] = p_new + d
+        faces["n"][sw] = -faces["n"][sw]
     faces["s1"] = g2l[np.where(swap, s2, s1)]
     faces["s2"] = g2l[np.where(swap, s1, s2)]
-    faces["centroid"] = c
-    faces["n"] = nv
     # cell lists of the active cells, local face ids in ascending mesh-face order
     f2l = np.full(mesh.nface, -1, dtype=np.int64)
     f2l[fl] = np.arange(len(fl))
