@@ -50,7 +50,8 @@ def _run_gpu(st, lim, precision, nsteps, dtmax):
     return W, U, dts, t, dg
 
 
-@pytest.mark.parametrize("kind,n,nsteps", [("alfven", 12, 5), ("ot", 12, 5), ("blast", 14, 4), ("blast_minmod", 12, 4)])
+@pytest.mark.parametrize("kind,n,nsteps", [("alfven", 12, 5), ("ot", 12, 5), ("blast", 14, 4), ("blast_minmod", 12, 4),
+                                            ("blast", 32, 2)])
 def test_fused_fp64_matches_oracle(kind, n, nsteps):
     m, st, lim = _case(kind, n)
     ref = st.copy()
