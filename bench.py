@@ -96,38 +96,56 @@ def cpu_baseline(st, lim, budget_s=30.0):
                       f"oracle/fvm_oracle.c FP64, {dt:.1f} s"}
 
 
-def substep_bench(nsub, n=2_097_152, dtmax=1.0 / 128):
+def substep_bench(nsub, n=2_097_152, dtmax=1.0 / 128, host_driver=True):
     """Individual timesteps on the disk (rho ~ exp(-z^2/2H^2), Keplerian v_phi, B_z at
-    beta = 400, 1 % velocity noise; 6 rungs): the host driver (rung clock, per-substep
-    layout and transfers, the sequential timestep-limiter sweep) over the path's kernels on
-    the GPU.  Reports active faces per second of wall time, and the split between the
-    library calls and the host side.  dtmax = 1/128: the 6 rungs (2.4e-4 .. 7.8e-3) then
-    cover the initial CFL steps of the 2M-generator disk (4.1e-4 .. 6.8e-3); the driver
-    clamps to the finest rung, so a coarser dtmax would overstep the CFL limit."""
+    beta = 400, 1 % velocity noise; 6 rungs).  The measured driver keeps the per-cell
+    state on the device (fvm_substep_run: layout, gathers, rungs and write-back on the
+    GPU; the host keeps the rung clock and the sequential timestep-limiter sweep).
+    Reports active faces per second of wall time.  With host_driver, the host-side
+    driver (per-substep layout and full-state transfers) runs the same substeps after
+    it, for its time and as a check: its per-substep counts and final state must equal
+    the device driver's bit for bit.  dtmax = 1/128: the 6 rungs (2.4e-4 .. 7.8e-3)
+    then cover the initial CFL steps of the 2M-generator disk (4.1e-4 .. 6.8e-3); the
+    driver clamps to the finest rung, so a coarser dtmax would overstep the CFL limit."""
     from fvmhd3d_amd import gpu as G
     from fvmhd3d_amd import mesh as M
-    from fvmhd3d_amd.substep import GpuBackend, Substepper, ic_disk
+    from fvmhd3d_amd.substep import DeviceSubstepper, GpuBackend, Substepper, ic_disk
     t = time.perf_counter()
     pos = M.uniform_random(n, seed=123)
     m = M.build_periodic(pos[M.morton_order(pos, (1.0, 1.0, 1.0))])
     log(f"[substeps] mesh {m.ncell} cells, {m.nface} faces in {time.perf_counter() - t:.0f} s")
-    S = Substepper(m, ic_disk(m.pos), dtmax=dtmax, limiter=G.LIMITER_SMOOTH)
-    be = GpuBackend(S.gamma, S.courant, tuple(float(b) for b in m.box), limiter=G.LIMITER_SMOOTH)
-    S.init_rungs(be)
-    be.kernel_s = 0.0
-    be.ctx.reset_diagnostics()
+    W0 = ic_disk(m.pos)
+    D = DeviceSubstepper(m, W0, dtmax=dtmax, limiter=G.LIMITER_SMOOTH)
+    D.init_rungs()
+    D.ctx.reset_diagnostics()
     t0 = time.perf_counter()
     for k in range(nsub):
-        S.substep(be)
-        log(f"[substeps] {k + 1}/{nsub}: {S.history[-1][1]} active cells, {S.history[-1][2]} active faces")
+        D.substep()
+        log(f"[substeps] device driver {k + 1}/{nsub}: {D.history[-1][1]} active cells, {D.history[-1][2]} active faces")
     wall = time.perf_counter() - t0
-    faces = [h[2] for h in S.history]
-    return {"metric": "individual-timestep active-face updates/s (host-driven substeps)",
-            "value": sum(faces) / wall, "unit": "face-flux updates/s", "substeps": nsub, "wall_s": round(wall, 3),
-            "library_call_s": round(be.kernel_s, 3), "host_s": round(wall - be.kernel_s, 3),
-            "active_face_fraction": [round(f / m.nface, 4) for f in faces],
-            "rungs": int(S.nrung), "dtmax": dtmax, "diagnostics": be.ctx.diagnostics(),
-            "workload": f"disk, {m.ncell} random generators"}
+    faces = [h[2] for h in D.history]
+    out = {"metric": "individual-timestep active-face updates/s (device-resident substeps)",
+           "value": sum(faces) / wall, "unit": "face-flux updates/s", "substeps": nsub, "wall_s": round(wall, 3),
+           "active_face_fraction": [round(f / m.nface, 4) for f in faces],
+           "rungs": int(D.nrung), "dtmax": dtmax, "diagnostics": D.ctx.diagnostics(),
+           "workload": f"disk, {m.ncell} random generators"}
+    if host_driver:
+        S = Substepper(m, W0, dtmax=dtmax, limiter=G.LIMITER_SMOOTH)
+        be = GpuBackend(S.gamma, S.courant, tuple(float(b) for b in m.box), limiter=G.LIMITER_SMOOTH)
+        S.init_rungs(be)
+        be.kernel_s = 0.0
+        t0 = time.perf_counter()
+        for k in range(nsub):
+            S.substep(be)
+            log(f"[substeps] host driver {k + 1}/{nsub}: {S.history[-1][2]} active faces")
+        hwall = time.perf_counter() - t0
+        same = S.history == D.history and all(
+            np.array_equal(np.asarray(getattr(S, f)).view(np.uint64), D.state(f).reshape(np.shape(getattr(S, f))).view(np.uint64))
+            for f in ("W", "U", "dU", "tend"))
+        out["host_driver"] = {"value": sum(faces) / hwall, "wall_s": round(hwall, 3),
+                              "library_call_s": round(be.kernel_s, 3), "host_s": round(hwall - be.kernel_s, 3),
+                              "bit_identical_to_device_driver": bool(same)}
+    return out
 
 
 def surrogate_bench(iters, n=128, nsteps=16, max_faces=4_000_000):

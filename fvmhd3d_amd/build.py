@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 
 HIP_SOURCES = ["fvm_gpu.hip", "fvm_scan.hip", "fvm_fast.hip", "fvm_fused_f64.hip", "fvm_fused_f32.hip",
-               "fvm_surrogate.hip", "fvm_diag.hip"]
+               "fvm_surrogate.hip", "fvm_diag.hip", "fvm_substep.hip"]
 # -ffp-contract=off keeps every FP64 expression as written (no FMA contraction),
 # which the reference-order and plain FP64 paths need to reproduce the
 # reference's arithmetic.  The FP32 face path (its own translation unit)

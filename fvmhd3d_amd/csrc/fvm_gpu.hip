@@ -1046,6 +1046,7 @@ int fvm_destroy(fvm_ctx *c) {
     release(*b);
   fvm::scan_release(c->scan);
   fvm_fast_release(c);
+  fvm_substep_release(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return FVM_OK;

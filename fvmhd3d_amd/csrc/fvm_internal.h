@@ -52,6 +52,18 @@ struct FusedState {
   long long launches[4] = {0, 0, 0, 0};
 };
 
+// individual-timestep state in mesh order (fvm_substep.hip): the global
+// mesh, the per-cell fields that persist between substeps, and the layout
+// scratch of one substep
+struct SubstepState {
+  bool ready = false;
+  int n = 0, nface = 0, nref = 0;
+  DBuf gs1, gs2, gc, gn, goff, gface, gvol;  // global mesh (SoA)
+  DBuf g[FVM_NFIELDS];                        // global fields, SoA [k*n + i]
+  DBuf rung;                                  // int per cell
+  DBuf act, apos, fmark, fpos, fl, cmark, iflag, ipos, g2l, perm, len, hist, err;
+};
+
 struct fvm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -67,6 +79,7 @@ struct fvm_ctx {
   DBuf scalars;  // unsigned long long[8]
   fvm::ScanScratch scan;
   FusedState fz;
+  SubstepState sub;
 };
 
 int fvm_fail(fvm_ctx *c, int code, const std::string &msg);
@@ -75,6 +88,8 @@ void fvm_release(DBuf &b);
 
 // release of the fused-path buffers of a context (fvm_fast.hip)
 void fvm_fast_release(fvm_ctx *c);
+// release of the individual-timestep buffers of a context (fvm_substep.hip)
+void fvm_substep_release(fvm_ctx *c);
 // fused stepping per face precision (fvm_fused_f32.hip, fvm_fused_f64.hip)
 int fvm_fused_setup_f32(fvm_ctx *c);
 int fvm_fused_setup_f64(fvm_ctx *c);

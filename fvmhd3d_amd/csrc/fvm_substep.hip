@@ -1,0 +1,600 @@
+// Individual timesteps with the state resident on the device.
+//
+// The host driver of the reference builds, for every substep, a local index
+// layout (active cells first, then the inactive cells that share a face with
+// one; localMesh.cpp:216-322, faces oriented so that s1 is active,
+// localMesh.cpp:497-591), predicts the inactive neighbours to the current
+// time (Iterate.cpp:99-106), runs the path on that subset and adds the
+// neighbours' dU back to their owners (computeFluidUpdate.cpp:320-345).  Here
+// the global per-cell state stays in HBM in mesh order and the layout is built
+// by marks and ballot scans on the device, so a substep moves only the
+// handful of counts the host's rung clock needs.  The path itself runs through
+// the same reference-order entry points as a host-built layout, and every
+// step here is a copy, an integer compaction or the same FP64 expression as
+// the host driver, so both give bit-identical states.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fvm_gpu.h"
+#include "fvm_internal.h"
+
+#define SFLD(p, k, i, n) (p)[(size_t)(k) * (size_t)(n) + (size_t)(i)]
+
+namespace {
+
+constexpr int kRungMax = 32;  // Scheduler.h: RUNGMAX
+constexpr int kBlock = 256;
+
+inline unsigned nblocks(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+__device__ inline int gidx() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+// global fields that persist between substeps; the others are rebuilt for
+// every substep (velocities and accelerations zero, limiter one, flags)
+constexpr int kKept[] = {FVM_FIELD_W,         FVM_FIELD_GRAD_X, FVM_FIELD_GRAD_Y, FVM_FIELD_GRAD_Z,
+                         FVM_FIELD_DWDT,      FVM_FIELD_POS,    FVM_FIELD_J,      FVM_FIELD_TEND,
+                         FVM_FIELD_BND,       FVM_FIELD_MINMAX_LO, FVM_FIELD_MINMAX_HI, FVM_FIELD_DU,
+                         FVM_FIELD_DGRADPSI,  FVM_FIELD_DDIVB,  FVM_FIELD_U,      FVM_FIELD_MESH_VOLUME,
+                         FVM_FIELD_TBEG,      FVM_FIELD_DT_NEW};
+// copied back to the owner for active cells
+constexpr int kBack[] = {FVM_FIELD_W,         FVM_FIELD_U,      FVM_FIELD_GRAD_X, FVM_FIELD_GRAD_Y,
+                         FVM_FIELD_GRAD_Z,    FVM_FIELD_DWDT,   FVM_FIELD_MINMAX_LO, FVM_FIELD_MINMAX_HI,
+                         FVM_FIELD_J,         FVM_FIELD_MESH_VOLUME, FVM_FIELD_TBEG, FVM_FIELD_TEND,
+                         FVM_FIELD_DT_NEW,    FVM_FIELD_DU,     FVM_FIELD_DGRADPSI, FVM_FIELD_DDIVB};
+// accumulated into the owner for inactive neighbours
+constexpr int kAccum[] = {FVM_FIELD_DU, FVM_FIELD_DGRADPSI, FVM_FIELD_DDIVB};
+
+// field widths (per-cell values) of the field table
+constexpr int kWidth[FVM_NFIELDS] = {10, 10, 10, 10, 10, 3, 3, 3, 1, 1, 1, 1, 10,
+                                     10, 10, 10, 3, 1, 10, 1, 1, 3, 3, 3, 1, 1};
+constexpr bool kInt[FVM_NFIELDS] = {false, false, false, false, false, false, false, false, false,
+                                    true,  true,  true,  false, false, false, false, false, false,
+                                    false, false, false, false, false, false, false, false};
+
+bool kept(int f) {
+  for (int k : kKept)
+    if (k == f) return true;
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+
+// active flag from the rung bitmask; an active cell must end its step now
+__global__ void k_active(const int *__restrict__ rung, const double *__restrict__ tend, int n, unsigned mask,
+                         double t_global, int check, int *__restrict__ act, int *__restrict__ err) {
+  const int i = gidx();
+  if (i >= n) return;
+  const int a = (int)((mask >> rung[i]) & 1u);
+  act[i] = a;
+  if (check && a && tend[i] != t_global) atomicOr(err, 1);
+}
+
+// out[base + pos[i]] = i for the flagged entries (ascending order)
+__global__ void k_compact(const int *__restrict__ flag, const int *__restrict__ pos, int n, int base,
+                          int *__restrict__ out) {
+  const int i = gidx();
+  if (i >= n || !flag[i]) return;
+  out[base + pos[i]] = i;
+}
+
+__global__ void k_fill_i(int *__restrict__ p, long long n, int v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_d(double *__restrict__ p, long long n, double v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// mark the faces of the active cells
+__global__ void k_mark_faces(const int *__restrict__ A, int nA, const int *__restrict__ goff,
+                             const int *__restrict__ gface, int *__restrict__ fmark) {
+  const int j = gidx();
+  if (j >= nA) return;
+  const int g = A[j];
+  for (int e = goff[g]; e < goff[g + 1]; e++) fmark[gface[e]] = 1;
+}
+
+// mark both cells of every marked face
+__global__ void k_mark_cells(const int *__restrict__ fl, int nfl, const int *__restrict__ gs1,
+                             const int *__restrict__ gs2, int *__restrict__ cmark) {
+  const int j = gidx();
+  if (j >= nfl) return;
+  const int f = fl[j];
+  cmark[gs1[f]] = 1;
+  cmark[gs2[f]] = 1;
+}
+
+__global__ void k_inactive_flags(const int *__restrict__ cmark, const int *__restrict__ act, int n,
+                                 int *__restrict__ iflag) {
+  const int i = gidx();
+  if (i >= n) return;
+  iflag[i] = cmark[i] && !act[i];
+}
+
+// global -> local index of every cell in the layout
+__global__ void k_g2l(const int *__restrict__ act, const int *__restrict__ apos, const int *__restrict__ iflag,
+                      const int *__restrict__ ipos, int n, int nA, int *__restrict__ g2l) {
+  const int i = gidx();
+  if (i >= n) return;
+  g2l[i] = act[i] ? apos[i] : (iflag[i] ? nA + ipos[i] : -1);
+}
+
+// local faces: the faces of the active cells in ascending global order,
+// turned so that s1 is active with the centroid moved to the periodic image
+// next to the new s1 (the same FP64 expressions as the host layout)
+__global__ void k_local_faces(const int *__restrict__ fl, int nfl, const int *__restrict__ gs1,
+                              const int *__restrict__ gs2, const double *__restrict__ gc,
+                              const double *__restrict__ gn, int nface, const double *__restrict__ pos, int n,
+                              const int *__restrict__ act, const int *__restrict__ g2l, double bx, double by,
+                              double bz, int *__restrict__ s1, int *__restrict__ s2, double *__restrict__ c,
+                              double *__restrict__ nn) {
+  const int j = gidx();
+  if (j >= nfl) return;
+  const int f = fl[j];
+  const int a = gs1[f], b = gs2[f];
+  const bool swap = !act[a] && act[b];
+  s1[j] = g2l[swap ? b : a];
+  s2[j] = g2l[swap ? a : b];
+  const double box[3] = {bx, by, bz};
+  for (int d = 0; d < 3; d++) {
+    double cd = SFLD(gc, d, f, nface), nd = SFLD(gn, d, f, nface);
+    if (swap) {
+      const double p = SFLD(pos, d, b, n);
+      double dd = cd - p;
+      dd = dd - rint(dd / box[d]) * box[d];
+      cd = p + dd;
+      nd = -nd;
+    }
+    SFLD(c, d, j, nfl) = cd;
+    SFLD(nn, d, j, nfl) = nd;
+  }
+}
+
+__global__ void k_face_counts(const int *__restrict__ A, int nA, const int *__restrict__ goff, int *__restrict__ len) {
+  const int j = gidx();
+  if (j >= nA) return;
+  len[j] = goff[A[j] + 1] - goff[A[j]];
+}
+
+// local cell lists (local face ids, in the order of the global list) and volumes
+__global__ void k_local_cells(const int *__restrict__ A, int nA, const int *__restrict__ goff,
+                              const int *__restrict__ gface, const int *__restrict__ fpos,
+                              const double *__restrict__ gvol, const int *__restrict__ coff, int *__restrict__ cface,
+                              double *__restrict__ cvol) {
+  const int j = gidx();
+  if (j >= nA) return;
+  const int g = A[j];
+  const int e0 = goff[g], ne = goff[g + 1] - e0, o = coff[j];
+  for (int e = 0; e < ne; e++) cface[o + e] = fpos[gface[e0 + e]];
+  cvol[j] = gvol[g];
+}
+
+// local[k][i] = global[k][perm[i]] for i < ncopy, zero for ncopy <= i < npl
+template <typename S>
+__global__ void k_gather(const S *__restrict__ G, int n, S *__restrict__ L, int npl, const int *__restrict__ perm,
+                         int width, int ncopy) {
+  const int i = gidx();
+  if (i >= npl) return;
+  const int g = perm[i];
+  for (int k = 0; k < width; k++) SFLD(L, k, i, npl) = i < ncopy ? SFLD(G, k, g, n) : (S)0;
+}
+
+__global__ void k_flags(int *__restrict__ is_ngb, int *__restrict__ is_active, int npl, int nA) {
+  const int i = gidx();
+  if (i >= npl) return;
+  is_ngb[i] = i >= nA;
+  is_active[i] = i < nA;
+}
+
+// inactive neighbours predicted to the current time: w + Wt * (t - tbeg)
+__global__ void k_predict_inactive(double *__restrict__ w, const double *__restrict__ wt,
+                                   const double *__restrict__ tbeg, int npl, int nA, double t_global) {
+  const int i = nA + gidx();
+  if (i >= npl) return;
+  const double dtp = t_global - tbeg[i];
+  for (int k = 0; k < 10; k++) SFLD(w, k, i, npl) = SFLD(w, k, i, npl) + SFLD(wt, k, i, npl) * dtp;
+}
+
+// new rungs of the active cells from dt_new (Scheduler.h get_rung with the
+// driver's clamp, Iterate.cpp:146-168): exponent difference to dtmax, at
+// least min_rung, at most nrung-1; tbeg/tend of the new step
+__global__ void k_rungs(const double *__restrict__ dt_new, double *__restrict__ tbeg, double *__restrict__ tend,
+                        const int *__restrict__ perm, int nA, int emax, int min_rung, int nrung, double dt_tick,
+                        double t_global, int *__restrict__ rung, int *__restrict__ err) {
+  const int i = gidx();
+  if (i >= nA) return;
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(dt_new[i]);
+  const int e = (int)((bits >> 52) & 0xFFFull);
+  int r = emax - e;
+  if (r < min_rung) r = min_rung;
+  if (r >= kRungMax) atomicOr(err, 2);
+  if (r > nrung - 1) r = nrung - 1;
+  if (r < min_rung) r = min_rung;
+  rung[perm[i]] = r;
+  tbeg[i] = t_global;
+  tend[i] = t_global + ldexp(dt_tick, kRungMax - 1 - r);
+}
+
+// active cells: the owner takes the local values
+template <typename S>
+__global__ void k_scatter(const S *__restrict__ L, int npl, S *__restrict__ G, int n, const int *__restrict__ perm,
+                          int nA, int width) {
+  const int i = gidx();
+  if (i >= nA) return;
+  const int g = perm[i];
+  for (int k = 0; k < width; k++) SFLD(G, k, g, n) = SFLD(L, k, i, npl);
+}
+
+// inactive neighbours: the owner adds the local dU (each owner appears once)
+__global__ void k_accumulate(const double *__restrict__ L, int npl, double *__restrict__ G, int n,
+                             const int *__restrict__ perm, int nA, int width) {
+  const int i = nA + gidx();
+  if (i >= npl) return;
+  const int g = perm[i];
+  for (int k = 0; k < width; k++) SFLD(G, k, g, n) += SFLD(L, k, i, npl);
+}
+
+// rung histogram of every cell (block-local counts first)
+__global__ void k_rung_hist(const int *__restrict__ rung, int n, int *__restrict__ hist) {
+  __shared__ int h[kRungMax];
+  if (threadIdx.x < kRungMax) h[threadIdx.x] = 0;
+  __syncthreads();
+  const int i = gidx();
+  if (i < n) {
+    const int r = rung[i];
+    atomicAdd(&h[(r >= 0 && r < kRungMax) ? r : kRungMax - 1], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < kRungMax && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+}  // namespace
+
+#define SCHK(c, call) FVM_HIPCHK(c, call)
+#define SLAUNCH(c, n, kern, ...)                                                           \
+  do {                                                                                     \
+    if ((n) > 0) {                                                                         \
+      hipLaunchKernelGGL(kern, dim3(nblocks(n)), dim3(kBlock), 0, (c)->stream, __VA_ARGS__); \
+      SCHK(c, hipGetLastError());                                                          \
+    }                                                                                      \
+  } while (0)
+
+void fvm_substep_release(fvm_ctx *c) {
+  SubstepState &s = c->sub;
+  for (DBuf *b : {&s.gs1, &s.gs2, &s.gc, &s.gn, &s.goff, &s.gface, &s.gvol, &s.rung, &s.act, &s.apos, &s.fmark,
+                  &s.fpos, &s.fl, &s.cmark, &s.iflag, &s.ipos, &s.g2l, &s.perm, &s.len, &s.hist, &s.err})
+    fvm_release(*b);
+  for (auto &b : s.g) fvm_release(b);
+  s.ready = false;
+}
+
+namespace {
+
+template <typename T>
+T *P(DBuf &b) {
+  return (T *)b.p;
+}
+
+int ensure_all(fvm_ctx *c, std::initializer_list<std::pair<DBuf *, size_t>> l) {
+  for (auto &e : l) {
+    int rc = fvm_ensure(c, *e.first, e.second ? e.second : 4);
+    if (rc) return rc;
+  }
+  return FVM_OK;
+}
+
+int read_err(fvm_ctx *c, int *e) {
+  SCHK(c, hipMemcpyAsync(e, c->sub.err.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  SCHK(c, hipStreamSynchronize(c->stream));
+  return FVM_OK;
+}
+
+// Build the local layout of one substep in the context's mesh and fields:
+// active cells (rung in `mask`) in mesh order, then the inactive cells that
+// share a face with them; faces of the active cells, s1 active.
+int build_layout(fvm_ctx *c, unsigned mask, double t_global, int check, int *counts) {
+  SubstepState &s = c->sub;
+  const int n = s.n, nface = s.nface;
+  hipStream_t st = c->stream;
+  SLAUNCH(c, 1, k_fill_i, P<int>(s.err), 1, 0);
+  SLAUNCH(c, n, k_active, P<int>(s.rung), P<double>(s.g[FVM_FIELD_TEND]), n, mask, t_global, check, P<int>(s.act),
+          P<int>(s.err));
+  int nA = 0, nfl = 0, nI = 0, nref = 0;
+  if (fvm::exclusive_scan_flags(c->scan, P<int>(s.act), P<int>(s.apos), n, &nA, st))
+    return fvm_fail(c, FVM_EHIP, "substep: scan of active cells failed");
+  int e = 0;
+  if (int rc = read_err(c, &e)) return rc;
+  if (e & 1) return fvm_fail(c, FVM_ESTATE, "substep: an active cell does not end its step at the current time");
+  SLAUNCH(c, n, k_compact, P<int>(s.act), P<int>(s.apos), n, 0, P<int>(s.perm));
+  SLAUNCH(c, nface, k_fill_i, P<int>(s.fmark), nface, 0);
+  SLAUNCH(c, nA, k_mark_faces, P<int>(s.perm), nA, P<int>(s.goff), P<int>(s.gface), P<int>(s.fmark));
+  if (fvm::exclusive_scan_flags(c->scan, P<int>(s.fmark), P<int>(s.fpos), nface, &nfl, st))
+    return fvm_fail(c, FVM_EHIP, "substep: scan of faces failed");
+  SLAUNCH(c, nface, k_compact, P<int>(s.fmark), P<int>(s.fpos), nface, 0, P<int>(s.fl));
+  SLAUNCH(c, n, k_fill_i, P<int>(s.cmark), n, 0);
+  SLAUNCH(c, nfl, k_mark_cells, P<int>(s.fl), nfl, P<int>(s.gs1), P<int>(s.gs2), P<int>(s.cmark));
+  SLAUNCH(c, n, k_inactive_flags, P<int>(s.cmark), P<int>(s.act), n, P<int>(s.iflag));
+  if (fvm::exclusive_scan_flags(c->scan, P<int>(s.iflag), P<int>(s.ipos), n, &nI, st))
+    return fvm_fail(c, FVM_EHIP, "substep: scan of neighbour cells failed");
+  SLAUNCH(c, n, k_compact, P<int>(s.iflag), P<int>(s.ipos), n, nA, P<int>(s.perm));
+  SLAUNCH(c, n, k_g2l, P<int>(s.act), P<int>(s.apos), P<int>(s.iflag), P<int>(s.ipos), n, nA, P<int>(s.g2l));
+  const int npl = nA + nI;
+  // the context's mesh (capacity for the whole mesh was reserved at setup)
+  SLAUNCH(c, nfl, k_local_faces, P<int>(s.fl), nfl, P<int>(s.gs1), P<int>(s.gs2), P<double>(s.gc), P<double>(s.gn),
+          nface, P<double>(s.g[FVM_FIELD_POS]), n, P<int>(s.act), P<int>(s.g2l), c->prm.domain_size[0],
+          c->prm.domain_size[1], c->prm.domain_size[2], P<int>(c->m_s1), P<int>(c->m_s2), P<double>(c->m_c),
+          P<double>(c->m_n));
+  SLAUNCH(c, nA, k_face_counts, P<int>(s.perm), nA, P<int>(s.goff), P<int>(s.len));
+  if (nA > 0) {
+    if (fvm::exclusive_scan(c->scan, P<int>(s.len), P<int>(c->m_coff), nA, st))
+      return fvm_fail(c, FVM_EHIP, "substep: scan of face counts failed");
+    SCHK(c, hipMemcpyAsync(P<int>(c->m_coff) + nA, c->scan.total, sizeof(int), hipMemcpyDeviceToDevice, st));
+    SCHK(c, hipMemcpyAsync(&nref, c->scan.total, sizeof(int), hipMemcpyDeviceToHost, st));
+  } else {
+    SLAUNCH(c, 1, k_fill_i, P<int>(c->m_coff), 1, 0);
+  }
+  SLAUNCH(c, nA, k_local_cells, P<int>(s.perm), nA, P<int>(s.goff), P<int>(s.gface), P<int>(s.fpos), P<double>(s.gvol),
+          P<int>(c->m_coff), P<int>(c->m_cface), P<double>(c->m_cvol));
+  SCHK(c, hipStreamSynchronize(st));
+  if (npl <= 0 || nref < 0 || nref > s.nref || nfl > nface)
+    return fvm_fail(c, FVM_ESTATE, "substep: empty or inconsistent layout");
+  c->np = npl;
+  c->nactive = nA;
+  c->nimport_loc = nI;
+  c->nface = nfl;
+  c->nref = nref;
+  c->nfa = -1;
+  c->have_mesh = true;
+  c->t_global = t_global;
+  // the context's fields
+  for (int f = 0; f < FVM_NFIELDS; f++) {
+    void *L = c->field[f].p;
+    const int w = kWidth[f];
+    if (kept(f)) {
+      const bool fresh_dU = f == FVM_FIELD_DU || f == FVM_FIELD_DGRADPSI || f == FVM_FIELD_DDIVB;
+      const int ncopy = fresh_dU ? nA : npl;
+      if (kInt[f])
+        SLAUNCH(c, npl, k_gather<int>, P<int>(s.g[f]), n, (int *)L, npl, P<int>(s.perm), w, ncopy);
+      else
+        SLAUNCH(c, npl, k_gather<double>, P<double>(s.g[f]), n, (double *)L, npl, P<int>(s.perm), w, ncopy);
+    } else if (f == FVM_FIELD_LIMITER) {
+      SLAUNCH(c, (long long)w * npl, k_fill_d, (double *)L, (long long)w * npl, 1.0);
+    } else if (!kInt[f]) {
+      SLAUNCH(c, (long long)w * npl, k_fill_d, (double *)L, (long long)w * npl, 0.0);
+    }
+  }
+  SLAUNCH(c, npl, k_flags, P<int>(c->field[FVM_FIELD_IS_NGB]), P<int>(c->field[FVM_FIELD_IS_ACTIVE]), npl, nA);
+  SLAUNCH(c, nI, k_predict_inactive, P<double>(c->field[FVM_FIELD_W]), P<double>(c->field[FVM_FIELD_DWDT]),
+          P<double>(c->field[FVM_FIELD_TBEG]), npl, nA, t_global);
+  if (counts) {
+    counts[0] = nA;
+    counts[1] = nI;
+    counts[2] = nfl;
+  }
+  return FVM_OK;
+}
+
+int exponent_of(double x) {
+  unsigned long long b;
+  std::memcpy(&b, &x, sizeof b);
+  return (int)((b >> 52) & 0xFFFull);
+}
+
+// rungs of the active cells, their tbeg/tend, and the histogram of all rungs
+int update_rungs(fvm_ctx *c, double dtmax, int min_rung, int nrung, double t_global, int *hist) {
+  SubstepState &s = c->sub;
+  const double dt_tick = dtmax / (double)(1u << (kRungMax - 1));
+  SLAUNCH(c, c->nactive, k_rungs, P<double>(c->field[FVM_FIELD_DT_NEW]), P<double>(c->field[FVM_FIELD_TBEG]),
+          P<double>(c->field[FVM_FIELD_TEND]), P<int>(s.perm), c->nactive, exponent_of(dtmax), min_rung, nrung, dt_tick,
+          t_global, P<int>(s.rung), P<int>(s.err));
+  int e = 0;
+  if (int rc = read_err(c, &e)) return rc;
+  if (e & 2) return fvm_fail(c, FVM_ESTATE, "substep: a timestep is below the finest rung of the clock");
+  return FVM_OK;
+}
+
+int rung_histogram(fvm_ctx *c, int *hist) {
+  SubstepState &s = c->sub;
+  SLAUNCH(c, kRungMax, k_fill_i, P<int>(s.hist), kRungMax, 0);
+  SLAUNCH(c, s.n, k_rung_hist, P<int>(s.rung), s.n, P<int>(s.hist));
+  if (hist) SCHK(c, hipMemcpyAsync(hist, s.hist.p, sizeof(int) * kRungMax, hipMemcpyDeviceToHost, c->stream));
+  SCHK(c, hipStreamSynchronize(c->stream));
+  return FVM_OK;
+}
+
+int write_back(fvm_ctx *c, bool full) {
+  SubstepState &s = c->sub;
+  const int n = s.n, npl = c->np, nA = c->nactive;
+  for (int f : kBack) {
+    if (!full && f != FVM_FIELD_TBEG && f != FVM_FIELD_TEND) continue;
+    SLAUNCH(c, nA, k_scatter<double>, P<double>(c->field[f]), npl, P<double>(s.g[f]), n, P<int>(s.perm), nA, kWidth[f]);
+  }
+  if (full)
+    for (int f : kAccum)
+      SLAUNCH(c, npl - nA, k_accumulate, P<double>(c->field[f]), npl, P<double>(s.g[f]), n, P<int>(s.perm), nA, kWidth[f]);
+  SCHK(c, hipStreamSynchronize(c->stream));
+  return FVM_OK;
+}
+
+int check_ready(fvm_ctx *c) {
+  if (!c) return FVM_EARG;
+  if (!c->sub.ready) return fvm_fail(c, FVM_ESTATE, "substep call before fvm_substep_setup");
+  if (c->prm.storage != FVM_STORAGE_F64) return fvm_fail(c, FVM_EARG, "device substeps need FVM_STORAGE_F64");
+  (void)hipSetDevice(c->device);
+  return FVM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvm_substep_setup(fvm_ctx *c, int ncell, int nface, const fvm_face *faces, const int *cell_face_off,
+                      const int *cell_face, const double *cell_volume) {
+  if (!c) return FVM_EARG;
+  if (c->prm.storage != FVM_STORAGE_F64) return fvm_fail(c, FVM_EARG, "device substeps need FVM_STORAGE_F64");
+  if (ncell <= 0 || nface < 0 || (nface > 0 && !faces) || !cell_face_off || !cell_volume)
+    return fvm_fail(c, FVM_EARG, "fvm_substep_setup: bad sizes or NULL array");
+  const int nref = cell_face_off[ncell];
+  if (cell_face_off[0] != 0 || nref < 0 || (nref > 0 && !cell_face))
+    return fvm_fail(c, FVM_EARG, "fvm_substep_setup: bad cell_face_off");
+  for (int i = 0; i < ncell; i++)
+    if (cell_face_off[i + 1] < cell_face_off[i]) return fvm_fail(c, FVM_EARG, "fvm_substep_setup: cell_face_off not monotone");
+  for (int e = 0; e < nref; e++)
+    if (cell_face[e] < 0 || cell_face[e] >= nface) return fvm_fail(c, FVM_EARG, "fvm_substep_setup: face index out of range");
+  for (int f = 0; f < nface; f++)
+    if (faces[f].s1 < 0 || faces[f].s1 >= ncell || faces[f].s2 < 0 || faces[f].s2 >= ncell || faces[f].s1 == faces[f].s2)
+      return fvm_fail(c, FVM_EARG, "fvm_substep_setup: face cell index out of range");
+  for (int i = 0; i < ncell; i++)
+    for (int e = cell_face_off[i]; e < cell_face_off[i + 1]; e++)
+      if (faces[cell_face[e]].s1 != i && faces[cell_face[e]].s2 != i)
+        return fvm_fail(c, FVM_EARG, "fvm_substep_setup: cell lists a face it does not own");
+  (void)hipSetDevice(c->device);
+  fvm_substep_release(c);
+  SubstepState &s = c->sub;
+  const size_t N = (size_t)ncell, F = (size_t)nface;
+  int rc;
+  if ((rc = ensure_all(c, {{&s.gs1, 4 * F}, {&s.gs2, 4 * F}, {&s.gc, 24 * F}, {&s.gn, 24 * F}, {&s.goff, 4 * (N + 1)},
+                           {&s.gface, 4 * (size_t)nref}, {&s.gvol, 8 * N}, {&s.rung, 4 * N}, {&s.act, 4 * N},
+                           {&s.apos, 4 * N}, {&s.fmark, 4 * F}, {&s.fpos, 4 * F}, {&s.fl, 4 * F}, {&s.cmark, 4 * N},
+                           {&s.iflag, 4 * N}, {&s.ipos, 4 * N}, {&s.g2l, 4 * N}, {&s.perm, 4 * N}, {&s.len, 4 * N},
+                           {&s.hist, 4 * kRungMax}, {&s.err, 4}})))
+    return rc;
+  // the context's mesh and fields at their largest (every cell local)
+  if ((rc = ensure_all(c, {{&c->m_s1, 4 * F}, {&c->m_s2, 4 * F}, {&c->m_c, 24 * F}, {&c->m_n, 24 * F},
+                           {&c->m_coff, 4 * (N + 1)}, {&c->m_cface, 4 * (size_t)nref}, {&c->m_cvol, 8 * N}})))
+    return rc;
+  for (int f = 0; f < FVM_NFIELDS; f++) {
+    const size_t bytes = (kInt[f] ? 4 : 8) * (size_t)kWidth[f] * N;
+    if ((rc = fvm_ensure(c, c->field[f], bytes))) return rc;
+    if (kept(f)) {
+      if ((rc = fvm_ensure(c, s.g[f], bytes))) return rc;
+      SCHK(c, hipMemsetAsync(s.g[f].p, 0, bytes, c->stream));
+    }
+  }
+  SCHK(c, hipMemsetAsync(s.rung.p, 0, 4 * N, c->stream));
+  SCHK(c, hipStreamSynchronize(c->stream));
+  if (nface > 0) {
+    // same conversion as the mesh upload: ints and SoA doubles
+    std::vector<int> a(F), b(F);
+    std::vector<double> cc(3 * F), nn(3 * F);
+    for (size_t f = 0; f < F; f++) {
+      a[f] = faces[f].s1;
+      b[f] = faces[f].s2;
+      for (int d = 0; d < 3; d++) {
+        cc[d * F + f] = faces[f].centroid[d];
+        nn[d * F + f] = faces[f].n[d];
+      }
+    }
+    SCHK(c, hipMemcpy(s.gs1.p, a.data(), 4 * F, hipMemcpyHostToDevice));
+    SCHK(c, hipMemcpy(s.gs2.p, b.data(), 4 * F, hipMemcpyHostToDevice));
+    SCHK(c, hipMemcpy(s.gc.p, cc.data(), 24 * F, hipMemcpyHostToDevice));
+    SCHK(c, hipMemcpy(s.gn.p, nn.data(), 24 * F, hipMemcpyHostToDevice));
+  }
+  SCHK(c, hipMemcpy(s.goff.p, cell_face_off, 4 * (N + 1), hipMemcpyHostToDevice));
+  if (nref > 0) SCHK(c, hipMemcpy(s.gface.p, cell_face, 4 * (size_t)nref, hipMemcpyHostToDevice));
+  SCHK(c, hipMemcpy(s.gvol.p, cell_volume, 8 * N, hipMemcpyHostToDevice));
+  SCHK(c, hipStreamSynchronize(c->stream));
+  s.n = ncell;
+  s.nface = nface;
+  s.nref = nref;
+  s.ready = true;
+  return FVM_OK;
+}
+
+int fvm_substep_upload(fvm_ctx *c, int field, const void *host, int ncell) {
+  if (int rc = check_ready(c)) return rc;
+  SubstepState &s = c->sub;
+  if (ncell != s.n || !host) return fvm_fail(c, FVM_EARG, "fvm_substep_upload: ncell mismatch or NULL");
+  if (field == FVM_SUBSTEP_RUNG) {
+    const int *r = (const int *)host;
+    for (int i = 0; i < ncell; i++)
+      if (r[i] < 0 || r[i] >= kRungMax) return fvm_fail(c, FVM_EARG, "fvm_substep_upload: rung out of range");
+    SCHK(c, hipMemcpy(s.rung.p, host, 4 * (size_t)ncell, hipMemcpyHostToDevice));
+    return FVM_OK;
+  }
+  if (field < 0 || field >= FVM_NFIELDS || !kept(field))
+    return fvm_fail(c, FVM_EARG, "fvm_substep_upload: field is not kept between substeps");
+  if (kInt[field]) {
+    SCHK(c, hipMemcpy(s.g[field].p, host, 4 * (size_t)ncell, hipMemcpyHostToDevice));
+    return FVM_OK;
+  }
+  const int w = kWidth[field];
+  std::vector<double> soa((size_t)w * ncell);
+  const double *h = (const double *)host;
+  for (int i = 0; i < ncell; i++)
+    for (int k = 0; k < w; k++) soa[(size_t)k * ncell + i] = h[(size_t)i * w + k];
+  SCHK(c, hipMemcpy(s.g[field].p, soa.data(), 8 * soa.size(), hipMemcpyHostToDevice));
+  return FVM_OK;
+}
+
+int fvm_substep_download(fvm_ctx *c, int field, void *host, int ncell) {
+  if (int rc = check_ready(c)) return rc;
+  SubstepState &s = c->sub;
+  if (ncell != s.n || !host) return fvm_fail(c, FVM_EARG, "fvm_substep_download: ncell mismatch or NULL");
+  SCHK(c, hipStreamSynchronize(c->stream));
+  if (field == FVM_SUBSTEP_RUNG) {
+    SCHK(c, hipMemcpy(host, s.rung.p, 4 * (size_t)ncell, hipMemcpyDeviceToHost));
+    return FVM_OK;
+  }
+  if (field < 0 || field >= FVM_NFIELDS || !kept(field))
+    return fvm_fail(c, FVM_EARG, "fvm_substep_download: field is not kept between substeps");
+  if (kInt[field]) {
+    SCHK(c, hipMemcpy(host, s.g[field].p, 4 * (size_t)ncell, hipMemcpyDeviceToHost));
+    return FVM_OK;
+  }
+  const int w = kWidth[field];
+  std::vector<double> soa((size_t)w * ncell);
+  SCHK(c, hipMemcpy(soa.data(), s.g[field].p, 8 * soa.size(), hipMemcpyDeviceToHost));
+  double *h = (double *)host;
+  for (int i = 0; i < ncell; i++)
+    for (int k = 0; k < w; k++) h[(size_t)i * w + k] = soa[(size_t)k * ncell + i];
+  return FVM_OK;
+}
+
+int fvm_substep_init_rungs(fvm_ctx *c, double dtmax, int nrung, int *rung_hist) {
+  if (int rc = check_ready(c)) return rc;
+  if (!(dtmax > 0.0) || nrung < 1 || nrung > kRungMax - 1) return fvm_fail(c, FVM_EARG, "fvm_substep_init_rungs: bad dtmax or nrung");
+  int rc;
+  if ((rc = build_layout(c, 0xFFFFFFFFu, 0.0, 0, nullptr))) return rc;
+  double dtmin = 0.0;
+  if ((rc = fvm_compute_timestep(c, &dtmin))) return rc;
+  if ((rc = update_rungs(c, dtmax, 0, nrung, 0.0, nullptr))) return rc;
+  if ((rc = write_back(c, false))) return rc;
+  return rung_histogram(c, rung_hist);
+}
+
+int fvm_substep_run(fvm_ctx *c, unsigned rung_mask, double t_global, double dtmax, int min_rung, int nrung, int *counts,
+                    int *rung_hist) {
+  if (int rc = check_ready(c)) return rc;
+  if (!(dtmax > 0.0) || nrung < 1 || nrung > kRungMax - 1 || min_rung < 0 || min_rung >= kRungMax)
+    return fvm_fail(c, FVM_EARG, "fvm_substep_run: bad dtmax, nrung or min_rung");
+  int cnt[3] = {0, 0, 0}, nfa = 0, rc;
+  if ((rc = build_layout(c, rung_mask, t_global, 1, cnt))) return rc;
+  if ((rc = fvm_get_active_faces(c, &nfa))) return rc;
+  if ((rc = fvm_compute_primitives(c, 0))) return rc;
+  double dtmin = 0.0;
+  if ((rc = fvm_compute_timestep(c, &dtmin))) return rc;
+  if ((rc = fvm_compute_timestep_limiter(c))) return rc;
+  if ((rc = fvm_compute_predictor(c, dtmax, min_rung))) return rc;
+  if ((rc = update_rungs(c, dtmax, min_rung, nrung, t_global, nullptr))) return rc;
+  if ((rc = fvm_compute_reconstruction(c))) return rc;
+  if ((rc = fvm_slope_limiter(c))) return rc;
+  const int pos = fvm_compute_fluid_update(c);
+  if (pos != FVM_OK && pos != FVM_EPOS) return pos;
+  if ((rc = write_back(c, true))) return rc;
+  if (counts) {
+    counts[0] = cnt[0];
+    counts[1] = nfa;
+    counts[2] = cnt[1];
+    counts[3] = cnt[2];
+    counts[4] = pos == FVM_EPOS;
+  }
+  return rung_histogram(c, rung_hist);
+}
+
+}  // extern "C"

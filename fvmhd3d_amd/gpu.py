@@ -27,6 +27,7 @@ FIELDS = {
     "U": (18, 10, "f"), "mvol": (19, 1, "f"), "tbeg": (20, 1, "f"), "mvel": (21, 3, "f"),
     "acc0": (22, 3, "f"), "acc1": (23, 3, "f"), "dt_new": (24, 1, "f"), "divB": (25, 1, "f"),
 }
+SUBSTEP_RUNG = len(FIELDS)  # rung per cell in fvm_substep_upload / _download
 NDIAG = 8
 DIAG_NAMES = ["positivity", "closure", "ethm_failed", "mass_failed", "dsl"]
 
@@ -82,6 +83,11 @@ SIGNATURES = {
     # learned HLLD surrogate (include/fvm_surrogate.h)
     "fvm_sg_create": [ctypes.POINTER(ctypes.c_void_p), _I, _D],
     "fvm_sg_destroy": [_P],
+    "fvm_substep_setup": [_P, _I, _I, _P, _P, _P, _P],
+    "fvm_substep_upload": [_P, _I, _P, _I],
+    "fvm_substep_download": [_P, _I, _P, _I],
+    "fvm_substep_init_rungs": [_P, _D, _I, _P],
+    "fvm_substep_run": [_P, ctypes.c_uint, _D, _D, _I, _I, _P, _P],
     "fvm_sg_set_params": [_P, _P, _I],
     "fvm_sg_get_params": [_P, _P, _I],
     "fvm_sg_set_gate": [_P, _P],
@@ -300,6 +306,47 @@ class Context:
         g = ctypes.c_double()
         self._chk(self.L.fvm_measure_copy_bandwidth(self.h, int(nbytes), int(iters), ctypes.byref(g)))
         return g.value
+
+    # individual timesteps with the state resident on the device
+    def substep_setup(self, faces, cell_off, cell_face, volume):
+        faces = np.ascontiguousarray(faces)
+        cell_off = np.ascontiguousarray(cell_off, dtype=np.int32)
+        cell_face = np.ascontiguousarray(cell_face, dtype=np.int32)
+        volume = np.ascontiguousarray(volume, dtype=np.float64)
+        self.nsub = len(volume)
+        self._chk(self.L.fvm_substep_setup(self.h, self.nsub, len(faces), _ptr(faces), _ptr(cell_off),
+                                           _ptr(cell_face), _ptr(volume)))
+
+    def substep_upload(self, name, arr):
+        if name == "rung":
+            a = np.ascontiguousarray(arr, dtype=np.int32).reshape(self.nsub)
+            fid = SUBSTEP_RUNG
+        else:
+            fid, width, kind = FIELDS[name]
+            a = np.ascontiguousarray(arr, dtype=np.int32 if kind == "i" else np.float64)
+        self._chk(self.L.fvm_substep_upload(self.h, fid, _ptr(a), self.nsub))
+
+    def substep_download(self, name):
+        if name == "rung":
+            a = np.empty(self.nsub, dtype=np.int32)
+            fid = SUBSTEP_RUNG
+        else:
+            fid, width, kind = FIELDS[name]
+            a = np.empty((self.nsub, width) if width > 1 else (self.nsub,), dtype=np.int32 if kind == "i" else np.float64)
+        self._chk(self.L.fvm_substep_download(self.h, fid, _ptr(a), self.nsub))
+        return a
+
+    def substep_init_rungs(self, dtmax, nrung):
+        hist = np.zeros(32, dtype=np.int32)
+        self._chk(self.L.fvm_substep_init_rungs(self.h, float(dtmax), int(nrung), _ptr(hist)))
+        return hist
+
+    def substep_run(self, rung_mask, t_global, dtmax, min_rung, nrung):
+        counts = np.zeros(5, dtype=np.int32)
+        hist = np.zeros(32, dtype=np.int32)
+        self._chk(self.L.fvm_substep_run(self.h, int(rung_mask), float(t_global), float(dtmax), int(min_rung),
+                                         int(nrung), _ptr(counts), _ptr(hist)))
+        return counts, hist
 
     # convenience: whole state in / out
     def upload_state(self, st):

@@ -64,21 +64,26 @@ class RungClock:
     def tsys(self) -> float:
         return self.dt_tick * self.tsysU
 
-    def pull(self, rung: np.ndarray) -> tuple[np.ndarray, float]:
-        """Advance to the next synchronisation time; return (active mask, dt_global)."""
-        max_rung = int(rung.max())
+    def advance(self, max_rung: int) -> tuple[int, float]:
+        """Advance to the next synchronisation time of the finest occupied rung;
+        return (bitmask of the active rungs, bit r = rung r; dt_global)."""
         assert max_rung != RUNGMAX - 1
         dtU = 1 << (RUNGMAX - 1 - max_rung)
         tnextU = (self.tsysU // dtU + 1) * dtU
         flags = self.tsysU ^ tnextU
-        active = np.zeros(len(rung), dtype=bool)
+        mask = 0
         for i in range(RUNGMAX):
             if flags & (1 << i):
-                active |= rung == (RUNGMAX - 1 - i)
+                mask |= 1 << (RUNGMAX - 1 - i)
                 self.min_rung = RUNGMAX - 1 - i
         prev = self.tsysU
         self.tsysU = tnextU
-        return active, self.dt_tick * (self.tsysU - prev)
+        return mask, self.dt_tick * (self.tsysU - prev)
+
+    def pull(self, rung: np.ndarray) -> tuple[np.ndarray, float]:
+        """Advance to the next synchronisation time; return (active mask, dt_global)."""
+        mask, dt = self.advance(int(rung.max()))
+        return ((mask >> np.asarray(rung, dtype=np.int64)) & 1).astype(bool), dt
 
 
 def substep_layout(mesh, active: np.ndarray, bnd: np.ndarray):
@@ -291,6 +296,50 @@ class GpuBackend:
 
     def compute_fluid_update(self, st):
         self._timed(self.ctx.compute_fluid_update, False)
+
+
+class DeviceSubstepper:
+    """The Substepper with the per-cell state resident on the device.
+
+    The same substeps (Iterate.cpp:92-202) as ``Substepper`` with ``GpuBackend``,
+    but the layout, the gathers, the new rungs and the write-back run on the
+    device (``fvm_substep_run``); the host keeps only the rung clock, fed by
+    the device's rung histogram.  The mesh must be periodic (no boundary cells)."""
+
+    def __init__(self, mesh, W, gamma=5.0 / 3.0, courant=0.4, dtmax=1.0 / 64, limiter=0, nrung=6):
+        from . import gpu as G
+        self.mesh = mesh
+        self.gamma, self.courant, self.limiter, self.nrung = gamma, courant, limiter, nrung
+        self.clock = RungClock(dtmax)
+        self.ctx = G.Context(gamma=gamma, courant=courant, box=tuple(float(b) for b in mesh.box), limiter=limiter)
+        self.ctx.substep_setup(mesh.faces, mesh.cell_off, mesh.cell_face, mesh.volume)
+        W = np.array(W, dtype=np.float64)
+        self.ctx.substep_upload("w", W)
+        self.ctx.substep_upload("U", to_conservative(W, mesh.volume))
+        self.ctx.substep_upload("pos", mesh.pos)
+        self.ctx.substep_upload("mvol", mesh.volume)
+        self.hist = np.zeros(RUNGMAX, dtype=np.int32)
+        self.t_global = 0.0
+        self.history = []
+
+    def init_rungs(self, backend=None):
+        self.hist = self.ctx.substep_init_rungs(self.clock.dtmax, self.nrung)
+
+    def max_rung(self) -> int:
+        return int(np.nonzero(self.hist)[0].max())
+
+    def substep(self, backend=None):
+        """One substep; returns the number of active cells and active faces."""
+        mask, _ = self.clock.advance(self.max_rung())
+        self.t_global = self.clock.tsys()
+        counts, self.hist = self.ctx.substep_run(mask, self.t_global, self.clock.dtmax, self.clock.min_rung, self.nrung)
+        nact, nfa = int(counts[0]), int(counts[1])
+        self.history.append((self.t_global, nact, nfa))
+        return nact, nfa
+
+    def state(self, name):
+        """A global per-cell array in mesh order ("W", "U", "rung", field names)."""
+        return self.ctx.substep_download({"W": "w"}.get(name, name))
 
 
 def ic_disk(pos, gamma=5.0 / 3.0, H=0.1, beta=400.0, noise=0.01, seed=7, rsoft=0.05, r_out=0.4):

@@ -208,6 +208,36 @@ int fvm_fused_sync(fvm_ctx *ctx);
 int fvm_fused_timing(fvm_ctx *ctx, int enable);
 int fvm_fused_kernel_ms(fvm_ctx *ctx, double *ms, long long *launches, int n);
 
+/* Individual timesteps with the per-cell state resident on the device, in
+ * mesh order.  One fvm_substep_run is one substep of the reference's driver
+ * loop (Iterate.cpp:92-202): the local layout of active cells and their
+ * inactive neighbours with faces turned so that s1 is active
+ * (localMesh.cpp:216-322, :497-591), the prediction of the neighbours to the
+ * current time (Iterate.cpp:99-106), the path (computePrimitives,
+ * computeTimestep, computeTimestepLimiter, computePredictor,
+ * computeReconstruction, slopeLimiter, computeFluidUpdate), the new rungs of
+ * the active cells (Scheduler.h get_rung, Iterate.cpp:146-168) and the return
+ * of the neighbours' dU to their owners (computeFluidUpdate.cpp:320-345).  The
+ * rung clock (Scheduler.h pull) stays with the caller, which passes the
+ * bitmask of the rungs that are active (bit r = rung r), the time, dtmax and
+ * min_rung.  The mesh is the whole periodic mesh; it and the context's mesh
+ * and fields are overwritten.  Needs FVM_STORAGE_F64.
+ *   counts[5]: active cells, active faces, inactive neighbours, local faces,
+ *              1 if the fluid update reported a non-positive state
+ *   rung_hist[32]: cells per rung after the substep */
+#define FVM_SUBSTEP_RUNG FVM_NFIELDS /* int rung per cell, for upload/download */
+int fvm_substep_setup(fvm_ctx *ctx, int ncell, int nface, const fvm_face *faces, const int *cell_face_off,
+                      const int *cell_face, const double *cell_volume);
+/* per-cell fields kept between substeps (W, gradients, Wt, POS, J, TEND, BND,
+ * MINMAX_LO/HI, DU, DGRADPSI, DDIVB, U, MESH_VOLUME, TBEG, DT_NEW) and the
+ * rungs, host layout [ncell][width] in mesh order */
+int fvm_substep_upload(fvm_ctx *ctx, int field, const void *host, int ncell);
+int fvm_substep_download(fvm_ctx *ctx, int field, void *host, int ncell);
+/* first rungs from the CFL timestep of every cell (System.cpp:255-300, per cell) */
+int fvm_substep_init_rungs(fvm_ctx *ctx, double dtmax, int nrung, int *rung_hist);
+int fvm_substep_run(fvm_ctx *ctx, unsigned rung_mask, double t_global, double dtmax, int min_rung, int nrung,
+                    int *counts, int *rung_hist);
+
 /* read+write bandwidth of a plain 16-byte-vector copy of `bytes` bytes, GB/s */
 int fvm_measure_copy_bandwidth(fvm_ctx *ctx, long long bytes, int iters, double *gbps);
 
