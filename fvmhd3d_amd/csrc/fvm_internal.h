@@ -62,6 +62,12 @@ struct SubstepState {
   DBuf g[FVM_NFIELDS];                        // global fields, SoA [k*n + i]
   DBuf rung;                                  // int per cell
   DBuf act, apos, fmark, fpos, fl, cmark, iflag, ipos, g2l, perm, len, hist, err;
+  DBuf sdti, hot, hpos, hlist, chg;  // timestep-limiter sweep: local dti, hot cells, changes
+  // host side of the sweep: the global adjacency (neighbour per face-end) and
+  // per-cell scratch that is cleared after every substep
+  std::vector<int> hoff, hnbr, g2lh, permh, hoth, chg_idx;
+  std::vector<unsigned char> acth, queued, changed;
+  std::vector<double> dtih, chg_val;
 };
 
 struct fvm_ctx {

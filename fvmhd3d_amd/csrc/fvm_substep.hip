@@ -16,6 +16,8 @@
 
 #include <cmath>
 #include <cstring>
+#include <functional>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -253,6 +255,99 @@ __global__ void k_rung_hist(const int *__restrict__ rung, int n, int *__restrict
   if (threadIdx.x < kRungMax && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
 
+// initial values of the timestep-limiter sweep in the substep layout
+// (computeTimestep.cpp:64-110): dt_new for active cells, tend - tbeg for the
+// inactive neighbours.  The sweep's final write gives every neighbour its
+// value, so dt_new takes it here; the cells the sweep changes are patched after.
+__global__ void k_sweep_init(double *__restrict__ dt_new, const double *__restrict__ tend,
+                             const double *__restrict__ tbeg, int npl, int nA, double *__restrict__ sdti) {
+  const int l = gidx();
+  if (l >= npl) return;
+  const double v = l < nA ? dt_new[l] : tend[l] - tbeg[l];
+  sdti[l] = v;
+  if (l >= nA) dt_new[l] = v;
+}
+
+// active cells with a face whose visit changes a value at the initial values:
+// the visit of (i, j) sets dti[j] when 2 dti[i] < dti[j] and dti[i] when
+// 2 dti[j] < dti[i]; with neither it changes nothing
+__global__ void k_hot(const int *__restrict__ coff, const int *__restrict__ cface, const int *__restrict__ s1,
+                      const int *__restrict__ s2, const double *__restrict__ sdti, int nA, int *__restrict__ hot) {
+  const int l = gidx();
+  if (l >= nA) return;
+  const double di = sdti[l];
+  int h = 0;
+  for (int e = coff[l]; e < coff[l + 1]; e++) {
+    const int f = cface[e];
+    const int j = (l == s1[f]) ? s2[f] : s1[f];
+    const double dj = sdti[j];
+    h |= (2.0 * di < dj) || (2.0 * dj < di);
+  }
+  hot[l] = h;
+}
+
+__global__ void k_apply_changes(const int *__restrict__ idx, const double *__restrict__ val, int n,
+                                double *__restrict__ dt_new) {
+  const int k = gidx();
+  if (k >= n) return;
+  dt_new[idx[k]] = val[k];
+}
+
+// The sequential sweep of computeTimestep.cpp:64-110 over a whole mesh:
+// active cells in ascending index, each cell's neighbours in list order,
+//   dti[j] = min(dti[j], 2 dti[i]);  dti[i] = min(dti[i], 2 dti[j]).
+// A visit can change something only if a face of the visited cell has a
+// ratio above 2 at the initial values ("hot"), or the cell or one of its
+// neighbours changed earlier in the sweep.  So the sweep visits, in ascending
+// order, the hot cells and every active cell after the current one that is,
+// or neighbours, a cell whose value just changed; every other visit is a
+// no-op.  The result equals the full sweep's bit for bit.
+// off/nbr: neighbour of every face-end; act: 1 for active cells; hot:
+// ascending hot cells; queued/changed: zero on entry and on return;
+// changed_list receives the cells whose value changed.
+void sparse_sweep(const int *off, const int *nbr, const unsigned char *act, const int *hot, int nhot, double *dti,
+                  unsigned char *queued, unsigned char *changed, std::vector<int> &changed_list) {
+  std::priority_queue<int, std::vector<int>, std::greater<int>> q;
+  auto push = [&](int a) {
+    if (!queued[a]) {
+      queued[a] = 1;
+      q.push(a);
+    }
+  };
+  // value of c changed while visiting g: c and its active neighbours after g
+  // must be visited
+  auto touched = [&](int c, int g) {
+    if (!changed[c]) {
+      changed[c] = 1;
+      changed_list.push_back(c);
+    }
+    if (act[c] && c > g) push(c);
+    for (int e = off[c]; e < off[c + 1]; e++) {
+      const int a = nbr[e];
+      if (act[a] && a > g) push(a);
+    }
+  };
+  for (int k = 0; k < nhot; k++) push(hot[k]);
+  const double scale_fac = 2.0;
+  while (!q.empty()) {
+    const int g = q.top();
+    q.pop();
+    queued[g] = 0;
+    for (int e = off[g]; e < off[g + 1]; e++) {
+      const int j = nbr[e];
+      if (scale_fac * dti[g] < dti[j]) {
+        dti[j] = scale_fac * dti[g];
+        touched(j, g);
+      }
+      if (scale_fac * dti[j] < dti[g]) {
+        dti[g] = scale_fac * dti[j];
+        touched(g, g);
+      }
+    }
+  }
+  for (int c : changed_list) changed[c] = 0;
+}
+
 }  // namespace
 
 #define SCHK(c, call) FVM_HIPCHK(c, call)
@@ -267,10 +362,11 @@ __global__ void k_rung_hist(const int *__restrict__ rung, int n, int *__restrict
 void fvm_substep_release(fvm_ctx *c) {
   SubstepState &s = c->sub;
   for (DBuf *b : {&s.gs1, &s.gs2, &s.gc, &s.gn, &s.goff, &s.gface, &s.gvol, &s.rung, &s.act, &s.apos, &s.fmark,
-                  &s.fpos, &s.fl, &s.cmark, &s.iflag, &s.ipos, &s.g2l, &s.perm, &s.len, &s.hist, &s.err})
+                  &s.fpos, &s.fl, &s.cmark, &s.iflag, &s.ipos, &s.g2l, &s.perm, &s.len, &s.hist, &s.err, &s.sdti,
+                  &s.hot, &s.hpos, &s.hlist, &s.chg})
     fvm_release(*b);
   for (auto &b : s.g) fvm_release(b);
-  s.ready = false;
+  s = SubstepState{};
 }
 
 namespace {
@@ -421,6 +517,62 @@ int write_back(fvm_ctx *c, bool full) {
   return FVM_OK;
 }
 
+// the timestep limiter of a substep: initial values and hot cells on the
+// device, the sparse sweep on the host over the global adjacency, the changed
+// values back into dt_new
+int substep_sweep(fvm_ctx *c) {
+  SubstepState &s = c->sub;
+  hipStream_t st = c->stream;
+  const int npl = c->np, nA = c->nactive;
+  SLAUNCH(c, npl, k_sweep_init, P<double>(c->field[FVM_FIELD_DT_NEW]), P<double>(c->field[FVM_FIELD_TEND]),
+          P<double>(c->field[FVM_FIELD_TBEG]), npl, nA, P<double>(s.sdti));
+  SLAUNCH(c, nA, k_hot, P<int>(c->m_coff), P<int>(c->m_cface), P<int>(c->m_s1), P<int>(c->m_s2), P<double>(s.sdti), nA,
+          P<int>(s.hot));
+  int nhot = 0;
+  if (fvm::exclusive_scan_flags(c->scan, P<int>(s.hot), P<int>(s.hpos), nA, &nhot, st))
+    return fvm_fail(c, FVM_EHIP, "substep: scan of the limiter's cells failed");
+  if (nhot == 0) {
+    SCHK(c, hipStreamSynchronize(st));
+    return FVM_OK;
+  }
+  SLAUNCH(c, nA, k_compact, P<int>(s.hot), P<int>(s.hpos), nA, 0, P<int>(s.hlist));
+  std::vector<double> sd(npl);
+  s.permh.resize(npl);
+  s.hoth.resize(nhot);
+  SCHK(c, hipMemcpyAsync(sd.data(), s.sdti.p, 8 * (size_t)npl, hipMemcpyDeviceToHost, st));
+  SCHK(c, hipMemcpyAsync(s.permh.data(), s.perm.p, 4 * (size_t)npl, hipMemcpyDeviceToHost, st));
+  SCHK(c, hipMemcpyAsync(s.hoth.data(), s.hlist.p, 4 * (size_t)nhot, hipMemcpyDeviceToHost, st));
+  SCHK(c, hipStreamSynchronize(st));
+  for (int l = 0; l < npl; l++) {
+    const int g = s.permh[l];
+    s.dtih[g] = sd[l];
+    s.g2lh[g] = l;
+    if (l < nA) s.acth[g] = 1;
+  }
+  for (int k = 0; k < nhot; k++) s.hoth[k] = s.permh[s.hoth[k]];
+  std::vector<int> chg;
+  sparse_sweep(s.hoff.data(), s.hnbr.data(), s.acth.data(), s.hoth.data(), nhot, s.dtih.data(), s.queued.data(),
+               s.changed.data(), chg);
+  for (int l = 0; l < nA; l++) s.acth[s.permh[l]] = 0;
+  const int nchg = (int)chg.size();
+  if (nchg > 0) {
+    s.chg_idx.resize(nchg);
+    s.chg_val.resize(nchg);
+    for (int k = 0; k < nchg; k++) {
+      s.chg_idx[k] = s.g2lh[chg[k]];
+      s.chg_val[k] = s.dtih[chg[k]];
+    }
+    if (int rc = fvm_ensure(c, s.chg, 12 * (size_t)nchg + 8)) return rc;
+    int *di = P<int>(s.chg);
+    double *dv = (double *)((char *)s.chg.p + 4 * (((size_t)nchg + 1) & ~(size_t)1));
+    SCHK(c, hipMemcpyAsync(di, s.chg_idx.data(), 4 * (size_t)nchg, hipMemcpyHostToDevice, st));
+    SCHK(c, hipMemcpyAsync(dv, s.chg_val.data(), 8 * (size_t)nchg, hipMemcpyHostToDevice, st));
+    SLAUNCH(c, nchg, k_apply_changes, di, dv, nchg, P<double>(c->field[FVM_FIELD_DT_NEW]));
+  }
+  SCHK(c, hipStreamSynchronize(st));
+  return FVM_OK;
+}
+
 int check_ready(fvm_ctx *c) {
   if (!c) return FVM_EARG;
   if (!c->sub.ready) return fvm_fail(c, FVM_ESTATE, "substep call before fvm_substep_setup");
@@ -462,8 +614,25 @@ int fvm_substep_setup(fvm_ctx *c, int ncell, int nface, const fvm_face *faces, c
                            {&s.gface, 4 * (size_t)nref}, {&s.gvol, 8 * N}, {&s.rung, 4 * N}, {&s.act, 4 * N},
                            {&s.apos, 4 * N}, {&s.fmark, 4 * F}, {&s.fpos, 4 * F}, {&s.fl, 4 * F}, {&s.cmark, 4 * N},
                            {&s.iflag, 4 * N}, {&s.ipos, 4 * N}, {&s.g2l, 4 * N}, {&s.perm, 4 * N}, {&s.len, 4 * N},
-                           {&s.hist, 4 * kRungMax}, {&s.err, 4}})))
+                           {&s.hist, 4 * kRungMax}, {&s.err, 4}, {&s.sdti, 8 * N}, {&s.hot, 4 * N},
+                           {&s.hpos, 4 * N}, {&s.hlist, 4 * N}})))
     return rc;
+  try {
+    s.hoff.assign(cell_face_off, cell_face_off + N + 1);
+    s.hnbr.resize((size_t)nref);
+    for (int i = 0; i < ncell; i++)
+      for (int e = cell_face_off[i]; e < cell_face_off[i + 1]; e++) {
+        const fvm_face &Fc = faces[cell_face[e]];
+        s.hnbr[e] = (i == Fc.s1) ? Fc.s2 : Fc.s1;
+      }
+    s.g2lh.assign(N, -1);
+    s.acth.assign(N, 0);
+    s.queued.assign(N, 0);
+    s.changed.assign(N, 0);
+    s.dtih.assign(N, 0.0);
+  } catch (...) {
+    return fvm_fail(c, FVM_EARG, "fvm_substep_setup: host allocation failed");
+  }
   // the context's mesh and fields at their largest (every cell local)
   if ((rc = ensure_all(c, {{&c->m_s1, 4 * F}, {&c->m_s2, 4 * F}, {&c->m_c, 24 * F}, {&c->m_n, 24 * F},
                            {&c->m_coff, 4 * (N + 1)}, {&c->m_cface, 4 * (size_t)nref}, {&c->m_cvol, 8 * N}})))
@@ -579,7 +748,7 @@ int fvm_substep_run(fvm_ctx *c, unsigned rung_mask, double t_global, double dtma
   if ((rc = fvm_compute_primitives(c, 0))) return rc;
   double dtmin = 0.0;
   if ((rc = fvm_compute_timestep(c, &dtmin))) return rc;
-  if ((rc = fvm_compute_timestep_limiter(c))) return rc;
+  if ((rc = substep_sweep(c))) return rc;
   if ((rc = fvm_compute_predictor(c, dtmax, min_rung))) return rc;
   if ((rc = update_rungs(c, dtmax, min_rung, nrung, t_global, nullptr))) return rc;
   if ((rc = fvm_compute_reconstruction(c))) return rc;
@@ -595,6 +764,36 @@ int fvm_substep_run(fvm_ctx *c, unsigned rung_mask, double t_global, double dtma
     counts[4] = pos == FVM_EPOS;
   }
   return rung_histogram(c, rung_hist);
+}
+
+int fvm_timestep_limiter_sweep(int ncell, const int *cell_off, const int *nbr, const unsigned char *active,
+                               double *dti) {
+  if (ncell <= 0 || !cell_off || !active || !dti || cell_off[0] != 0) return FVM_EARG;
+  for (int i = 0; i < ncell; i++)
+    if (cell_off[i + 1] < cell_off[i]) return FVM_EARG;
+  const int nref = cell_off[ncell];
+  if (nref > 0 && !nbr) return FVM_EARG;
+  for (int e = 0; e < nref; e++)
+    if (nbr[e] < 0 || nbr[e] >= ncell) return FVM_EARG;
+  try {
+    std::vector<int> hot;
+    for (int i = 0; i < ncell; i++) {
+      if (!active[i]) continue;
+      bool h = false;
+      for (int e = cell_off[i]; e < cell_off[i + 1]; e++) {
+        const double di = dti[i], dj = dti[nbr[e]];
+        h = h || (2.0 * di < dj) || (2.0 * dj < di);
+      }
+      if (h) hot.push_back(i);
+    }
+    std::vector<unsigned char> act(active, active + ncell), queued(ncell, 0), changed(ncell, 0);
+    for (auto &a : act) a = a ? 1 : 0;
+    std::vector<int> chg;
+    sparse_sweep(cell_off, nbr, act.data(), hot.data(), (int)hot.size(), dti, queued.data(), changed.data(), chg);
+  } catch (...) {
+    return FVM_EARG;
+  }
+  return FVM_OK;
 }
 
 }  // extern "C"

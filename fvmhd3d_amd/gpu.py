@@ -84,6 +84,7 @@ SIGNATURES = {
     "fvm_sg_create": [ctypes.POINTER(ctypes.c_void_p), _I, _D],
     "fvm_sg_destroy": [_P],
     "fvm_substep_setup": [_P, _I, _I, _P, _P, _P, _P],
+    "fvm_timestep_limiter_sweep": [_I, _P, _P, _P, _P],
     "fvm_substep_upload": [_P, _I, _P, _I],
     "fvm_substep_download": [_P, _I, _P, _I],
     "fvm_substep_init_rungs": [_P, _D, _I, _P],
@@ -132,6 +133,18 @@ def lib():
         L.fvm_sg_destroy.restype = None
         _lib = L
     return _lib
+
+
+def timestep_limiter_sweep(cell_off, nbr, active, dti):
+    """fvm_timestep_limiter_sweep on host arrays (no GPU): returns the swept dti."""
+    cell_off = np.ascontiguousarray(cell_off, dtype=np.int32)
+    nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+    active = np.ascontiguousarray(active, dtype=np.uint8)
+    out = np.array(dti, dtype=np.float64)
+    rc = lib().fvm_timestep_limiter_sweep(len(cell_off) - 1, _ptr(cell_off), _ptr(nbr), _ptr(active), _ptr(out))
+    if rc != FVM_OK:
+        raise FvmError(rc, "fvm_timestep_limiter_sweep: bad arguments")
+    return out
 
 
 def _ptr(a):

@@ -238,6 +238,15 @@ int fvm_substep_init_rungs(fvm_ctx *ctx, double dtmax, int nrung, int *rung_hist
 int fvm_substep_run(fvm_ctx *ctx, unsigned rung_mask, double t_global, double dtmax, int min_rung, int nrung,
                     int *counts, int *rung_hist);
 
+/* The sequential timestep-limiter sweep of computeTimestep.cpp:64-110 on host
+ * arrays of a whole mesh (no context, no GPU): active cells in ascending index,
+ * each cell's neighbours nbr[cell_off[i] .. cell_off[i+1]) in list order,
+ * dti[j] = min(dti[j], 2 dti[i]) then dti[i] = min(dti[i], 2 dti[j]).  Visits
+ * only the cells a change can reach (the same routine fvm_substep_run uses);
+ * the result equals the full sweep's.  dti is updated in place. */
+int fvm_timestep_limiter_sweep(int ncell, const int *cell_off, const int *nbr, const unsigned char *active,
+                               double *dti);
+
 /* read+write bandwidth of a plain 16-byte-vector copy of `bytes` bytes, GB/s */
 int fvm_measure_copy_bandwidth(fvm_ctx *ctx, long long bytes, int iters, double *gbps);
 

@@ -145,3 +145,38 @@ def test_device_substeps_match_host_driver():
         assert np.array_equal(ref.view(np.uint64), got.view(np.uint64)), host
     # some substeps were partial (the layout with inactive neighbours was exercised)
     assert any(0 < h[1] < m.ncell for h in Sd.history)
+
+
+@pytest.mark.parametrize("spread,frac", [(0.05, 0.3), (0.35, 0.5), (1.0, 0.2), (2.0, 1.0), (1.5, 0.05)])
+def test_sparse_timestep_limiter_sweep_matches_oracle(spread, frac):
+    """The sweep that fvm_substep_run uses (visiting only the cells a change can
+    reach; fvm_timestep_limiter_sweep runs it on host arrays) against the
+    oracle's full sequential sweep (computeTimestep.cpp:64-110) on the
+    reference layout of a substep: dt_new of every layout cell, bit for bit.
+    Spreads from almost no hot faces to most faces hot, and partial and full
+    active sets."""
+    from fvmhd3d_amd import gpu as G
+    m = random_mesh(1500)
+    rng = np.random.default_rng(int(spread * 100) + int(frac * 10))
+    active = rng.random(m.ncell) < frac
+    active[0] = True
+    S = Substepper(m, ic_disk(m.pos), dtmax=1.0 / 64)
+    dt = 1e-3 * np.exp(spread * rng.standard_normal(m.ncell))
+    S.dt_new[:] = dt
+    S.tbeg[:] = rng.uniform(0.0, 1e-3, m.ncell)
+    S.tend[:] = S.tbeg + 1e-3 * np.exp(spread * rng.standard_normal(m.ncell))
+    st, perm = S._state(active)
+    nact = st.nactive
+    OracleBackend().compute_timestep_limiter(st)
+    # global form: neighbour of every face-end, initial values as in the layout
+    coff = m.cell_off
+    i_of = np.repeat(np.arange(m.ncell), np.diff(coff))
+    f = m.cell_face
+    nbr = np.where(m.faces["s1"][f] == i_of, m.faces["s2"][f], m.faces["s1"][f])
+    dti = np.where(active, S.dt_new, S.tend - S.tbeg)
+    got = G.timestep_limiter_sweep(coff, nbr, active, dti)
+    assert np.array_equal(st.dt_new.view(np.uint64), got[perm].view(np.uint64))
+    changed = np.sum(got[perm] != dti[perm])
+    print(f"spread {spread} active {nact}: {changed} of {len(perm)} layout cells changed")
+    if spread >= 1.0:
+        assert changed > 0
