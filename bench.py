@@ -194,6 +194,8 @@ def main():
                     help="the learned HLLD surrogate against the FP64/FP32 HLLD on harvested blast faces")
     ap.add_argument("--substeps", type=int, default=0,
                     help="individual timesteps: this many substeps of the 2M-generator disk (6 rungs)")
+    ap.add_argument("--no-host-driver", action="store_true",
+                    help="with --substeps: skip the host-layout driver run (comparison and bit-identity check)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -205,7 +207,7 @@ def main():
         print(json.dumps(surrogate_bench(args.steps)), flush=True)
         return
     if args.substeps:
-        print(json.dumps(substep_bench(args.substeps)), flush=True)
+        print(json.dumps(substep_bench(args.substeps, host_driver=not args.no_host_driver)), flush=True)
         return
     if args.copy_only:
         c = G.Context()

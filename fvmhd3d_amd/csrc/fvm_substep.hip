@@ -66,7 +66,7 @@ bool kept(int f) {
 // kernels
 
 // active flag from the rung bitmask; an active cell must end its step now
-__global__ void k_active(const int *__restrict__ rung, const double *__restrict__ tend, int n, unsigned mask,
+__global__ void k_sub_active(const int *__restrict__ rung, const double *__restrict__ tend, int n, unsigned mask,
                          double t_global, int check, int *__restrict__ act, int *__restrict__ err) {
   const int i = gidx();
   if (i >= n) return;
@@ -76,24 +76,24 @@ __global__ void k_active(const int *__restrict__ rung, const double *__restrict_
 }
 
 // out[base + pos[i]] = i for the flagged entries (ascending order)
-__global__ void k_compact(const int *__restrict__ flag, const int *__restrict__ pos, int n, int base,
+__global__ void k_sub_compact(const int *__restrict__ flag, const int *__restrict__ pos, int n, int base,
                           int *__restrict__ out) {
   const int i = gidx();
   if (i >= n || !flag[i]) return;
   out[base + pos[i]] = i;
 }
 
-__global__ void k_fill_i(int *__restrict__ p, long long n, int v) {
+__global__ void k_sub_fill_i(int *__restrict__ p, long long n, int v) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
-__global__ void k_fill_d(double *__restrict__ p, long long n, double v) {
+__global__ void k_sub_fill_d(double *__restrict__ p, long long n, double v) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
 // mark the faces of the active cells
-__global__ void k_mark_faces(const int *__restrict__ A, int nA, const int *__restrict__ goff,
+__global__ void k_sub_mark_faces(const int *__restrict__ A, int nA, const int *__restrict__ goff,
                              const int *__restrict__ gface, int *__restrict__ fmark) {
   const int j = gidx();
   if (j >= nA) return;
@@ -102,7 +102,7 @@ __global__ void k_mark_faces(const int *__restrict__ A, int nA, const int *__res
 }
 
 // mark both cells of every marked face
-__global__ void k_mark_cells(const int *__restrict__ fl, int nfl, const int *__restrict__ gs1,
+__global__ void k_sub_mark_cells(const int *__restrict__ fl, int nfl, const int *__restrict__ gs1,
                              const int *__restrict__ gs2, int *__restrict__ cmark) {
   const int j = gidx();
   if (j >= nfl) return;
@@ -111,7 +111,7 @@ __global__ void k_mark_cells(const int *__restrict__ fl, int nfl, const int *__r
   cmark[gs2[f]] = 1;
 }
 
-__global__ void k_inactive_flags(const int *__restrict__ cmark, const int *__restrict__ act, int n,
+__global__ void k_sub_inactive_flags(const int *__restrict__ cmark, const int *__restrict__ act, int n,
                                  int *__restrict__ iflag) {
   const int i = gidx();
   if (i >= n) return;
@@ -119,7 +119,7 @@ __global__ void k_inactive_flags(const int *__restrict__ cmark, const int *__res
 }
 
 // global -> local index of every cell in the layout
-__global__ void k_g2l(const int *__restrict__ act, const int *__restrict__ apos, const int *__restrict__ iflag,
+__global__ void k_sub_g2l(const int *__restrict__ act, const int *__restrict__ apos, const int *__restrict__ iflag,
                       const int *__restrict__ ipos, int n, int nA, int *__restrict__ g2l) {
   const int i = gidx();
   if (i >= n) return;
@@ -129,7 +129,7 @@ __global__ void k_g2l(const int *__restrict__ act, const int *__restrict__ apos,
 // local faces: the faces of the active cells in ascending global order,
 // turned so that s1 is active with the centroid moved to the periodic image
 // next to the new s1 (the same FP64 expressions as the host layout)
-__global__ void k_local_faces(const int *__restrict__ fl, int nfl, const int *__restrict__ gs1,
+__global__ void k_sub_local_faces(const int *__restrict__ fl, int nfl, const int *__restrict__ gs1,
                               const int *__restrict__ gs2, const double *__restrict__ gc,
                               const double *__restrict__ gn, int nface, const double *__restrict__ pos, int n,
                               const int *__restrict__ act, const int *__restrict__ g2l, double bx, double by,
@@ -157,14 +157,14 @@ __global__ void k_local_faces(const int *__restrict__ fl, int nfl, const int *__
   }
 }
 
-__global__ void k_face_counts(const int *__restrict__ A, int nA, const int *__restrict__ goff, int *__restrict__ len) {
+__global__ void k_sub_face_counts(const int *__restrict__ A, int nA, const int *__restrict__ goff, int *__restrict__ len) {
   const int j = gidx();
   if (j >= nA) return;
   len[j] = goff[A[j] + 1] - goff[A[j]];
 }
 
 // local cell lists (local face ids, in the order of the global list) and volumes
-__global__ void k_local_cells(const int *__restrict__ A, int nA, const int *__restrict__ goff,
+__global__ void k_sub_local_cells(const int *__restrict__ A, int nA, const int *__restrict__ goff,
                               const int *__restrict__ gface, const int *__restrict__ fpos,
                               const double *__restrict__ gvol, const int *__restrict__ coff, int *__restrict__ cface,
                               double *__restrict__ cvol) {
@@ -178,7 +178,7 @@ __global__ void k_local_cells(const int *__restrict__ A, int nA, const int *__re
 
 // local[k][i] = global[k][perm[i]] for i < ncopy, zero for ncopy <= i < npl
 template <typename S>
-__global__ void k_gather(const S *__restrict__ G, int n, S *__restrict__ L, int npl, const int *__restrict__ perm,
+__global__ void k_sub_gather(const S *__restrict__ G, int n, S *__restrict__ L, int npl, const int *__restrict__ perm,
                          int width, int ncopy) {
   const int i = gidx();
   if (i >= npl) return;
@@ -186,7 +186,7 @@ __global__ void k_gather(const S *__restrict__ G, int n, S *__restrict__ L, int 
   for (int k = 0; k < width; k++) SFLD(L, k, i, npl) = i < ncopy ? SFLD(G, k, g, n) : (S)0;
 }
 
-__global__ void k_flags(int *__restrict__ is_ngb, int *__restrict__ is_active, int npl, int nA) {
+__global__ void k_sub_flags(int *__restrict__ is_ngb, int *__restrict__ is_active, int npl, int nA) {
   const int i = gidx();
   if (i >= npl) return;
   is_ngb[i] = i >= nA;
@@ -194,7 +194,7 @@ __global__ void k_flags(int *__restrict__ is_ngb, int *__restrict__ is_active, i
 }
 
 // inactive neighbours predicted to the current time: w + Wt * (t - tbeg)
-__global__ void k_predict_inactive(double *__restrict__ w, const double *__restrict__ wt,
+__global__ void k_sub_predict_inactive(double *__restrict__ w, const double *__restrict__ wt,
                                    const double *__restrict__ tbeg, int npl, int nA, double t_global) {
   const int i = nA + gidx();
   if (i >= npl) return;
@@ -205,7 +205,7 @@ __global__ void k_predict_inactive(double *__restrict__ w, const double *__restr
 // new rungs of the active cells from dt_new (Scheduler.h get_rung with the
 // driver's clamp, Iterate.cpp:146-168): exponent difference to dtmax, at
 // least min_rung, at most nrung-1; tbeg/tend of the new step
-__global__ void k_rungs(const double *__restrict__ dt_new, double *__restrict__ tbeg, double *__restrict__ tend,
+__global__ void k_sub_rungs(const double *__restrict__ dt_new, double *__restrict__ tbeg, double *__restrict__ tend,
                         const int *__restrict__ perm, int nA, int emax, int min_rung, int nrung, double dt_tick,
                         double t_global, int *__restrict__ rung, int *__restrict__ err) {
   const int i = gidx();
@@ -224,7 +224,7 @@ __global__ void k_rungs(const double *__restrict__ dt_new, double *__restrict__ 
 
 // active cells: the owner takes the local values
 template <typename S>
-__global__ void k_scatter(const S *__restrict__ L, int npl, S *__restrict__ G, int n, const int *__restrict__ perm,
+__global__ void k_sub_scatter(const S *__restrict__ L, int npl, S *__restrict__ G, int n, const int *__restrict__ perm,
                           int nA, int width) {
   const int i = gidx();
   if (i >= nA) return;
@@ -233,7 +233,7 @@ __global__ void k_scatter(const S *__restrict__ L, int npl, S *__restrict__ G, i
 }
 
 // inactive neighbours: the owner adds the local dU (each owner appears once)
-__global__ void k_accumulate(const double *__restrict__ L, int npl, double *__restrict__ G, int n,
+__global__ void k_sub_accumulate(const double *__restrict__ L, int npl, double *__restrict__ G, int n,
                              const int *__restrict__ perm, int nA, int width) {
   const int i = nA + gidx();
   if (i >= npl) return;
@@ -242,7 +242,7 @@ __global__ void k_accumulate(const double *__restrict__ L, int npl, double *__re
 }
 
 // rung histogram of every cell (block-local counts first)
-__global__ void k_rung_hist(const int *__restrict__ rung, int n, int *__restrict__ hist) {
+__global__ void k_sub_rung_hist(const int *__restrict__ rung, int n, int *__restrict__ hist) {
   __shared__ int h[kRungMax];
   if (threadIdx.x < kRungMax) h[threadIdx.x] = 0;
   __syncthreads();
@@ -259,7 +259,7 @@ __global__ void k_rung_hist(const int *__restrict__ rung, int n, int *__restrict
 // (computeTimestep.cpp:64-110): dt_new for active cells, tend - tbeg for the
 // inactive neighbours.  The sweep's final write gives every neighbour its
 // value, so dt_new takes it here; the cells the sweep changes are patched after.
-__global__ void k_sweep_init(double *__restrict__ dt_new, const double *__restrict__ tend,
+__global__ void k_sub_sweep_init(double *__restrict__ dt_new, const double *__restrict__ tend,
                              const double *__restrict__ tbeg, int npl, int nA, double *__restrict__ sdti) {
   const int l = gidx();
   if (l >= npl) return;
@@ -271,7 +271,7 @@ __global__ void k_sweep_init(double *__restrict__ dt_new, const double *__restri
 // active cells with a face whose visit changes a value at the initial values:
 // the visit of (i, j) sets dti[j] when 2 dti[i] < dti[j] and dti[i] when
 // 2 dti[j] < dti[i]; with neither it changes nothing
-__global__ void k_hot(const int *__restrict__ coff, const int *__restrict__ cface, const int *__restrict__ s1,
+__global__ void k_sub_hot(const int *__restrict__ coff, const int *__restrict__ cface, const int *__restrict__ s1,
                       const int *__restrict__ s2, const double *__restrict__ sdti, int nA, int *__restrict__ hot) {
   const int l = gidx();
   if (l >= nA) return;
@@ -286,7 +286,7 @@ __global__ void k_hot(const int *__restrict__ coff, const int *__restrict__ cfac
   hot[l] = h;
 }
 
-__global__ void k_apply_changes(const int *__restrict__ idx, const double *__restrict__ val, int n,
+__global__ void k_sub_apply_changes(const int *__restrict__ idx, const double *__restrict__ val, int n,
                                 double *__restrict__ dt_new) {
   const int k = gidx();
   if (k >= n) return;
@@ -397,8 +397,8 @@ int build_layout(fvm_ctx *c, unsigned mask, double t_global, int check, int *cou
   SubstepState &s = c->sub;
   const int n = s.n, nface = s.nface;
   hipStream_t st = c->stream;
-  SLAUNCH(c, 1, k_fill_i, P<int>(s.err), 1, 0);
-  SLAUNCH(c, n, k_active, P<int>(s.rung), P<double>(s.g[FVM_FIELD_TEND]), n, mask, t_global, check, P<int>(s.act),
+  SLAUNCH(c, 1, k_sub_fill_i, P<int>(s.err), 1, 0);
+  SLAUNCH(c, n, k_sub_active, P<int>(s.rung), P<double>(s.g[FVM_FIELD_TEND]), n, mask, t_global, check, P<int>(s.act),
           P<int>(s.err));
   int nA = 0, nfl = 0, nI = 0, nref = 0;
   if (fvm::exclusive_scan_flags(c->scan, P<int>(s.act), P<int>(s.apos), n, &nA, st))
@@ -406,35 +406,35 @@ int build_layout(fvm_ctx *c, unsigned mask, double t_global, int check, int *cou
   int e = 0;
   if (int rc = read_err(c, &e)) return rc;
   if (e & 1) return fvm_fail(c, FVM_ESTATE, "substep: an active cell does not end its step at the current time");
-  SLAUNCH(c, n, k_compact, P<int>(s.act), P<int>(s.apos), n, 0, P<int>(s.perm));
-  SLAUNCH(c, nface, k_fill_i, P<int>(s.fmark), nface, 0);
-  SLAUNCH(c, nA, k_mark_faces, P<int>(s.perm), nA, P<int>(s.goff), P<int>(s.gface), P<int>(s.fmark));
+  SLAUNCH(c, n, k_sub_compact, P<int>(s.act), P<int>(s.apos), n, 0, P<int>(s.perm));
+  SLAUNCH(c, nface, k_sub_fill_i, P<int>(s.fmark), nface, 0);
+  SLAUNCH(c, nA, k_sub_mark_faces, P<int>(s.perm), nA, P<int>(s.goff), P<int>(s.gface), P<int>(s.fmark));
   if (fvm::exclusive_scan_flags(c->scan, P<int>(s.fmark), P<int>(s.fpos), nface, &nfl, st))
     return fvm_fail(c, FVM_EHIP, "substep: scan of faces failed");
-  SLAUNCH(c, nface, k_compact, P<int>(s.fmark), P<int>(s.fpos), nface, 0, P<int>(s.fl));
-  SLAUNCH(c, n, k_fill_i, P<int>(s.cmark), n, 0);
-  SLAUNCH(c, nfl, k_mark_cells, P<int>(s.fl), nfl, P<int>(s.gs1), P<int>(s.gs2), P<int>(s.cmark));
-  SLAUNCH(c, n, k_inactive_flags, P<int>(s.cmark), P<int>(s.act), n, P<int>(s.iflag));
+  SLAUNCH(c, nface, k_sub_compact, P<int>(s.fmark), P<int>(s.fpos), nface, 0, P<int>(s.fl));
+  SLAUNCH(c, n, k_sub_fill_i, P<int>(s.cmark), n, 0);
+  SLAUNCH(c, nfl, k_sub_mark_cells, P<int>(s.fl), nfl, P<int>(s.gs1), P<int>(s.gs2), P<int>(s.cmark));
+  SLAUNCH(c, n, k_sub_inactive_flags, P<int>(s.cmark), P<int>(s.act), n, P<int>(s.iflag));
   if (fvm::exclusive_scan_flags(c->scan, P<int>(s.iflag), P<int>(s.ipos), n, &nI, st))
     return fvm_fail(c, FVM_EHIP, "substep: scan of neighbour cells failed");
-  SLAUNCH(c, n, k_compact, P<int>(s.iflag), P<int>(s.ipos), n, nA, P<int>(s.perm));
-  SLAUNCH(c, n, k_g2l, P<int>(s.act), P<int>(s.apos), P<int>(s.iflag), P<int>(s.ipos), n, nA, P<int>(s.g2l));
+  SLAUNCH(c, n, k_sub_compact, P<int>(s.iflag), P<int>(s.ipos), n, nA, P<int>(s.perm));
+  SLAUNCH(c, n, k_sub_g2l, P<int>(s.act), P<int>(s.apos), P<int>(s.iflag), P<int>(s.ipos), n, nA, P<int>(s.g2l));
   const int npl = nA + nI;
   // the context's mesh (capacity for the whole mesh was reserved at setup)
-  SLAUNCH(c, nfl, k_local_faces, P<int>(s.fl), nfl, P<int>(s.gs1), P<int>(s.gs2), P<double>(s.gc), P<double>(s.gn),
+  SLAUNCH(c, nfl, k_sub_local_faces, P<int>(s.fl), nfl, P<int>(s.gs1), P<int>(s.gs2), P<double>(s.gc), P<double>(s.gn),
           nface, P<double>(s.g[FVM_FIELD_POS]), n, P<int>(s.act), P<int>(s.g2l), c->prm.domain_size[0],
           c->prm.domain_size[1], c->prm.domain_size[2], P<int>(c->m_s1), P<int>(c->m_s2), P<double>(c->m_c),
           P<double>(c->m_n));
-  SLAUNCH(c, nA, k_face_counts, P<int>(s.perm), nA, P<int>(s.goff), P<int>(s.len));
+  SLAUNCH(c, nA, k_sub_face_counts, P<int>(s.perm), nA, P<int>(s.goff), P<int>(s.len));
   if (nA > 0) {
     if (fvm::exclusive_scan(c->scan, P<int>(s.len), P<int>(c->m_coff), nA, st))
       return fvm_fail(c, FVM_EHIP, "substep: scan of face counts failed");
     SCHK(c, hipMemcpyAsync(P<int>(c->m_coff) + nA, c->scan.total, sizeof(int), hipMemcpyDeviceToDevice, st));
     SCHK(c, hipMemcpyAsync(&nref, c->scan.total, sizeof(int), hipMemcpyDeviceToHost, st));
   } else {
-    SLAUNCH(c, 1, k_fill_i, P<int>(c->m_coff), 1, 0);
+    SLAUNCH(c, 1, k_sub_fill_i, P<int>(c->m_coff), 1, 0);
   }
-  SLAUNCH(c, nA, k_local_cells, P<int>(s.perm), nA, P<int>(s.goff), P<int>(s.gface), P<int>(s.fpos), P<double>(s.gvol),
+  SLAUNCH(c, nA, k_sub_local_cells, P<int>(s.perm), nA, P<int>(s.goff), P<int>(s.gface), P<int>(s.fpos), P<double>(s.gvol),
           P<int>(c->m_coff), P<int>(c->m_cface), P<double>(c->m_cvol));
   SCHK(c, hipStreamSynchronize(st));
   if (npl <= 0 || nref < 0 || nref > s.nref || nfl > nface)
@@ -455,17 +455,17 @@ int build_layout(fvm_ctx *c, unsigned mask, double t_global, int check, int *cou
       const bool fresh_dU = f == FVM_FIELD_DU || f == FVM_FIELD_DGRADPSI || f == FVM_FIELD_DDIVB;
       const int ncopy = fresh_dU ? nA : npl;
       if (kInt[f])
-        SLAUNCH(c, npl, k_gather<int>, P<int>(s.g[f]), n, (int *)L, npl, P<int>(s.perm), w, ncopy);
+        SLAUNCH(c, npl, k_sub_gather<int>, P<int>(s.g[f]), n, (int *)L, npl, P<int>(s.perm), w, ncopy);
       else
-        SLAUNCH(c, npl, k_gather<double>, P<double>(s.g[f]), n, (double *)L, npl, P<int>(s.perm), w, ncopy);
+        SLAUNCH(c, npl, k_sub_gather<double>, P<double>(s.g[f]), n, (double *)L, npl, P<int>(s.perm), w, ncopy);
     } else if (f == FVM_FIELD_LIMITER) {
-      SLAUNCH(c, (long long)w * npl, k_fill_d, (double *)L, (long long)w * npl, 1.0);
+      SLAUNCH(c, (long long)w * npl, k_sub_fill_d, (double *)L, (long long)w * npl, 1.0);
     } else if (!kInt[f]) {
-      SLAUNCH(c, (long long)w * npl, k_fill_d, (double *)L, (long long)w * npl, 0.0);
+      SLAUNCH(c, (long long)w * npl, k_sub_fill_d, (double *)L, (long long)w * npl, 0.0);
     }
   }
-  SLAUNCH(c, npl, k_flags, P<int>(c->field[FVM_FIELD_IS_NGB]), P<int>(c->field[FVM_FIELD_IS_ACTIVE]), npl, nA);
-  SLAUNCH(c, nI, k_predict_inactive, P<double>(c->field[FVM_FIELD_W]), P<double>(c->field[FVM_FIELD_DWDT]),
+  SLAUNCH(c, npl, k_sub_flags, P<int>(c->field[FVM_FIELD_IS_NGB]), P<int>(c->field[FVM_FIELD_IS_ACTIVE]), npl, nA);
+  SLAUNCH(c, nI, k_sub_predict_inactive, P<double>(c->field[FVM_FIELD_W]), P<double>(c->field[FVM_FIELD_DWDT]),
           P<double>(c->field[FVM_FIELD_TBEG]), npl, nA, t_global);
   if (counts) {
     counts[0] = nA;
@@ -485,7 +485,7 @@ int exponent_of(double x) {
 int update_rungs(fvm_ctx *c, double dtmax, int min_rung, int nrung, double t_global, int *hist) {
   SubstepState &s = c->sub;
   const double dt_tick = dtmax / (double)(1u << (kRungMax - 1));
-  SLAUNCH(c, c->nactive, k_rungs, P<double>(c->field[FVM_FIELD_DT_NEW]), P<double>(c->field[FVM_FIELD_TBEG]),
+  SLAUNCH(c, c->nactive, k_sub_rungs, P<double>(c->field[FVM_FIELD_DT_NEW]), P<double>(c->field[FVM_FIELD_TBEG]),
           P<double>(c->field[FVM_FIELD_TEND]), P<int>(s.perm), c->nactive, exponent_of(dtmax), min_rung, nrung, dt_tick,
           t_global, P<int>(s.rung), P<int>(s.err));
   int e = 0;
@@ -496,8 +496,8 @@ int update_rungs(fvm_ctx *c, double dtmax, int min_rung, int nrung, double t_glo
 
 int rung_histogram(fvm_ctx *c, int *hist) {
   SubstepState &s = c->sub;
-  SLAUNCH(c, kRungMax, k_fill_i, P<int>(s.hist), kRungMax, 0);
-  SLAUNCH(c, s.n, k_rung_hist, P<int>(s.rung), s.n, P<int>(s.hist));
+  SLAUNCH(c, kRungMax, k_sub_fill_i, P<int>(s.hist), kRungMax, 0);
+  SLAUNCH(c, s.n, k_sub_rung_hist, P<int>(s.rung), s.n, P<int>(s.hist));
   if (hist) SCHK(c, hipMemcpyAsync(hist, s.hist.p, sizeof(int) * kRungMax, hipMemcpyDeviceToHost, c->stream));
   SCHK(c, hipStreamSynchronize(c->stream));
   return FVM_OK;
@@ -508,11 +508,11 @@ int write_back(fvm_ctx *c, bool full) {
   const int n = s.n, npl = c->np, nA = c->nactive;
   for (int f : kBack) {
     if (!full && f != FVM_FIELD_TBEG && f != FVM_FIELD_TEND) continue;
-    SLAUNCH(c, nA, k_scatter<double>, P<double>(c->field[f]), npl, P<double>(s.g[f]), n, P<int>(s.perm), nA, kWidth[f]);
+    SLAUNCH(c, nA, k_sub_scatter<double>, P<double>(c->field[f]), npl, P<double>(s.g[f]), n, P<int>(s.perm), nA, kWidth[f]);
   }
   if (full)
     for (int f : kAccum)
-      SLAUNCH(c, npl - nA, k_accumulate, P<double>(c->field[f]), npl, P<double>(s.g[f]), n, P<int>(s.perm), nA, kWidth[f]);
+      SLAUNCH(c, npl - nA, k_sub_accumulate, P<double>(c->field[f]), npl, P<double>(s.g[f]), n, P<int>(s.perm), nA, kWidth[f]);
   SCHK(c, hipStreamSynchronize(c->stream));
   return FVM_OK;
 }
@@ -524,9 +524,9 @@ int substep_sweep(fvm_ctx *c) {
   SubstepState &s = c->sub;
   hipStream_t st = c->stream;
   const int npl = c->np, nA = c->nactive;
-  SLAUNCH(c, npl, k_sweep_init, P<double>(c->field[FVM_FIELD_DT_NEW]), P<double>(c->field[FVM_FIELD_TEND]),
+  SLAUNCH(c, npl, k_sub_sweep_init, P<double>(c->field[FVM_FIELD_DT_NEW]), P<double>(c->field[FVM_FIELD_TEND]),
           P<double>(c->field[FVM_FIELD_TBEG]), npl, nA, P<double>(s.sdti));
-  SLAUNCH(c, nA, k_hot, P<int>(c->m_coff), P<int>(c->m_cface), P<int>(c->m_s1), P<int>(c->m_s2), P<double>(s.sdti), nA,
+  SLAUNCH(c, nA, k_sub_hot, P<int>(c->m_coff), P<int>(c->m_cface), P<int>(c->m_s1), P<int>(c->m_s2), P<double>(s.sdti), nA,
           P<int>(s.hot));
   int nhot = 0;
   if (fvm::exclusive_scan_flags(c->scan, P<int>(s.hot), P<int>(s.hpos), nA, &nhot, st))
@@ -535,7 +535,7 @@ int substep_sweep(fvm_ctx *c) {
     SCHK(c, hipStreamSynchronize(st));
     return FVM_OK;
   }
-  SLAUNCH(c, nA, k_compact, P<int>(s.hot), P<int>(s.hpos), nA, 0, P<int>(s.hlist));
+  SLAUNCH(c, nA, k_sub_compact, P<int>(s.hot), P<int>(s.hpos), nA, 0, P<int>(s.hlist));
   std::vector<double> sd(npl);
   s.permh.resize(npl);
   s.hoth.resize(nhot);
@@ -567,7 +567,7 @@ int substep_sweep(fvm_ctx *c) {
     double *dv = (double *)((char *)s.chg.p + 4 * (((size_t)nchg + 1) & ~(size_t)1));
     SCHK(c, hipMemcpyAsync(di, s.chg_idx.data(), 4 * (size_t)nchg, hipMemcpyHostToDevice, st));
     SCHK(c, hipMemcpyAsync(dv, s.chg_val.data(), 8 * (size_t)nchg, hipMemcpyHostToDevice, st));
-    SLAUNCH(c, nchg, k_apply_changes, di, dv, nchg, P<double>(c->field[FVM_FIELD_DT_NEW]));
+    SLAUNCH(c, nchg, k_sub_apply_changes, di, dv, nchg, P<double>(c->field[FVM_FIELD_DT_NEW]));
   }
   SCHK(c, hipStreamSynchronize(st));
   return FVM_OK;
