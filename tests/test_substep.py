@@ -180,3 +180,38 @@ def test_sparse_timestep_limiter_sweep_matches_oracle(spread, frac):
     print(f"spread {spread} active {nact}: {changed} of {len(perm)} layout cells changed")
     if spread >= 1.0:
         assert changed > 0
+
+
+def test_sparse_sweep_rejects_bad_adjacency():
+    from fvmhd3d_amd import gpu as G
+    off = np.array([0, 1, 2], dtype=np.int32)
+    with pytest.raises(G.FvmError):
+        G.timestep_limiter_sweep(off, np.array([1, 2], dtype=np.int32), np.ones(2, np.uint8), np.ones(2))
+    with pytest.raises(G.FvmError):
+        G.timestep_limiter_sweep(np.array([0, 2, 1], dtype=np.int32), np.array([1, 0], dtype=np.int32),
+                                 np.ones(2, np.uint8), np.ones(2))
+    # two cells, both active: 1 -> 2 dt[0] caps dt[1]
+    got = G.timestep_limiter_sweep(off, np.array([1, 0], dtype=np.int32), np.ones(2, np.uint8), np.array([1.0, 5.0]))
+    assert got.tolist() == [1.0, 2.0]
+
+
+@pytest.mark.gpu
+def test_device_substep_rejects_inconsistent_calls():
+    """A rung mask that activates cells whose step does not end now is refused
+    (the host driver asserts the same), leaving the state usable; the
+    shipped-precision storage is refused at setup."""
+    from fvmhd3d_amd import gpu as G
+    from fvmhd3d_amd.substep import DeviceSubstepper
+    pos = M.uniform_random(8 ** 3, seed=2)
+    m = M.build_periodic(pos[M.morton_order(pos, (1, 1, 1))])
+    D = DeviceSubstepper(m, ic_disk(m.pos), dtmax=0.25)
+    D.init_rungs()
+    W0 = D.state("W")
+    with pytest.raises(G.FvmError):
+        D.ctx.substep_run(0xFFFFFFFF, 0.0, 0.25, 0, 6)
+    assert np.array_equal(W0, D.state("W"))
+    nact, nfa = D.substep()
+    assert nact > 0 and nfa > 0
+    ctx = G.Context(storage=G.STORAGE_SHIPPED)
+    with pytest.raises(G.FvmError):
+        ctx.substep_setup(m.faces, m.cell_off, m.cell_face, m.volume)
