@@ -63,11 +63,16 @@ struct SubstepState {
   DBuf rung;                                  // int per cell
   DBuf act, apos, fmark, fpos, fl, cmark, iflag, ipos, g2l, perm, len, hist, err;
   DBuf sdti, hot, hpos, hlist, chg;  // timestep-limiter sweep: local dti, hot cells, changes
-  // host side of the sweep: the global adjacency (neighbour per face-end) and
-  // per-cell scratch that is cleared after every substep
-  std::vector<int> hoff, hnbr, g2lh, permh, hoth, chg_idx;
-  std::vector<unsigned char> acth, queued, changed;
-  std::vector<double> dtih, chg_val;
+  DBuf dtig, actg;                   // the same in mesh order (large layouts)
+  // host side of the sweep: the global adjacency (neighbour per face-end),
+  // pinned mirrors of dti and the active flags in mesh order (acth is all
+  // zero between substeps), and scratch
+  std::vector<int> hoff, hnbr, permh, hoth, chg_idx;
+  std::vector<unsigned char> changed;
+  std::vector<unsigned long long> pending;  // one bit per cell
+  std::vector<double> sdh, chg_val;
+  double *dtih = nullptr;       // pinned, n
+  unsigned char *acth = nullptr;  // pinned, n
 };
 
 struct fvm_ctx {

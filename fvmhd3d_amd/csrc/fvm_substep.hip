@@ -14,10 +14,9 @@
 // the host driver, so both give bit-identical states.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <functional>
-#include <queue>
 #include <string>
 #include <vector>
 
@@ -286,53 +285,80 @@ __global__ void k_sub_hot(const int *__restrict__ coff, const int *__restrict__ 
   hot[l] = h;
 }
 
-__global__ void k_sub_apply_changes(const int *__restrict__ idx, const double *__restrict__ val, int n,
-                                double *__restrict__ dt_new) {
+// changed values, by mesh cell, into the local dt_new
+__global__ void k_sub_apply_changes(const int *__restrict__ gidx_, const double *__restrict__ val, int n,
+                                    const int *__restrict__ g2l, double *__restrict__ dt_new) {
   const int k = gidx();
   if (k >= n) return;
-  dt_new[idx[k]] = val[k];
+  dt_new[g2l[gidx_[k]]] = val[k];
+}
+
+// the sweep's initial values and active flags in mesh order (the flags of
+// cells outside the layout were cleared before)
+__global__ void k_sub_sweep_global(const double *__restrict__ sdti, const int *__restrict__ perm, int npl, int nA,
+                                   double *__restrict__ dtig, unsigned char *__restrict__ actg) {
+  const int l = gidx();
+  if (l >= npl) return;
+  const int g = perm[l];
+  dtig[g] = sdti[l];
+  actg[g] = l < nA;
+}
+
+__global__ void k_sub_fill_u8(unsigned char *__restrict__ p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// hot cells as mesh cells
+__global__ void k_sub_map(int *__restrict__ list, int n, const int *__restrict__ perm) {
+  const int k = gidx();
+  if (k >= n) return;
+  list[k] = perm[list[k]];
 }
 
 // The sequential sweep of computeTimestep.cpp:64-110 over a whole mesh:
 // active cells in ascending index, each cell's neighbours in list order,
 //   dti[j] = min(dti[j], 2 dti[i]);  dti[i] = min(dti[i], 2 dti[j]).
-// A visit can change something only if a face of the visited cell has a
-// ratio above 2 at the initial values ("hot"), or the cell or one of its
-// neighbours changed earlier in the sweep.  So the sweep visits, in ascending
-// order, the hot cells and every active cell after the current one that is,
-// or neighbours, a cell whose value just changed; every other visit is a
+// A visit changes something only if one of the visited cell's faces exceeds
+// ratio 2 at the values of that moment.  So the sweep visits, in ascending
+// order, the cells with such a face at the initial values ("hot"), every
+// active cell after the current one whose own value changed, and every one
+// whose face with a changed cell then exceeds ratio 2; every other visit is a
 // no-op.  The result equals the full sweep's bit for bit.
 // off/nbr: neighbour of every face-end; act: 1 for active cells; hot:
-// ascending hot cells; queued/changed: zero on entry and on return;
-// changed_list receives the cells whose value changed.
+// ascending hot cells; pending (one bit per cell) and changed: zero on entry
+// and on return; changed_list receives the cells whose value changed.
+// Every cell queued lies after the one being visited, so the queue is a
+// bitset read forward from a word cursor.
 void sparse_sweep(const int *off, const int *nbr, const unsigned char *act, const int *hot, int nhot, double *dti,
-                  unsigned char *queued, unsigned char *changed, std::vector<int> &changed_list) {
-  std::priority_queue<int, std::vector<int>, std::greater<int>> q;
-  auto push = [&](int a) {
-    if (!queued[a]) {
-      queued[a] = 1;
-      q.push(a);
-    }
-  };
-  // value of c changed while visiting g: c and its active neighbours after g
-  // must be visited
+                  unsigned long long *pending, size_t nwords, unsigned char *changed, std::vector<int> &changed_list) {
+  auto push = [&](int a) { pending[(size_t)a >> 6] |= 1ull << (a & 63); };
+  const double scale_fac = 2.0;
+  // Invariant: every active cell after the one being visited that is not
+  // pending has all its faces within ratio 2 at the current values, so its
+  // visit would change nothing.  When the value of c changes while visiting
+  // g, c itself (if active and after g) is queued, and of its active
+  // neighbours after g those whose face with c now exceeds ratio 2.
   auto touched = [&](int c, int g) {
     if (!changed[c]) {
       changed[c] = 1;
       changed_list.push_back(c);
     }
     if (act[c] && c > g) push(c);
+    const double dc = dti[c];
     for (int e = off[c]; e < off[c + 1]; e++) {
       const int a = nbr[e];
-      if (act[a] && a > g) push(a);
+      if (act[a] && a > g && (scale_fac * dti[a] < dc || scale_fac * dc < dti[a])) push(a);
     }
   };
   for (int k = 0; k < nhot; k++) push(hot[k]);
-  const double scale_fac = 2.0;
-  while (!q.empty()) {
-    const int g = q.top();
-    q.pop();
-    queued[g] = 0;
+  size_t w = nhot > 0 ? (size_t)hot[0] >> 6 : nwords;
+  for (;;) {
+    while (w < nwords && pending[w] == 0) w++;
+    if (w >= nwords) break;
+    const int b = __builtin_ctzll(pending[w]);
+    pending[w] &= pending[w] - 1;
+    const int g = (int)(w * 64 + (size_t)b);
     for (int e = off[g]; e < off[g + 1]; e++) {
       const int j = nbr[e];
       if (scale_fac * dti[g] < dti[j]) {
@@ -363,9 +389,11 @@ void fvm_substep_release(fvm_ctx *c) {
   SubstepState &s = c->sub;
   for (DBuf *b : {&s.gs1, &s.gs2, &s.gc, &s.gn, &s.goff, &s.gface, &s.gvol, &s.rung, &s.act, &s.apos, &s.fmark,
                   &s.fpos, &s.fl, &s.cmark, &s.iflag, &s.ipos, &s.g2l, &s.perm, &s.len, &s.hist, &s.err, &s.sdti,
-                  &s.hot, &s.hpos, &s.hlist, &s.chg})
+                  &s.hot, &s.hpos, &s.hlist, &s.chg, &s.dtig, &s.actg})
     fvm_release(*b);
   for (auto &b : s.g) fvm_release(b);
+  if (s.dtih) (void)hipHostFree(s.dtih);
+  if (s.acth) (void)hipHostFree(s.acth);
   s = SubstepState{};
 }
 
@@ -519,15 +547,16 @@ int write_back(fvm_ctx *c, bool full) {
 
 // the timestep limiter of a substep: initial values and hot cells on the
 // device, the sparse sweep on the host over the global adjacency, the changed
-// values back into dt_new
+// values back into dt_new.  Large layouts hand the values over in mesh order
+// (one contiguous copy each into pinned memory); small ones per layout cell.
 int substep_sweep(fvm_ctx *c) {
   SubstepState &s = c->sub;
   hipStream_t st = c->stream;
-  const int npl = c->np, nA = c->nactive;
+  const int npl = c->np, nA = c->nactive, n = s.n;
   SLAUNCH(c, npl, k_sub_sweep_init, P<double>(c->field[FVM_FIELD_DT_NEW]), P<double>(c->field[FVM_FIELD_TEND]),
           P<double>(c->field[FVM_FIELD_TBEG]), npl, nA, P<double>(s.sdti));
-  SLAUNCH(c, nA, k_sub_hot, P<int>(c->m_coff), P<int>(c->m_cface), P<int>(c->m_s1), P<int>(c->m_s2), P<double>(s.sdti), nA,
-          P<int>(s.hot));
+  SLAUNCH(c, nA, k_sub_hot, P<int>(c->m_coff), P<int>(c->m_cface), P<int>(c->m_s1), P<int>(c->m_s2), P<double>(s.sdti),
+          nA, P<int>(s.hot));
   int nhot = 0;
   if (fvm::exclusive_scan_flags(c->scan, P<int>(s.hot), P<int>(s.hpos), nA, &nhot, st))
     return fvm_fail(c, FVM_EHIP, "substep: scan of the limiter's cells failed");
@@ -536,38 +565,47 @@ int substep_sweep(fvm_ctx *c) {
     return FVM_OK;
   }
   SLAUNCH(c, nA, k_sub_compact, P<int>(s.hot), P<int>(s.hpos), nA, 0, P<int>(s.hlist));
-  std::vector<double> sd(npl);
-  s.permh.resize(npl);
+  SLAUNCH(c, nhot, k_sub_map, P<int>(s.hlist), nhot, P<int>(s.perm));
   s.hoth.resize(nhot);
-  SCHK(c, hipMemcpyAsync(sd.data(), s.sdti.p, 8 * (size_t)npl, hipMemcpyDeviceToHost, st));
-  SCHK(c, hipMemcpyAsync(s.permh.data(), s.perm.p, 4 * (size_t)npl, hipMemcpyDeviceToHost, st));
   SCHK(c, hipMemcpyAsync(s.hoth.data(), s.hlist.p, 4 * (size_t)nhot, hipMemcpyDeviceToHost, st));
-  SCHK(c, hipStreamSynchronize(st));
-  for (int l = 0; l < npl; l++) {
-    const int g = s.permh[l];
-    s.dtih[g] = sd[l];
-    s.g2lh[g] = l;
-    if (l < nA) s.acth[g] = 1;
+  const bool global = (long long)npl * 8 > n;
+  if (global) {
+    SLAUNCH(c, n, k_sub_fill_u8, P<unsigned char>(s.actg), n);
+    SLAUNCH(c, npl, k_sub_sweep_global, P<double>(s.sdti), P<int>(s.perm), npl, nA, P<double>(s.dtig),
+            P<unsigned char>(s.actg));
+    SCHK(c, hipMemcpyAsync(s.dtih, s.dtig.p, 8 * (size_t)n, hipMemcpyDeviceToHost, st));
+    SCHK(c, hipMemcpyAsync(s.acth, s.actg.p, (size_t)n, hipMemcpyDeviceToHost, st));
+    SCHK(c, hipStreamSynchronize(st));
+  } else {
+    s.sdh.resize(npl);
+    s.permh.resize(npl);
+    SCHK(c, hipMemcpyAsync(s.sdh.data(), s.sdti.p, 8 * (size_t)npl, hipMemcpyDeviceToHost, st));
+    SCHK(c, hipMemcpyAsync(s.permh.data(), s.perm.p, 4 * (size_t)npl, hipMemcpyDeviceToHost, st));
+    SCHK(c, hipStreamSynchronize(st));
+    for (int l = 0; l < npl; l++) {
+      const int g = s.permh[l];
+      s.dtih[g] = s.sdh[l];
+      s.acth[g] = l < nA;
+    }
   }
-  for (int k = 0; k < nhot; k++) s.hoth[k] = s.permh[s.hoth[k]];
   std::vector<int> chg;
-  sparse_sweep(s.hoff.data(), s.hnbr.data(), s.acth.data(), s.hoth.data(), nhot, s.dtih.data(), s.queued.data(),
+  sparse_sweep(s.hoff.data(), s.hnbr.data(), s.acth, s.hoth.data(), nhot, s.dtih, s.pending.data(), s.pending.size(),
                s.changed.data(), chg);
-  for (int l = 0; l < nA; l++) s.acth[s.permh[l]] = 0;
+  // acth is all zero between substeps
+  if (global)
+    std::memset(s.acth, 0, (size_t)n);
+  else
+    for (int l = 0; l < nA; l++) s.acth[s.permh[l]] = 0;
   const int nchg = (int)chg.size();
   if (nchg > 0) {
-    s.chg_idx.resize(nchg);
     s.chg_val.resize(nchg);
-    for (int k = 0; k < nchg; k++) {
-      s.chg_idx[k] = s.g2lh[chg[k]];
-      s.chg_val[k] = s.dtih[chg[k]];
-    }
-    if (int rc = fvm_ensure(c, s.chg, 12 * (size_t)nchg + 8)) return rc;
+    for (int k = 0; k < nchg; k++) s.chg_val[k] = s.dtih[chg[k]];
+    if (int rc = fvm_ensure(c, s.chg, 12 * (size_t)nchg + 8)) return rc;  // reserved at setup for every cell
     int *di = P<int>(s.chg);
     double *dv = (double *)((char *)s.chg.p + 4 * (((size_t)nchg + 1) & ~(size_t)1));
-    SCHK(c, hipMemcpyAsync(di, s.chg_idx.data(), 4 * (size_t)nchg, hipMemcpyHostToDevice, st));
+    SCHK(c, hipMemcpyAsync(di, chg.data(), 4 * (size_t)nchg, hipMemcpyHostToDevice, st));
     SCHK(c, hipMemcpyAsync(dv, s.chg_val.data(), 8 * (size_t)nchg, hipMemcpyHostToDevice, st));
-    SLAUNCH(c, nchg, k_sub_apply_changes, di, dv, nchg, P<double>(c->field[FVM_FIELD_DT_NEW]));
+    SLAUNCH(c, nchg, k_sub_apply_changes, di, dv, nchg, P<int>(s.g2l), P<double>(c->field[FVM_FIELD_DT_NEW]));
   }
   SCHK(c, hipStreamSynchronize(st));
   return FVM_OK;
@@ -615,8 +653,12 @@ int fvm_substep_setup(fvm_ctx *c, int ncell, int nface, const fvm_face *faces, c
                            {&s.apos, 4 * N}, {&s.fmark, 4 * F}, {&s.fpos, 4 * F}, {&s.fl, 4 * F}, {&s.cmark, 4 * N},
                            {&s.iflag, 4 * N}, {&s.ipos, 4 * N}, {&s.g2l, 4 * N}, {&s.perm, 4 * N}, {&s.len, 4 * N},
                            {&s.hist, 4 * kRungMax}, {&s.err, 4}, {&s.sdti, 8 * N}, {&s.hot, 4 * N},
-                           {&s.hpos, 4 * N}, {&s.hlist, 4 * N}})))
+                           {&s.hpos, 4 * N}, {&s.hlist, 4 * N}, {&s.dtig, 8 * N}, {&s.actg, N},
+                           {&s.chg, 12 * N + 8}})))
     return rc;
+  SCHK(c, hipHostMalloc((void **)&s.dtih, 8 * N, hipHostMallocDefault));
+  SCHK(c, hipHostMalloc((void **)&s.acth, N, hipHostMallocDefault));
+  std::memset(s.acth, 0, N);
   try {
     s.hoff.assign(cell_face_off, cell_face_off + N + 1);
     s.hnbr.resize((size_t)nref);
@@ -625,17 +667,20 @@ int fvm_substep_setup(fvm_ctx *c, int ncell, int nface, const fvm_face *faces, c
         const fvm_face &Fc = faces[cell_face[e]];
         s.hnbr[e] = (i == Fc.s1) ? Fc.s2 : Fc.s1;
       }
-    s.g2lh.assign(N, -1);
-    s.acth.assign(N, 0);
-    s.queued.assign(N, 0);
+    s.pending.assign((N + 63) / 64, 0ull);
     s.changed.assign(N, 0);
-    s.dtih.assign(N, 0.0);
   } catch (...) {
     return fvm_fail(c, FVM_EARG, "fvm_substep_setup: host allocation failed");
   }
-  // the context's mesh and fields at their largest (every cell local)
+  // the context's mesh, fields and the path's scratch at their largest (every
+  // cell local), so that no substep grows a buffer
+  const size_t R = (size_t)nref;
   if ((rc = ensure_all(c, {{&c->m_s1, 4 * F}, {&c->m_s2, 4 * F}, {&c->m_c, 24 * F}, {&c->m_n, 24 * F},
-                           {&c->m_coff, 4 * (N + 1)}, {&c->m_cface, 4 * (size_t)nref}, {&c->m_cvol, 8 * N}})))
+                           {&c->m_coff, 4 * (N + 1)}, {&c->m_cface, 4 * R}, {&c->m_cvol, 8 * N},
+                           {&c->m_soff, 4 * (N + 1)}, {&c->m_sent, 8 * F + 4}, {&c->m_fact, 4 * (F + 1)},
+                           {&c->tmp0, 4 * (std::max(N, R) + 1)}, {&c->tmp1, 4 * (std::max(F, R) + 1)},
+                           {&c->tmp2, 4 * (R + 1)}, {&c->rec, 8 * 15 * F} /* 15-double face records */,
+                           {&c->swp, 4 * F}})))
     return rc;
   for (int f = 0; f < FVM_NFIELDS; f++) {
     const size_t bytes = (kInt[f] ? 4 : 8) * (size_t)kWidth[f] * N;
@@ -786,10 +831,12 @@ int fvm_timestep_limiter_sweep(int ncell, const int *cell_off, const int *nbr, c
       }
       if (h) hot.push_back(i);
     }
-    std::vector<unsigned char> act(active, active + ncell), queued(ncell, 0), changed(ncell, 0);
+    std::vector<unsigned char> act(active, active + ncell), changed(ncell, 0);
     for (auto &a : act) a = a ? 1 : 0;
+    std::vector<unsigned long long> pending(((size_t)ncell + 63) / 64, 0ull);
     std::vector<int> chg;
-    sparse_sweep(cell_off, nbr, act.data(), hot.data(), (int)hot.size(), dti, queued.data(), changed.data(), chg);
+    sparse_sweep(cell_off, nbr, act.data(), hot.data(), (int)hot.size(), dti, pending.data(), pending.size(),
+                 changed.data(), chg);
   } catch (...) {
     return FVM_EARG;
   }

@@ -215,3 +215,32 @@ def test_device_substep_rejects_inconsistent_calls():
     ctx = G.Context(storage=G.STORAGE_SHIPPED)
     with pytest.raises(G.FvmError):
         ctx.substep_setup(m.faces, m.cell_off, m.cell_face, m.volume)
+
+
+@pytest.mark.parametrize("frac", [1.0, 0.4])
+def test_sparse_sweep_long_cascades_match_oracle(frac):
+    """A few cells with a tiny dt in an otherwise uniform field: each change
+    makes the next face exceed ratio 2, so the sweep's changes run in long
+    chains through the mesh.  Compared with the oracle's full sweep bit for bit."""
+    from fvmhd3d_amd import gpu as G
+    m = random_mesh(1500)
+    rng = np.random.default_rng(11)
+    active = rng.random(m.ncell) < frac
+    active[[0, 700, 1499]] = True
+    S = Substepper(m, ic_disk(m.pos), dtmax=1.0 / 64)
+    S.dt_new[:] = 1e-2 * (1.0 + 0.1 * rng.random(m.ncell))
+    S.dt_new[[0, 700, 1499]] = [1e-7, 3e-8, 2e-6]
+    S.tbeg[:] = 0.0
+    S.tend[:] = 1e-2
+    st, perm = S._state(active)
+    OracleBackend().compute_timestep_limiter(st)
+    coff = m.cell_off
+    i_of = np.repeat(np.arange(m.ncell), np.diff(coff))
+    f = m.cell_face
+    nbr = np.where(m.faces["s1"][f] == i_of, m.faces["s2"][f], m.faces["s1"][f])
+    dti = np.where(active, S.dt_new, S.tend - S.tbeg)
+    got = G.timestep_limiter_sweep(coff, nbr, active, dti)
+    assert np.array_equal(st.dt_new.view(np.uint64), got[perm].view(np.uint64))
+    changed = np.sum(got[perm] != dti[perm])
+    print(f"active {st.nactive}: {changed} of {len(perm)} layout cells changed")
+    assert changed > 50
