@@ -830,6 +830,39 @@ __global__ void k_fill_int(int *p, int v, int n) {
   const int i = gid();
   if (i < n) p[i] = v;
 }
+
+// Cell lists from the face list as localMesh.cpp:586-589 builds them: every
+// face is listed by its s1 and, when s2 is active, by its s2, in the order the
+// faces were created (ascending index).  Count, scan, fill, then sort each
+// cell's short list.
+__global__ void k_csr_count(const int *__restrict__ s1, const int *__restrict__ s2, int nface, int nactive,
+                            int *__restrict__ cnt) {
+  const int f = gid();
+  if (f >= nface) return;
+  atomicAdd(&cnt[s1[f]], 1);
+  if (s2[f] < nactive) atomicAdd(&cnt[s2[f]], 1);
+}
+__global__ void k_csr_fill(const int *__restrict__ s1, const int *__restrict__ s2, int nface, int nactive,
+                           int *__restrict__ cursor, int *__restrict__ cface) {
+  const int f = gid();
+  if (f >= nface) return;
+  cface[atomicAdd(&cursor[s1[f]], 1)] = f;
+  if (s2[f] < nactive) cface[atomicAdd(&cursor[s2[f]], 1)] = f;
+}
+__global__ void k_csr_sort(const int *__restrict__ coff, int nactive, int *__restrict__ cface) {
+  const int i = gid();
+  if (i >= nactive) return;
+  const int b = coff[i], e = coff[i + 1];
+  for (int k = b + 1; k < e; k++) {
+    const int v = cface[k];
+    int m = k - 1;
+    while (m >= b && cface[m] > v) {
+      cface[m + 1] = cface[m];
+      m--;
+    }
+    cface[m + 1] = v;
+  }
+}
 // first position of each face in the concatenated face lists of active,
 // non-boundary cells
 __global__ void k_face_firstpos(MeshView M, const int *bnd, int nactive, int *firstpos) {
@@ -1125,6 +1158,75 @@ int fvm_upload_mesh(fvm_ctx *c, int np, int nactive, int nimport_loc, int nface,
   if (nactive > 0) HIPCHK(c, hipMemcpyAsync(c->m_cvol.p, cell_volume, sizeof(double) * (size_t)nactive, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->have_mesh = true;
+  return FVM_OK;
+}
+
+int fvm_upload_faces(fvm_ctx *c, int np, int nactive, int nimport_loc, int nface, const fvm_face *faces,
+                     const double *cell_volume) {
+  if (!c) return FVM_EARG;
+  if (np <= 0 || nactive < 0 || nactive > np || nimport_loc < 0 || nactive + nimport_loc > np || nface < 0)
+    return fail(c, FVM_EARG, "fvm_upload_faces: inconsistent sizes");
+  if ((nface > 0 && !faces) || (nactive > 0 && !cell_volume)) return fail(c, FVM_EARG, "fvm_upload_faces: NULL array");
+  // every face is created by an active cell (its s1); s2 may be any local cell
+  for (int f = 0; f < nface; f++)
+    if (faces[f].s1 < 0 || faces[f].s1 >= nactive || faces[f].s2 < 0 || faces[f].s2 >= np || faces[f].s1 == faces[f].s2)
+      return fail(c, FVM_EARG, "fvm_upload_faces: face cell index out of range (s1 must be active)");
+  (void)hipSetDevice(c->device);
+  const bool resize = np != c->np;
+  for (int f = 0; f < FVM_NFIELDS; f++) {
+    const size_t bytes = field_bytes(c, f, np);
+    if (resize || c->field[f].bytes < bytes) {
+      int rc = ensure(c, c->field[f], bytes);
+      if (rc) return rc;
+      HIPCHK(c, hipMemsetAsync(c->field[f].p, 0, bytes, c->stream));
+    }
+  }
+  int rc;
+  if ((rc = ensure(c, c->m_s1, sizeof(int) * (size_t)nface)) || (rc = ensure(c, c->m_s2, sizeof(int) * (size_t)nface)) ||
+      (rc = ensure(c, c->m_c, sizeof(double) * 3 * (size_t)nface)) || (rc = ensure(c, c->m_n, sizeof(double) * 3 * (size_t)nface)) ||
+      (rc = ensure(c, c->m_coff, sizeof(int) * (size_t)(nactive + 1))) || (rc = ensure(c, c->m_cface, sizeof(int) * 2 * (size_t)nface + 4)) ||
+      (rc = ensure(c, c->m_cvol, sizeof(double) * (size_t)nactive + 8)) || (rc = ensure(c, c->stage, sizeof(fvm_face) * (size_t)nface + 8)) ||
+      (rc = ensure(c, c->tmp0, sizeof(int) * (size_t)(nactive + 1))) || (rc = ensure(c, c->tmp1, sizeof(int) * (size_t)(nactive + 1))))
+    return rc;
+  int *cnt = (int *)c->tmp0.p, *cursor = (int *)c->tmp1.p;
+  LAUNCH(c, fvm::k_fill_int, nactive + 1, cnt, 0, nactive + 1);
+  if (nface > 0) {
+    HIPCHK(c, hipMemcpyAsync(c->stage.p, faces, sizeof(fvm_face) * (size_t)nface, hipMemcpyHostToDevice, c->stream));
+    LAUNCH(c, fvm::k_faces_to_soa, nface, (const fvm_face *)c->stage.p, nface, (int *)c->m_s1.p, (int *)c->m_s2.p,
+           (double *)c->m_c.p, (double *)c->m_n.p);
+    LAUNCH(c, fvm::k_csr_count, nface, (const int *)c->m_s1.p, (const int *)c->m_s2.p, nface, nactive, cnt);
+  }
+  if (fvm::exclusive_scan(c->scan, cnt, (int *)c->m_coff.p, nactive + 1, c->stream))
+    return fail(c, FVM_EHIP, "fvm_upload_faces: scan failed");
+  int nref = 0;
+  HIPCHK(c, hipMemcpyAsync(&nref, (int *)c->m_coff.p + nactive, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(cursor, c->m_coff.p, sizeof(int) * (size_t)(nactive + 1), hipMemcpyDeviceToDevice, c->stream));
+  if (nface > 0) {
+    LAUNCH(c, fvm::k_csr_fill, nface, (const int *)c->m_s1.p, (const int *)c->m_s2.p, nface, nactive, cursor,
+           (int *)c->m_cface.p);
+    LAUNCH(c, fvm::k_csr_sort, nactive, (const int *)c->m_coff.p, nactive, (int *)c->m_cface.p);
+  }
+  if (nactive > 0) HIPCHK(c, hipMemcpyAsync(c->m_cvol.p, cell_volume, sizeof(double) * (size_t)nactive, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->np = np;
+  c->nactive = nactive;
+  c->nimport_loc = nimport_loc;
+  c->nface = nface;
+  c->nref = nref;
+  c->nfa = -1;
+  c->have_mesh = true;
+  return FVM_OK;
+}
+
+int fvm_download_cell_faces(fvm_ctx *c, int *cell_face_off, int *cell_face) {
+  if (!c) return FVM_EARG;
+  if (!c->have_mesh) return fail(c, FVM_ESTATE, "fvm_download_cell_faces before a mesh upload");
+  if (!cell_face_off && !cell_face) return fail(c, FVM_EARG, "fvm_download_cell_faces: NULL arrays");
+  (void)hipSetDevice(c->device);
+  if (cell_face_off)
+    HIPCHK(c, hipMemcpy(cell_face_off, c->m_coff.p, sizeof(int) * (size_t)(c->nactive + 1), hipMemcpyDeviceToHost));
+  if (cell_face && c->nref > 0)
+    HIPCHK(c, hipMemcpy(cell_face, c->m_cface.p, sizeof(int) * (size_t)c->nref, hipMemcpyDeviceToHost));
   return FVM_OK;
 }
 

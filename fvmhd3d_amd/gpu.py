@@ -83,6 +83,8 @@ SIGNATURES = {
     # learned HLLD surrogate (include/fvm_surrogate.h)
     "fvm_sg_create": [ctypes.POINTER(ctypes.c_void_p), _I, _D],
     "fvm_sg_destroy": [_P],
+    "fvm_upload_faces": [_P, _I, _I, _I, _I, _P, _P],
+    "fvm_download_cell_faces": [_P, _P, _P],
     "fvm_substep_setup": [_P, _I, _I, _P, _P, _P, _P],
     "fvm_timestep_limiter_sweep": [_I, _P, _P, _P, _P],
     "fvm_substep_upload": [_P, _I, _P, _I],
@@ -192,6 +194,22 @@ class Context:
         self._chk(self.L.fvm_upload_mesh(self.h, np_, nactive, nimport_loc, len(faces), _ptr(faces),
                                          _ptr(cell_off), _ptr(cell_face), _ptr(cvol)))
         self.np = np_
+
+    def upload_faces(self, np_, nactive, nimport_loc, faces, cvol):
+        """Mesh upload with the cell lists built on the device (fvm_upload_faces)."""
+        faces = np.ascontiguousarray(faces)
+        cvol = np.ascontiguousarray(cvol, dtype=np.float64)
+        self._chk(self.L.fvm_upload_faces(self.h, np_, nactive, nimport_loc, len(faces), _ptr(faces), _ptr(cvol)))
+        self.np = np_
+        self.nactive = nactive
+
+    def cell_faces(self):
+        """The uploaded mesh's cell lists (offsets, face indices)."""
+        off = np.empty(self.nactive + 1, dtype=np.int32)
+        self._chk(self.L.fvm_download_cell_faces(self.h, _ptr(off), None))
+        cf = np.empty(max(int(off[-1]), 1), dtype=np.int32)
+        self._chk(self.L.fvm_download_cell_faces(self.h, None, _ptr(cf)))
+        return off, cf[:int(off[-1])]
 
     def upload(self, name, arr):
         fid, width, kind = FIELDS[name]

@@ -127,6 +127,16 @@ int fvm_upload_mesh(fvm_ctx *ctx, int np, int nactive, int nimport_loc, int nfac
                     const double *cell_volume);
 
 /* ---- per-cell state (the pointer tables Wrec_act, mesh_act, U_act, dU_act, ...) ---- */
+/* fvm_upload_mesh without the cell lists: the face records (converted to SoA
+ * on the device) and each active cell's face list built on the device the way
+ * localMesh.cpp:586-589 builds cell_list[i].ngb: every face is listed by its
+ * s1 and, when s2 is active, by its s2, in ascending face index.  Every face's
+ * s1 must be an active cell (the cell that created it). */
+int fvm_upload_faces(fvm_ctx *ctx, int np, int nactive, int nimport_loc, int nface, const fvm_face *faces,
+                     const double *cell_volume);
+/* the uploaded mesh's cell lists: nactive+1 offsets, then offsets[nactive]
+ * face indices (either pointer may be NULL) */
+int fvm_download_cell_faces(fvm_ctx *ctx, int *cell_face_off, int *cell_face);
 int fvm_upload_field(fvm_ctx *ctx, int field, const void *host, int np);
 int fvm_download_field(fvm_ctx *ctx, int field, void *host, int np);
 
