@@ -92,6 +92,10 @@ struct Phys {
 };
 
 #define FLD(p, k, i, n) (p)[(size_t)(k) * (size_t)(n) + (size_t)(i)]
+// per-face records of the flux and predictor sweeps, one record of `stride`
+// doubles per active face: the cell-side loops read a face's whole record from
+// one or two cache lines
+#define REC(p, k, a, stride) (p)[(size_t)(a) * (size_t)(stride) + (size_t)(k)]
 
 __device__ __forceinline__ int gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
@@ -340,6 +344,7 @@ __global__ void __launch_bounds__(256) k_slope_limiter(Cells<S> C, MeshView M, P
 // and swp[a] = 1 when i and j were swapped (s1 on a boundary).
 // ---------------------------------------------------------------------------
 // 14 entries per face record (NREC_P = 15 adds the predictor's dV)
+constexpr int NREC_F = NF + 4;
 
 // computeFluidUpdate.cpp:99-209: the inputs of computeFlux for active face a
 // (boundary swap, space-time extrapolation of both sides, face velocity)
@@ -423,16 +428,15 @@ __global__ void __launch_bounds__(256) k_face_flux(Cells<S> C, MeshView M, Phys 
                                    unsigned long long *diag) {
   const int a = gid();
   if (a >= M.nfa) return;
-  const int nfa = M.nfa;
   FaceIn q;
   face_inputs(C, M, P, a, diag, q);
   double psi, Bn, flux[NF];
   compute_flux<double>(P.gamma, q.wij, q.Wi, q.Wj, q.jbnd, q.normal, psi, Bn, flux);
 
   const double tarea = q.dt * q.area;
-  for (int k = 0; k < NF; k++) FLD(rec, k, a, nfa) = flux[k] * tarea;
-  for (int d = 0; d < 3; d++) FLD(rec, NF + d, a, nfa) = q.normal[d] * (psi * tarea);
-  FLD(rec, NF + 3, a, nfa) = Bn * tarea;
+  for (int k = 0; k < NF; k++) REC(rec, k, a, NREC_F) = flux[k] * tarea;
+  for (int d = 0; d < 3; d++) REC(rec, NF + d, a, NREC_F) = q.normal[d] * (psi * tarea);
+  REC(rec, NF + 3, a, NREC_F) = Bn * tarea;
   swp[a] = q.sw ? 1 : 0;
 }
 
@@ -461,7 +465,7 @@ template <typename S>
 __global__ void __launch_bounds__(256) k_scatter(Cells<S> C, MeshView M, const double *rec, const int *swp) {
   const int c = gid();
   if (c >= C.np) return;
-  const int np = C.np, nfa = M.nfa;
+  const int np = C.np;
   S U[NF], gp[3], dv;
   for (int k = 0; k < NF; k++) U[k] = (S)0.0;
   gp[0] = gp[1] = gp[2] = (S)0.0;
@@ -471,13 +475,13 @@ __global__ void __launch_bounds__(256) k_scatter(Cells<S> C, MeshView M, const d
     const int a = ent >> 1, side = ent & 1;
     const bool is_i = (side == 0) != (swp[a] != 0);
     if (is_i) {
-      for (int k = 0; k < NF; k++) U[k] = (S)((double)U[k] - FLD(rec, k, a, nfa));
-      for (int d = 0; d < 3; d++) gp[d] = gp[d] + (S)FLD(rec, NF + d, a, nfa);
-      dv = (S)((double)dv + FLD(rec, NF + 3, a, nfa));
+      for (int k = 0; k < NF; k++) U[k] = (S)((double)U[k] - REC(rec, k, a, NREC_F));
+      for (int d = 0; d < 3; d++) gp[d] = gp[d] + (S)REC(rec, NF + d, a, NREC_F);
+      dv = (S)((double)dv + REC(rec, NF + 3, a, NREC_F));
     } else {
-      for (int k = 0; k < NF; k++) U[k] = (S)((double)U[k] + FLD(rec, k, a, nfa));
-      for (int d = 0; d < 3; d++) gp[d] = gp[d] - (S)FLD(rec, NF + d, a, nfa);
-      dv = (S)((double)dv - FLD(rec, NF + 3, a, nfa));
+      for (int k = 0; k < NF; k++) U[k] = (S)((double)U[k] + REC(rec, k, a, NREC_F));
+      for (int d = 0; d < 3; d++) gp[d] = gp[d] - (S)REC(rec, NF + d, a, NREC_F);
+      dv = (S)((double)dv - REC(rec, NF + 3, a, NREC_F));
     }
   }
   FLD(C.ddivB, 0, c, np) = FLD(C.ddivB, 0, c, np) + dv;
@@ -668,7 +672,7 @@ __global__ void __launch_bounds__(256) k_predictor_face(Cells<S> C, MeshView M, 
                                         unsigned long long *diag) {
   const int a = gid();
   if (a >= M.nfa) return;
-  const int np = C.np, nfa = M.nfa;
+  const int np = C.np;
   const int f = M.fact[a];
   int i = M.s1[f], j = M.s2[f];
   const double fn[3] = {FLD(M.n, 0, f, M.nface), FLD(M.n, 1, f, M.nface), FLD(M.n, 2, f, M.nface)};
@@ -709,10 +713,10 @@ __global__ void __launch_bounds__(256) k_predictor_face(Cells<S> C, MeshView M, 
                          vij[2] * 0.5 - rij[2] * fdv * r};
   double psi, Bn, flux[NF];
   compute_flux<double>(P.gamma, wij, Wi, Wj, jbnd, normal, psi, Bn, flux);
-  for (int k = 0; k < NF; k++) FLD(rec, k, a, nfa) = flux[k] * area;
-  for (int d = 0; d < 3; d++) FLD(rec, NF + d, a, nfa) = normal[d] * (psi * area);
-  FLD(rec, NF + 3, a, nfa) = Bn * area;
-  FLD(rec, NF + 4, a, nfa) = area * (rij[0] * wij[0] + rij[1] * wij[1] + rij[2] * wij[2]) / dsl;
+  for (int k = 0; k < NF; k++) REC(rec, k, a, NREC_P) = flux[k] * area;
+  for (int d = 0; d < 3; d++) REC(rec, NF + d, a, NREC_P) = normal[d] * (psi * area);
+  REC(rec, NF + 3, a, NREC_P) = Bn * area;
+  REC(rec, NF + 4, a, NREC_P) = area * (rij[0] * wij[0] + rij[1] * wij[1] + rij[2] * wij[2]) / dsl;
   swp[a] = sw ? 1 : 0;
 }
 
@@ -733,7 +737,7 @@ __global__ void __launch_bounds__(256) k_predictor_cell(Cells<S> C, MeshView M, 
                                         unsigned long long *diag) {
   const int c = gid();
   if (c >= C.nactive) return;
-  const int np = C.np, nfa = M.nfa, nactive = C.nactive;
+  const int np = C.np, nactive = C.nactive;
   S dU[NF], gp[3], dv;
   double dV = 0.0;
   for (int k = 0; k < NF; k++) dU[k] = (S)0.0;
@@ -746,19 +750,19 @@ __global__ void __launch_bounds__(256) k_predictor_cell(Cells<S> C, MeshView M, 
     const bool sw = swp[a] != 0;
     const bool is_i = (side == 0) != sw;
     if (is_i) {
-      for (int k = 0; k < NF; k++) dU[k] = (S)((double)dU[k] - FLD(rec, k, a, nfa));
-      for (int d = 0; d < 3; d++) gp[d] = gp[d] + (S)FLD(rec, NF + d, a, nfa);
-      dv = (S)((double)dv + FLD(rec, NF + 3, a, nfa));
-      dV += FLD(rec, NF + 4, a, nfa);
+      for (int k = 0; k < NF; k++) dU[k] = (S)((double)dU[k] - REC(rec, k, a, NREC_P));
+      for (int d = 0; d < 3; d++) gp[d] = gp[d] + (S)REC(rec, NF + d, a, NREC_P);
+      dv = (S)((double)dv + REC(rec, NF + 3, a, NREC_P));
+      dV += REC(rec, NF + 4, a, NREC_P);
     } else {
       // j receives only when active and not a boundary (computePredictor.cpp:127-128)
       const int jeff = sw ? M.s1[f] : M.s2[f];
       (void)jeff;
       if (c >= nactive || C.bnd[c] != NO_BOUNDARY) continue;
-      dV -= FLD(rec, NF + 4, a, nfa);
-      for (int k = 0; k < NF; k++) dU[k] = (S)((double)dU[k] + FLD(rec, k, a, nfa));
-      for (int d = 0; d < 3; d++) gp[d] = gp[d] - (S)FLD(rec, NF + d, a, nfa);
-      dv = (S)((double)dv - FLD(rec, NF + 3, a, nfa));
+      dV -= REC(rec, NF + 4, a, NREC_P);
+      for (int k = 0; k < NF; k++) dU[k] = (S)((double)dU[k] + REC(rec, k, a, NREC_P));
+      for (int d = 0; d < 3; d++) gp[d] = gp[d] - (S)REC(rec, NF + d, a, NREC_P);
+      dv = (S)((double)dv - REC(rec, NF + 3, a, NREC_P));
     }
   }
   if (C.bnd[c] != NO_BOUNDARY) return;
