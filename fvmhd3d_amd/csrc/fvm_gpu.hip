@@ -79,6 +79,7 @@ struct MeshView {
   int nface, nfa;
   const int *s1, *s2;
   const double *c, *n;  // [3][nface]
+  const double *cn;     // [nface][6]: centroid, n (the same values per face)
   const int *coff, *cface;
   const double *cvol;
   const int *fact;        // face_active_list
@@ -96,6 +97,9 @@ struct Phys {
 // doubles per active face: the cell-side loops read a face's whole record from
 // one or two cache lines
 #define REC(p, k, a, stride) (p)[(size_t)(a) * (size_t)(stride) + (size_t)(k)]
+// per-face geometry record: centroid[3] then area vector n[3]
+#define GEO_C(M, d, f) (M).cn[(size_t)(f) * 6 + (size_t)(d)]
+#define GEO_N(M, d, f) (M).cn[(size_t)(f) * 6 + 3 + (size_t)(d)]
 
 __device__ __forceinline__ int gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
@@ -115,7 +119,7 @@ __global__ void k_soa_to_aos(const S *__restrict__ src, double *__restrict__ dst
   for (int k = 0; k < width; k++) dst[(size_t)i * width + k] = (double)FLD(src, k, i, n);
 }
 __global__ void k_faces_to_soa(const fvm_face *__restrict__ f, int n, int *s1, int *s2, double *c,
-                               double *nn) {
+                               double *nn, double *cn) {
   const int i = gid();
   if (i >= n) return;
   const fvm_face F = f[i];
@@ -124,6 +128,8 @@ __global__ void k_faces_to_soa(const fvm_face *__restrict__ f, int n, int *s1, i
   for (int d = 0; d < 3; d++) {
     FLD(c, d, i, n) = F.centroid[d];
     FLD(nn, d, i, n) = F.n[d];
+    cn[(size_t)i * 6 + d] = F.centroid[d];
+    cn[(size_t)i * 6 + 3 + d] = F.n[d];
   }
 }
 
@@ -162,7 +168,7 @@ __global__ void __launch_bounds__(256) k_reconstruction(Cells<S> C, MeshView M, 
 
   auto face_dri = [&](int f, double *dri) {
     for (int d = 0; d < 3; d++) {
-      double v = FLD(M.c, d, f, M.nface) - ipos[d];
+      double v = GEO_C(M, d, f) - ipos[d];
       if (v > 0.5 * P.L[d]) v -= P.L[d];
       else if (v < -0.5 * P.L[d]) v += P.L[d];
       dri[d] = v;
@@ -173,7 +179,7 @@ __global__ void __launch_bounds__(256) k_reconstruction(Cells<S> C, MeshView M, 
     const int f = M.cface[e];
     double cen[3];
     face_dri(f, cen);
-    const double fn[3] = {FLD(M.n, 0, f, M.nface), FLD(M.n, 1, f, M.nface), FLD(M.n, 2, f, M.nface)};
+    const double fn[3] = {GEO_N(M, 0, f), GEO_N(M, 1, f), GEO_N(M, 2, f)};
     const double area = dsqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
     const double cn = cen[0] * fn[0] + cen[1] * fn[1] + cen[2] * fn[2];
     const double sc = (cn < 0.0) ? (-1.0 / area) : (1.0 / area);
@@ -264,12 +270,12 @@ struct FaceGeo {
 __device__ __forceinline__ void face_geo(const MeshView &M, const double *pos_i, int f, int np,
                                          FaceGeo &g) {
   (void)np;
-  const double fn[3] = {FLD(M.n, 0, f, M.nface), FLD(M.n, 1, f, M.nface), FLD(M.n, 2, f, M.nface)};
+  const double fn[3] = {GEO_N(M, 0, f), GEO_N(M, 1, f), GEO_N(M, 2, f)};
   g.area = dsqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
   const double ia = 1.0 / g.area;
   for (int d = 0; d < 3; d++) {
     g.normal[d] = fn[d] * ia;
-    g.dri[d] = FLD(M.c, d, f, M.nface) - pos_i[d];
+    g.dri[d] = GEO_C(M, d, f) - pos_i[d];
   }
   g.dsh = g.dri[0] * g.normal[0] + g.dri[1] * g.normal[1] + g.dri[2] * g.normal[2];
   g.dsl = 2.0 * g.dsh;
@@ -360,12 +366,12 @@ __device__ __forceinline__ void face_inputs(const Cells<S> &C, const MeshView &M
   const int np = C.np;
   const int f = M.fact[a];
   int i = M.s1[f], j = M.s2[f];
-  const double fn[3] = {FLD(M.n, 0, f, M.nface), FLD(M.n, 1, f, M.nface), FLD(M.n, 2, f, M.nface)};
+  const double fn[3] = {GEO_N(M, 0, f), GEO_N(M, 1, f), GEO_N(M, 2, f)};
   const double area = dsqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
   const double ia = 1.0 / area;
   double normal[3] = {fn[0] * ia, fn[1] * ia, fn[2] * ia};
   double dri[3];
-  for (int d = 0; d < 3; d++) dri[d] = FLD(M.c, d, f, M.nface) - FLD(C.pos, d, i, np);
+  for (int d = 0; d < 3; d++) dri[d] = GEO_C(M, d, f) - FLD(C.pos, d, i, np);
   const bool sw = C.bnd[i] != NO_BOUNDARY;
   if (sw) {
     const int tmp = i;
@@ -675,12 +681,12 @@ __global__ void __launch_bounds__(256) k_predictor_face(Cells<S> C, MeshView M, 
   const int np = C.np;
   const int f = M.fact[a];
   int i = M.s1[f], j = M.s2[f];
-  const double fn[3] = {FLD(M.n, 0, f, M.nface), FLD(M.n, 1, f, M.nface), FLD(M.n, 2, f, M.nface)};
+  const double fn[3] = {GEO_N(M, 0, f), GEO_N(M, 1, f), GEO_N(M, 2, f)};
   const double area = dsqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
   const double ia = 1.0 / area;
   double normal[3] = {fn[0] * ia, fn[1] * ia, fn[2] * ia};
   double dri[3];
-  for (int d = 0; d < 3; d++) dri[d] = FLD(M.c, d, f, M.nface) - FLD(C.pos, d, i, np);
+  for (int d = 0; d < 3; d++) dri[d] = GEO_C(M, d, f) - FLD(C.pos, d, i, np);
   const bool sw = C.bnd[i] != NO_BOUNDARY;
   if (sw) {
     const int tmp = i;
@@ -996,6 +1002,7 @@ static fvm::MeshView meshview(fvm_ctx *c) {
   M.s2 = (const int *)c->m_s2.p;
   M.c = (const double *)c->m_c.p;
   M.n = (const double *)c->m_n.p;
+  M.cn = (const double *)c->m_cn.p;
   M.coff = (const int *)c->m_coff.p;
   M.cface = (const int *)c->m_cface.p;
   M.cvol = (const double *)c->m_cvol.p;
@@ -1077,7 +1084,7 @@ int fvm_destroy(fvm_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &b : c->field) release(b);
-  for (DBuf *b : {&c->m_s1, &c->m_s2, &c->m_c, &c->m_n, &c->m_coff, &c->m_cface, &c->m_cvol, &c->m_fact,
+  for (DBuf *b : {&c->m_s1, &c->m_s2, &c->m_c, &c->m_n, &c->m_cn, &c->m_coff, &c->m_cface, &c->m_cvol, &c->m_fact,
                   &c->m_soff, &c->m_sent, &c->rec, &c->swp, &c->tmp0, &c->tmp1, &c->tmp2, &c->stage,
                   &c->diag, &c->scalars})
     release(*b);
@@ -1149,13 +1156,14 @@ int fvm_upload_mesh(fvm_ctx *c, int np, int nactive, int nimport_loc, int nface,
   int rc;
   if ((rc = ensure(c, c->m_s1, sizeof(int) * (size_t)nface)) || (rc = ensure(c, c->m_s2, sizeof(int) * (size_t)nface)) ||
       (rc = ensure(c, c->m_c, sizeof(double) * 3 * (size_t)nface)) || (rc = ensure(c, c->m_n, sizeof(double) * 3 * (size_t)nface)) ||
+      (rc = ensure(c, c->m_cn, sizeof(double) * 6 * (size_t)nface + 8)) ||
       (rc = ensure(c, c->m_coff, sizeof(int) * (size_t)(nactive + 1))) || (rc = ensure(c, c->m_cface, sizeof(int) * (size_t)nref)) ||
       (rc = ensure(c, c->m_cvol, sizeof(double) * (size_t)nactive)) || (rc = ensure(c, c->stage, sizeof(fvm_face) * (size_t)nface)))
     return rc;
   if (nface > 0) {
     HIPCHK(c, hipMemcpyAsync(c->stage.p, faces, sizeof(fvm_face) * (size_t)nface, hipMemcpyHostToDevice, c->stream));
     LAUNCH(c, fvm::k_faces_to_soa, nface, (const fvm_face *)c->stage.p, nface, (int *)c->m_s1.p, (int *)c->m_s2.p,
-           (double *)c->m_c.p, (double *)c->m_n.p);
+           (double *)c->m_c.p, (double *)c->m_n.p, (double *)c->m_cn.p);
   }
   HIPCHK(c, hipMemcpyAsync(c->m_coff.p, cell_face_off, sizeof(int) * (size_t)(nactive + 1), hipMemcpyHostToDevice, c->stream));
   if (nref > 0) HIPCHK(c, hipMemcpyAsync(c->m_cface.p, cell_face, sizeof(int) * (size_t)nref, hipMemcpyHostToDevice, c->stream));
@@ -1188,6 +1196,7 @@ int fvm_upload_faces(fvm_ctx *c, int np, int nactive, int nimport_loc, int nface
   int rc;
   if ((rc = ensure(c, c->m_s1, sizeof(int) * (size_t)nface)) || (rc = ensure(c, c->m_s2, sizeof(int) * (size_t)nface)) ||
       (rc = ensure(c, c->m_c, sizeof(double) * 3 * (size_t)nface)) || (rc = ensure(c, c->m_n, sizeof(double) * 3 * (size_t)nface)) ||
+      (rc = ensure(c, c->m_cn, sizeof(double) * 6 * (size_t)nface + 8)) ||
       (rc = ensure(c, c->m_coff, sizeof(int) * (size_t)(nactive + 1))) || (rc = ensure(c, c->m_cface, sizeof(int) * 2 * (size_t)nface + 4)) ||
       (rc = ensure(c, c->m_cvol, sizeof(double) * (size_t)nactive + 8)) || (rc = ensure(c, c->stage, sizeof(fvm_face) * (size_t)nface + 8)) ||
       (rc = ensure(c, c->tmp0, sizeof(int) * (size_t)(nactive + 1))) || (rc = ensure(c, c->tmp1, sizeof(int) * (size_t)(nactive + 1))))
@@ -1197,7 +1206,7 @@ int fvm_upload_faces(fvm_ctx *c, int np, int nactive, int nimport_loc, int nface
   if (nface > 0) {
     HIPCHK(c, hipMemcpyAsync(c->stage.p, faces, sizeof(fvm_face) * (size_t)nface, hipMemcpyHostToDevice, c->stream));
     LAUNCH(c, fvm::k_faces_to_soa, nface, (const fvm_face *)c->stage.p, nface, (int *)c->m_s1.p, (int *)c->m_s2.p,
-           (double *)c->m_c.p, (double *)c->m_n.p);
+           (double *)c->m_c.p, (double *)c->m_n.p, (double *)c->m_cn.p);
     LAUNCH(c, fvm::k_csr_count, nface, (const int *)c->m_s1.p, (const int *)c->m_s2.p, nface, nactive, cnt);
   }
   if (fvm::exclusive_scan(c->scan, cnt, (int *)c->m_coff.p, nactive + 1, c->stream))

@@ -85,6 +85,7 @@ struct fvm_ctx {
   bool have_mesh = false;
   DBuf field[FVM_NFIELDS];
   DBuf m_s1, m_s2, m_c, m_n, m_coff, m_cface, m_cvol, m_fact, m_soff, m_sent;
+  DBuf m_cn;  // per-face geometry record (centroid, n) of the same faces as m_c / m_n
   DBuf rec, swp, tmp0, tmp1, tmp2, stage;
   DBuf diag;     // unsigned long long[FVM_NDIAG]
   DBuf scalars;  // unsigned long long[8]

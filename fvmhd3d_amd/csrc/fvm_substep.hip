@@ -133,7 +133,7 @@ __global__ void k_sub_local_faces(const int *__restrict__ fl, int nfl, const int
                               const double *__restrict__ gn, int nface, const double *__restrict__ pos, int n,
                               const int *__restrict__ act, const int *__restrict__ g2l, double bx, double by,
                               double bz, int *__restrict__ s1, int *__restrict__ s2, double *__restrict__ c,
-                              double *__restrict__ nn) {
+                              double *__restrict__ nn, double *__restrict__ cn) {
   const int j = gidx();
   if (j >= nfl) return;
   const int f = fl[j];
@@ -153,6 +153,8 @@ __global__ void k_sub_local_faces(const int *__restrict__ fl, int nfl, const int
     }
     SFLD(c, d, j, nfl) = cd;
     SFLD(nn, d, j, nfl) = nd;
+    cn[(size_t)j * 6 + d] = cd;
+    cn[(size_t)j * 6 + 3 + d] = nd;
   }
 }
 
@@ -452,7 +454,7 @@ int build_layout(fvm_ctx *c, unsigned mask, double t_global, int check, int *cou
   SLAUNCH(c, nfl, k_sub_local_faces, P<int>(s.fl), nfl, P<int>(s.gs1), P<int>(s.gs2), P<double>(s.gc), P<double>(s.gn),
           nface, P<double>(s.g[FVM_FIELD_POS]), n, P<int>(s.act), P<int>(s.g2l), c->prm.domain_size[0],
           c->prm.domain_size[1], c->prm.domain_size[2], P<int>(c->m_s1), P<int>(c->m_s2), P<double>(c->m_c),
-          P<double>(c->m_n));
+          P<double>(c->m_n), P<double>(c->m_cn));
   SLAUNCH(c, nA, k_sub_face_counts, P<int>(s.perm), nA, P<int>(s.goff), P<int>(s.len));
   if (nA > 0) {
     if (fvm::exclusive_scan(c->scan, P<int>(s.len), P<int>(c->m_coff), nA, st))
@@ -675,7 +677,7 @@ int fvm_substep_setup(fvm_ctx *c, int ncell, int nface, const fvm_face *faces, c
   // the context's mesh, fields and the path's scratch at their largest (every
   // cell local), so that no substep grows a buffer
   const size_t R = (size_t)nref;
-  if ((rc = ensure_all(c, {{&c->m_s1, 4 * F}, {&c->m_s2, 4 * F}, {&c->m_c, 24 * F}, {&c->m_n, 24 * F},
+  if ((rc = ensure_all(c, {{&c->m_s1, 4 * F}, {&c->m_s2, 4 * F}, {&c->m_c, 24 * F}, {&c->m_n, 24 * F}, {&c->m_cn, 48 * F},
                            {&c->m_coff, 4 * (N + 1)}, {&c->m_cface, 4 * R}, {&c->m_cvol, 8 * N},
                            {&c->m_soff, 4 * (N + 1)}, {&c->m_sent, 8 * F + 4}, {&c->m_fact, 4 * (F + 1)},
                            {&c->tmp0, 4 * (std::max(N, R) + 1)}, {&c->tmp1, 4 * (std::max(F, R) + 1)},
