@@ -96,7 +96,67 @@ def cpu_baseline(st, lim, budget_s=30.0):
                       f"oracle/fvm_oracle.c FP64, {dt:.1f} s"}
 
 
-def substep_bench(nsub, n=2_097_152, dtmax=1.0 / 128, host_driver=True):
+def substep_cpu_baseline(m, W0, dtmax, nsub, budget_s=15.0):
+    """The oracle (CPU restatement, one core) behind the host-layout driver, on the
+    same disk and substep cycle from its start, for about budget_s seconds."""
+    from oracle.oracle import Oracle
+    from fvmhd3d_amd import gpu as G
+    from fvmhd3d_amd.substep import Substepper
+
+    class OracleCalls:
+        def __init__(self):
+            self.o = Oracle("f64", limiter=G.LIMITER_SMOOTH)
+
+        def load(self, st):
+            pass
+
+        def put(self, st, names):
+            pass
+
+        def fetch(self, st, names):
+            pass
+
+        def get_active_faces(self, st):
+            return len(self.o.get_active_faces(st))
+
+        def compute_primitives(self, st):
+            self.o.compute_primitives(st, False)
+
+        def compute_timestep(self, st):
+            self.o.compute_timestep(st)
+
+        def compute_timestep_limiter(self, st):
+            self.o.compute_timestep_limiter(st)
+
+        def compute_predictor(self, st, dtmax, min_rung):
+            self.o.compute_predictor(st, dtmax, min_rung)
+
+        def compute_reconstruction(self, st):
+            self.o.compute_reconstruction(st)
+
+        def slope_limiter(self, st):
+            self.o.slope_limiter(st)
+
+        def compute_fluid_update(self, st):
+            self.o.compute_fluid_update(st)
+
+    S = Substepper(m, W0, dtmax=dtmax, limiter=G.LIMITER_SMOOTH)
+    B = OracleCalls()
+    S.init_rungs(B)
+    t0 = time.perf_counter()
+    k = 0
+    while k < nsub and time.perf_counter() - t0 < budget_s:
+        S.substep(B)
+        k += 1
+        log(f"[substeps] oracle {k}/{nsub}: {S.history[-1][2]} active faces, {time.perf_counter() - t0:.1f} s")
+    wall = time.perf_counter() - t0
+    faces = sum(h[2] for h in S.history)
+    return {"value": faces / wall, "unit": "face-flux updates/s", "cores": 1, "kind": "port",
+            "sample": f"the first {k} of the {nsub} substeps ({faces} active faces), host-layout driver over "
+                      f"oracle/fvm_oracle.c FP64, {wall:.1f} s"}
+
+
+def substep_bench(nsub, n=2_097_152, dtmax=1.0 / 128, host_driver=True, cpu=True):
     """Individual timesteps on the disk (rho ~ exp(-z^2/2H^2), Keplerian v_phi, B_z at
     beta = 400, 1 % velocity noise; 6 rungs).  The measured driver keeps the per-cell
     state on the device (fvm_substep_run: layout, gathers, rungs and write-back on the
@@ -145,6 +205,8 @@ def substep_bench(nsub, n=2_097_152, dtmax=1.0 / 128, host_driver=True):
         out["host_driver"] = {"value": sum(faces) / hwall, "wall_s": round(hwall, 3),
                               "library_call_s": round(be.kernel_s, 3), "host_s": round(hwall - be.kernel_s, 3),
                               "bit_identical_to_device_driver": bool(same)}
+    if cpu:
+        out["cpu_baseline"] = substep_cpu_baseline(m, W0, dtmax, nsub)
     return out
 
 
@@ -207,7 +269,8 @@ def main():
         print(json.dumps(surrogate_bench(args.steps)), flush=True)
         return
     if args.substeps:
-        print(json.dumps(substep_bench(args.substeps, host_driver=not args.no_host_driver)), flush=True)
+        print(json.dumps(substep_bench(args.substeps, host_driver=not args.no_host_driver,
+                                       cpu=not args.no_cpu_baseline)), flush=True)
         return
     if args.copy_only:
         c = G.Context()
