@@ -102,6 +102,15 @@ struct Phys {
 #define GEO_N(M, d, f) (M).cn[(size_t)(f) * 6 + 3 + (size_t)(d)]
 
 __device__ __forceinline__ int gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+// XCD-aware index of the cell and face sweeps: block b runs on XCD b mod 8
+// and the 8 XCDs each take a contiguous eighth of the range, so neighbour
+// gathers of nearby (Morton-ordered) cells share one XCD's L2.  gridDim.x is
+// a multiple of 8 (LAUNCH_X); every thread still computes exactly one item.
+__device__ __forceinline__ int xgid() {
+  const int per = (int)gridDim.x >> 3;
+  const int tile = ((int)blockIdx.x & 7) * per + ((int)blockIdx.x >> 3);
+  return tile * (int)blockDim.x + (int)threadIdx.x;
+}
 
 // ---------------------------------------------------------------------------
 // layout conversion
@@ -139,7 +148,7 @@ __global__ void k_faces_to_soa(const fvm_face *__restrict__ f, int n, int *s1, i
 // ---------------------------------------------------------------------------
 template <typename S>
 __global__ void __launch_bounds__(256) k_reconstruction(Cells<S> C, MeshView M, Phys P) {
-  const int i = gid();
+  const int i = xgid();
   if (i >= C.nactive) return;
   const int np = C.np;
   double Wi[NF];
@@ -284,7 +293,7 @@ __device__ __forceinline__ void face_geo(const MeshView &M, const double *pos_i,
 
 template <typename S>
 __global__ void __launch_bounds__(256) k_slope_limiter(Cells<S> C, MeshView M, Phys P, unsigned long long *diag) {
-  const int c = gid();
+  const int c = xgid();
   if (c >= C.np) return;
   const int np = C.np;
   S lim[NF];
@@ -432,7 +441,7 @@ __device__ __forceinline__ void face_inputs(const Cells<S> &C, const MeshView &M
 template <typename S>
 __global__ void __launch_bounds__(256) k_face_flux(Cells<S> C, MeshView M, Phys P, double *rec, int *swp,
                                    unsigned long long *diag) {
-  const int a = gid();
+  const int a = xgid();
   if (a >= M.nfa) return;
   FaceIn q;
   face_inputs(C, M, P, a, diag, q);
@@ -469,7 +478,7 @@ __global__ void __launch_bounds__(256) k_face_states(Cells<S> C, MeshView M, Phy
 // computeFluidUpdate.cpp:225-245: per cell, in face_active_list order
 template <typename S>
 __global__ void __launch_bounds__(256) k_scatter(Cells<S> C, MeshView M, const double *rec, const int *swp) {
-  const int c = gid();
+  const int c = xgid();
   if (c >= C.np) return;
   const int np = C.np;
   S U[NF], gp[3], dv;
@@ -510,7 +519,7 @@ __device__ __forceinline__ double entropy_from_ethm(double g, const double *W) {
 template <typename S>
 __global__ void __launch_bounds__(256) k_primitives(Cells<S> C, MeshView M, Phys P, int predictor_flag,
                                     unsigned long long *diag) {
-  const int i = gid();
+  const int i = xgid();
   if (i >= C.nactive) return;
   const int np = C.np;
   const double g = P.gamma;
@@ -620,7 +629,7 @@ __global__ void __launch_bounds__(256) k_primitives(Cells<S> C, MeshView M, Phys
 // ---------------------------------------------------------------------------
 template <typename S>
 __global__ void __launch_bounds__(256) k_timestep(Cells<S> C, MeshView M, Phys P, unsigned long long *dtmin_bits) {
-  const int i = gid();
+  const int i = xgid();
   double dti_out = REF_HUGE;
   if (i < C.nactive) {
     const int np = C.np;
@@ -676,7 +685,7 @@ constexpr int NREC_P = 15;
 template <typename S>
 __global__ void __launch_bounds__(256) k_predictor_face(Cells<S> C, MeshView M, Phys P, double *rec, int *swp,
                                         unsigned long long *diag) {
-  const int a = gid();
+  const int a = xgid();
   if (a >= M.nfa) return;
   const int np = C.np;
   const int f = M.fact[a];
@@ -741,7 +750,7 @@ template <typename S>
 __global__ void __launch_bounds__(256) k_predictor_cell(Cells<S> C, MeshView M, Phys P, const double *rec,
                                         const int *swp, double dtmax, int min_rung,
                                         unsigned long long *diag) {
-  const int c = gid();
+  const int c = xgid();
   if (c >= C.nactive) return;
   const int np = C.np, nactive = C.nactive;
   S dU[NF], gp[3], dv;
@@ -1028,6 +1037,14 @@ static inline unsigned nblk(long n, int bs = 256) { return (unsigned)((n + bs - 
   do {                                                                                 \
     if ((n) > 0) {                                                                     \
       hipLaunchKernelGGL(kern, dim3(nblk(n)), dim3(256), 0, (c)->stream, __VA_ARGS__); \
+      HIPCHK(c, hipGetLastError());                                                    \
+    }                                                                                  \
+  } while (0)
+// the same with the grid padded to a multiple of 8 blocks (kernels indexed by xgid)
+#define LAUNCH_X(c, kern, n, ...)                                                        \
+  do {                                                                                 \
+    if ((n) > 0) {                                                                     \
+      hipLaunchKernelGGL(kern, dim3((nblk(n) + 7u) & ~7u), dim3(256), 0, (c)->stream, __VA_ARGS__); \
       HIPCHK(c, hipGetLastError());                                                    \
     }                                                                                  \
   } while (0)
@@ -1369,7 +1386,7 @@ int fvm_download_active_faces(fvm_ctx *c, int *face_active, int n) {
   ((c)->prm.storage == FVM_STORAGE_SHIPPED ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
 
 template <typename S> static int do_reconstruction(fvm_ctx *c) {
-  LAUNCH(c, fvm::k_reconstruction<S>, c->nactive, cells<S>(c), meshview(c), phys(c));
+  LAUNCH_X(c, fvm::k_reconstruction<S>, c->nactive, cells<S>(c), meshview(c), phys(c));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return FVM_OK;
 }
@@ -1381,7 +1398,7 @@ int fvm_compute_reconstruction(fvm_ctx *c) {
 }
 
 template <typename S> static int do_limiter(fvm_ctx *c) {
-  LAUNCH(c, fvm::k_slope_limiter<S>, c->np, cells<S>(c), meshview(c), phys(c), (unsigned long long *)c->diag.p);
+  LAUNCH_X(c, fvm::k_slope_limiter<S>, c->np, cells<S>(c), meshview(c), phys(c), (unsigned long long *)c->diag.p);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return FVM_OK;
 }
@@ -1405,9 +1422,9 @@ template <typename S> static int do_fluid_update(fvm_ctx *c) {
     return rc;
   unsigned long long before[FVM_NDIAG], after[FVM_NDIAG];
   if ((rc = read_diag(c, before))) return rc;
-  LAUNCH(c, fvm::k_face_flux<S>, nfa, cells<S>(c), meshview(c), phys(c), (double *)c->rec.p, (int *)c->swp.p,
+  LAUNCH_X(c, fvm::k_face_flux<S>, nfa, cells<S>(c), meshview(c), phys(c), (double *)c->rec.p, (int *)c->swp.p,
          (unsigned long long *)c->diag.p);
-  LAUNCH(c, fvm::k_scatter<S>, c->np, cells<S>(c), meshview(c), (const double *)c->rec.p, (const int *)c->swp.p);
+  LAUNCH_X(c, fvm::k_scatter<S>, c->np, cells<S>(c), meshview(c), (const double *)c->rec.p, (const int *)c->swp.p);
   if ((rc = read_diag(c, after))) return rc;
   if (after[FVM_DIAG_POSITIVITY] != before[FVM_DIAG_POSITIVITY])
     return fail(c, FVM_EPOS, "computeFluidUpdate: non-positive density or thermal energy at a face");
@@ -1451,7 +1468,7 @@ int fvm_get_face_states(fvm_ctx *c, int first, int count, double *wij, double *W
 }
 
 template <typename S> static int do_primitives(fvm_ctx *c, int flag) {
-  LAUNCH(c, fvm::k_primitives<S>, c->nactive, cells<S>(c), meshview(c), phys(c), flag, (unsigned long long *)c->diag.p);
+  LAUNCH_X(c, fvm::k_primitives<S>, c->nactive, cells<S>(c), meshview(c), phys(c), flag, (unsigned long long *)c->diag.p);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return FVM_OK;
 }
@@ -1466,7 +1483,7 @@ template <typename S> static int do_timestep(fvm_ctx *c, double *dt_min) {
   unsigned long long *bits = (unsigned long long *)c->scalars.p;
   const unsigned long long init = 0x7FF0000000000000ULL;  // +inf
   HIPCHK(c, hipMemcpyAsync(bits, &init, sizeof init, hipMemcpyHostToDevice, c->stream));
-  LAUNCH(c, fvm::k_timestep<S>, c->nactive, cells<S>(c), meshview(c), phys(c), bits);
+  LAUNCH_X(c, fvm::k_timestep<S>, c->nactive, cells<S>(c), meshview(c), phys(c), bits);
   unsigned long long h = 0;
   HIPCHK(c, hipMemcpyAsync(&h, bits, sizeof h, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1525,9 +1542,9 @@ template <typename S> static int do_predictor(fvm_ctx *c, double dtmax, int min_
   int rc;
   if ((rc = ensure(c, c->rec, sizeof(double) * fvm::NREC_P * (size_t)nfa)) || (rc = ensure(c, c->swp, sizeof(int) * (size_t)nfa)))
     return rc;
-  LAUNCH(c, fvm::k_predictor_face<S>, nfa, cells<S>(c), meshview(c), phys(c), (double *)c->rec.p, (int *)c->swp.p,
+  LAUNCH_X(c, fvm::k_predictor_face<S>, nfa, cells<S>(c), meshview(c), phys(c), (double *)c->rec.p, (int *)c->swp.p,
          (unsigned long long *)c->diag.p);
-  LAUNCH(c, fvm::k_predictor_cell<S>, c->nactive, cells<S>(c), meshview(c), phys(c), (const double *)c->rec.p,
+  LAUNCH_X(c, fvm::k_predictor_cell<S>, c->nactive, cells<S>(c), meshview(c), phys(c), (const double *)c->rec.p,
          (const int *)c->swp.p, dtmax, min_rung, (unsigned long long *)c->diag.p);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return FVM_OK;
