@@ -267,3 +267,19 @@ def test_device_substeps_conserve_mass_minmod():
     assert abs(np.sum(U[:, 0] + dU[:, 0]) - M0) < 1e-12 * M0
     assert np.array_equal(D.state("W").view(np.uint64), Sh.W.view(np.uint64))
     assert len({h[1] for h in D.history}) > 2  # several distinct active sets
+
+
+def test_rung_clock_advance_matches_pull():
+    """advance(max_rung) (the device driver's form: a bitmask of active rungs)
+    against pull(rung) (the host driver's per-cell mask), over a sequence of
+    substeps with changing rung sets."""
+    rng = np.random.default_rng(5)
+    a, b = RungClock(1.0 / 64), RungClock(1.0 / 64)
+    rung = rng.integers(0, 5, 400)
+    for _ in range(40):
+        mask, dt_a = a.advance(int(rung.max()))
+        act, dt_b = b.pull(rung)
+        assert dt_a == dt_b and a.tsysU == b.tsysU and a.min_rung == b.min_rung
+        assert np.array_equal(((mask >> rung) & 1).astype(bool), act)
+        # active cells may move to any rung at or above min_rung
+        rung[act] = rng.integers(a.min_rung, 6, int(act.sum()))
