@@ -1155,6 +1155,7 @@ int fvm_upload_mesh(fvm_ctx *c, int np, int nactive, int nimport_loc, int nface,
       if (F.s1 != i && F.s2 != i) return fail(c, FVM_EARG, "fvm_upload_mesh: cell lists a face it does not own");
     }
   (void)hipSetDevice(c->device);
+  c->have_mesh = false;  // until the new mesh is complete
   const bool resize = np != c->np;
   c->np = np;
   c->nactive = nactive;
@@ -1201,6 +1202,7 @@ int fvm_upload_faces(fvm_ctx *c, int np, int nactive, int nimport_loc, int nface
     if (faces[f].s1 < 0 || faces[f].s1 >= nactive || faces[f].s2 < 0 || faces[f].s2 >= np || faces[f].s1 == faces[f].s2)
       return fail(c, FVM_EARG, "fvm_upload_faces: face cell index out of range (s1 must be active)");
   (void)hipSetDevice(c->device);
+  c->have_mesh = false;  // until the new mesh is complete
   const bool resize = np != c->np;
   for (int f = 0; f < FVM_NFIELDS; f++) {
     const size_t bytes = field_bytes(c, f, np);

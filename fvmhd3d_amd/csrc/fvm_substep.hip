@@ -450,7 +450,9 @@ int build_layout(fvm_ctx *c, unsigned mask, double t_global, int check, int *cou
   SLAUNCH(c, n, k_sub_compact, P<int>(s.iflag), P<int>(s.ipos), n, nA, P<int>(s.perm));
   SLAUNCH(c, n, k_sub_g2l, P<int>(s.act), P<int>(s.apos), P<int>(s.iflag), P<int>(s.ipos), n, nA, P<int>(s.g2l));
   const int npl = nA + nI;
-  // the context's mesh (capacity for the whole mesh was reserved at setup)
+  // the context's mesh (capacity for the whole mesh was reserved at setup);
+  // not valid until the layout is complete
+  c->have_mesh = false;
   SLAUNCH(c, nfl, k_sub_local_faces, P<int>(s.fl), nfl, P<int>(s.gs1), P<int>(s.gs2), P<double>(s.gc), P<double>(s.gn),
           nface, P<double>(s.g[FVM_FIELD_POS]), n, P<int>(s.act), P<int>(s.g2l), c->prm.domain_size[0],
           c->prm.domain_size[1], c->prm.domain_size[2], P<int>(c->m_s1), P<int>(c->m_s2), P<double>(c->m_c),
