@@ -260,8 +260,8 @@ def main():
                     help="with --substeps: skip the host-layout driver run (comparison and bit-identity check)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md, not run", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); "
+              "sharding the path over several GPUs is a design, not an implementation", file=sys.stderr)
         sys.exit(2)
 
     from fvmhd3d_amd import gpu as G
