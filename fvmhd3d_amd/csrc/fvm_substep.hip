@@ -86,10 +86,6 @@ __global__ void k_sub_fill_i(int *__restrict__ p, long long n, int v) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
-__global__ void k_sub_fill_d(double *__restrict__ p, long long n, double v) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
 
 // mark the faces of the active cells
 __global__ void k_sub_mark_faces(const int *__restrict__ A, int nA, const int *__restrict__ goff,
@@ -177,31 +173,8 @@ __global__ void k_sub_local_cells(const int *__restrict__ A, int nA, const int *
   cvol[j] = gvol[g];
 }
 
-// local[k][i] = global[k][perm[i]] for i < ncopy, zero for ncopy <= i < npl
-template <typename S>
-__global__ void k_sub_gather(const S *__restrict__ G, int n, S *__restrict__ L, int npl, const int *__restrict__ perm,
-                         int width, int ncopy) {
-  const int i = gidx();
-  if (i >= npl) return;
-  const int g = perm[i];
-  for (int k = 0; k < width; k++) SFLD(L, k, i, npl) = i < ncopy ? SFLD(G, k, g, n) : (S)0;
-}
 
-__global__ void k_sub_flags(int *__restrict__ is_ngb, int *__restrict__ is_active, int npl, int nA) {
-  const int i = gidx();
-  if (i >= npl) return;
-  is_ngb[i] = i >= nA;
-  is_active[i] = i < nA;
-}
 
-// inactive neighbours predicted to the current time: w + Wt * (t - tbeg)
-__global__ void k_sub_predict_inactive(double *__restrict__ w, const double *__restrict__ wt,
-                                   const double *__restrict__ tbeg, int npl, int nA, double t_global) {
-  const int i = nA + gidx();
-  if (i >= npl) return;
-  const double dtp = t_global - tbeg[i];
-  for (int k = 0; k < 10; k++) SFLD(w, k, i, npl) = SFLD(w, k, i, npl) + SFLD(wt, k, i, npl) * dtp;
-}
 
 // new rungs of the active cells from dt_new (Scheduler.h get_rung with the
 // driver's clamp, Iterate.cpp:146-168): exponent difference to dtmax, at
@@ -223,23 +196,90 @@ __global__ void k_sub_rungs(const double *__restrict__ dt_new, double *__restric
   tend[i] = t_global + ldexp(dt_tick, kRungMax - 1 - r);
 }
 
-// active cells: the owner takes the local values
-template <typename S>
-__global__ void k_sub_scatter(const S *__restrict__ L, int npl, S *__restrict__ G, int n, const int *__restrict__ perm,
-                          int nA, int width) {
-  const int i = gidx();
-  if (i >= nA) return;
-  const int g = perm[i];
-  for (int k = 0; k < width; k++) SFLD(G, k, g, n) = SFLD(L, k, i, npl);
-}
 
-// inactive neighbours: the owner adds the local dU (each owner appears once)
-__global__ void k_sub_accumulate(const double *__restrict__ L, int npl, double *__restrict__ G, int n,
-                             const int *__restrict__ perm, int nA, int width) {
-  const int i = nA + gidx();
+
+// The context's per-cell fields of one substep, all in one launch.  Per field:
+enum : signed char {
+  F_COPY = 0,     // kept field, copied for every local cell
+  F_ACTIVE = 1,   // kept field, copied for active cells, zero for neighbours (fresh dU)
+  F_PREDICT = 2,  // W: copied, neighbours predicted to the current time w + Wt*(t - tbeg)
+  F_ZERO = 3,     // rebuilt: zero
+  F_ONE = 4,      // rebuilt: one (the limiter)
+  F_NGB = 5,      // int: 1 for neighbours
+  F_ACT = 6,      // int: 1 for active cells
+  F_SKIP = 7
+};
+struct FieldTable {
+  const void *G[FVM_NFIELDS];
+  void *L[FVM_NFIELDS];
+  signed char mode[FVM_NFIELDS], width[FVM_NFIELDS], is_int[FVM_NFIELDS];
+};
+__global__ void k_sub_gather_all(FieldTable T, int n, int npl, int nA, const int *__restrict__ perm, double t_global,
+                                 const double *__restrict__ gwt, const double *__restrict__ gtbeg) {
+  const int i = gidx();
   if (i >= npl) return;
   const int g = perm[i];
-  for (int k = 0; k < width; k++) SFLD(G, k, g, n) += SFLD(L, k, i, npl);
+  for (int f = 0; f < FVM_NFIELDS; f++) {
+    const int w = T.width[f];
+    switch (T.mode[f]) {
+      case F_COPY:
+      case F_ACTIVE:
+        if (T.is_int[f]) {
+          for (int k = 0; k < w; k++)
+            SFLD((int *)T.L[f], k, i, npl) =
+                (T.mode[f] == F_ACTIVE && i >= nA) ? 0 : SFLD((const int *)T.G[f], k, g, n);
+        } else {
+          for (int k = 0; k < w; k++)
+            SFLD((double *)T.L[f], k, i, npl) =
+                (T.mode[f] == F_ACTIVE && i >= nA) ? 0.0 : SFLD((const double *)T.G[f], k, g, n);
+        }
+        break;
+      case F_PREDICT: {
+        const double *G = (const double *)T.G[f];
+        double *L = (double *)T.L[f];
+        if (i < nA) {
+          for (int k = 0; k < w; k++) SFLD(L, k, i, npl) = SFLD(G, k, g, n);
+        } else {
+          const double dtp = t_global - gtbeg[g];
+          for (int k = 0; k < w; k++) SFLD(L, k, i, npl) = SFLD(G, k, g, n) + SFLD(gwt, k, g, n) * dtp;
+        }
+        break;
+      }
+      case F_ZERO:
+      case F_ONE:
+        for (int k = 0; k < w; k++) SFLD((double *)T.L[f], k, i, npl) = T.mode[f] == F_ONE ? 1.0 : 0.0;
+        break;
+      case F_NGB:
+        ((int *)T.L[f])[i] = i >= nA;
+        break;
+      case F_ACT:
+        ((int *)T.L[f])[i] = i < nA;
+        break;
+      default:
+        break;
+    }
+  }
+}
+
+// Write-back of one substep in one launch: the active cells' fields go to
+// their owners, the neighbours' fresh dU, gradPsi and divB are added.
+struct BackTable {
+  double *G[FVM_NFIELDS];
+  const double *L[FVM_NFIELDS];
+  signed char back[FVM_NFIELDS], accum[FVM_NFIELDS], width[FVM_NFIELDS];
+};
+__global__ void k_sub_writeback(BackTable T, int n, int npl, int nA, const int *__restrict__ perm) {
+  const int i = gidx();
+  if (i >= npl) return;
+  const int g = perm[i];
+  for (int f = 0; f < FVM_NFIELDS; f++) {
+    const int w = T.width[f];
+    if (i < nA && T.back[f]) {
+      for (int k = 0; k < w; k++) SFLD(T.G[f], k, g, n) = SFLD(T.L[f], k, i, npl);
+    } else if (i >= nA && T.accum[f]) {
+      for (int k = 0; k < w; k++) SFLD(T.G[f], k, g, n) += SFLD(T.L[f], k, i, npl);
+    }
+  }
 }
 
 // rung histogram of every cell (block-local counts first)
@@ -247,10 +287,26 @@ __global__ void k_sub_rung_hist(const int *__restrict__ rung, int n, int *__rest
   __shared__ int h[kRungMax];
   if (threadIdx.x < kRungMax) h[threadIdx.x] = 0;
   __syncthreads();
-  const int i = gidx();
-  if (i < n) {
-    const int r = rung[i];
-    atomicAdd(&h[(r >= 0 && r < kRungMax) ? r : kRungMax - 1], 1);
+  // a fixed grid strides over the cells, so few blocks add into the global
+  // bins; one LDS add per distinct rung of a wavefront (nearby cells share few)
+  const int lane = (int)(threadIdx.x & 63);
+  const int stride = (int)(gridDim.x * blockDim.x);
+  const int first = (int)(blockIdx.x * blockDim.x);
+  for (int base = first; base < n; base += stride) {
+    const int i = base + (int)threadIdx.x;
+    int r = -1;
+    if (i < n) {
+      r = rung[i];
+      r = (r >= 0 && r < kRungMax) ? r : kRungMax - 1;
+    }
+    unsigned long long left = __ballot(r >= 0);
+    while (left) {
+      const int lead = __ffsll((long long)left) - 1;
+      const int rv = __shfl(r, lead);
+      const unsigned long long same = __ballot(r == rv);
+      if (lane == lead) atomicAdd(&h[rv], __popcll(same));
+      left &= ~same;
+    }
   }
   __syncthreads();
   if (threadIdx.x < kRungMax && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
@@ -479,26 +535,28 @@ int build_layout(fvm_ctx *c, unsigned mask, double t_global, int check, int *cou
   c->nfa = -1;
   c->have_mesh = true;
   c->t_global = t_global;
-  // the context's fields
+  // the context's fields, one launch
+  FieldTable T{};
   for (int f = 0; f < FVM_NFIELDS; f++) {
-    void *L = c->field[f].p;
-    const int w = kWidth[f];
-    if (kept(f)) {
-      const bool fresh_dU = f == FVM_FIELD_DU || f == FVM_FIELD_DGRADPSI || f == FVM_FIELD_DDIVB;
-      const int ncopy = fresh_dU ? nA : npl;
-      if (kInt[f])
-        SLAUNCH(c, npl, k_sub_gather<int>, P<int>(s.g[f]), n, (int *)L, npl, P<int>(s.perm), w, ncopy);
-      else
-        SLAUNCH(c, npl, k_sub_gather<double>, P<double>(s.g[f]), n, (double *)L, npl, P<int>(s.perm), w, ncopy);
-    } else if (f == FVM_FIELD_LIMITER) {
-      SLAUNCH(c, (long long)w * npl, k_sub_fill_d, (double *)L, (long long)w * npl, 1.0);
-    } else if (!kInt[f]) {
-      SLAUNCH(c, (long long)w * npl, k_sub_fill_d, (double *)L, (long long)w * npl, 0.0);
-    }
+    T.L[f] = c->field[f].p;
+    T.G[f] = kept(f) ? s.g[f].p : nullptr;
+    T.width[f] = (signed char)kWidth[f];
+    T.is_int[f] = kInt[f];
+    if (f == FVM_FIELD_W)
+      T.mode[f] = F_PREDICT;
+    else if (kept(f))
+      T.mode[f] = (f == FVM_FIELD_DU || f == FVM_FIELD_DGRADPSI || f == FVM_FIELD_DDIVB) ? F_ACTIVE : F_COPY;
+    else if (f == FVM_FIELD_LIMITER)
+      T.mode[f] = F_ONE;
+    else if (f == FVM_FIELD_IS_NGB)
+      T.mode[f] = F_NGB;
+    else if (f == FVM_FIELD_IS_ACTIVE)
+      T.mode[f] = F_ACT;
+    else
+      T.mode[f] = kInt[f] ? F_SKIP : F_ZERO;
   }
-  SLAUNCH(c, npl, k_sub_flags, P<int>(c->field[FVM_FIELD_IS_NGB]), P<int>(c->field[FVM_FIELD_IS_ACTIVE]), npl, nA);
-  SLAUNCH(c, nI, k_sub_predict_inactive, P<double>(c->field[FVM_FIELD_W]), P<double>(c->field[FVM_FIELD_DWDT]),
-          P<double>(c->field[FVM_FIELD_TBEG]), npl, nA, t_global);
+  SLAUNCH(c, npl, k_sub_gather_all, T, n, npl, nA, P<int>(s.perm), t_global, P<double>(s.g[FVM_FIELD_DWDT]),
+          P<double>(s.g[FVM_FIELD_TBEG]));
   if (counts) {
     counts[0] = nA;
     counts[1] = nI;
@@ -529,7 +587,9 @@ int update_rungs(fvm_ctx *c, double dtmax, int min_rung, int nrung, double t_glo
 int rung_histogram(fvm_ctx *c, int *hist) {
   SubstepState &s = c->sub;
   SLAUNCH(c, kRungMax, k_sub_fill_i, P<int>(s.hist), kRungMax, 0);
-  SLAUNCH(c, s.n, k_sub_rung_hist, P<int>(s.rung), s.n, P<int>(s.hist));
+  hipLaunchKernelGGL(k_sub_rung_hist, dim3(std::min(nblocks(s.n), 1024u)), dim3(kBlock), 0, c->stream, P<int>(s.rung), s.n,
+                     P<int>(s.hist));
+  SCHK(c, hipGetLastError());
   if (hist) SCHK(c, hipMemcpyAsync(hist, s.hist.p, sizeof(int) * kRungMax, hipMemcpyDeviceToHost, c->stream));
   SCHK(c, hipStreamSynchronize(c->stream));
   return FVM_OK;
@@ -538,13 +598,19 @@ int rung_histogram(fvm_ctx *c, int *hist) {
 int write_back(fvm_ctx *c, bool full) {
   SubstepState &s = c->sub;
   const int n = s.n, npl = c->np, nA = c->nactive;
-  for (int f : kBack) {
-    if (!full && f != FVM_FIELD_TBEG && f != FVM_FIELD_TEND) continue;
-    SLAUNCH(c, nA, k_sub_scatter<double>, P<double>(c->field[f]), npl, P<double>(s.g[f]), n, P<int>(s.perm), nA, kWidth[f]);
-  }
+  BackTable T{};
+  for (int f : kBack)
+    if (full || f == FVM_FIELD_TBEG || f == FVM_FIELD_TEND) T.back[f] = 1;
   if (full)
-    for (int f : kAccum)
-      SLAUNCH(c, npl - nA, k_sub_accumulate, P<double>(c->field[f]), npl, P<double>(s.g[f]), n, P<int>(s.perm), nA, kWidth[f]);
+    for (int f : kAccum) T.accum[f] = 1;
+  for (int f = 0; f < FVM_NFIELDS; f++) {
+    T.width[f] = (signed char)kWidth[f];
+    if (T.back[f] || T.accum[f]) {
+      T.G[f] = P<double>(s.g[f]);
+      T.L[f] = P<double>(c->field[f]);
+    }
+  }
+  SLAUNCH(c, npl, k_sub_writeback, T, n, npl, nA, P<int>(s.perm));
   SCHK(c, hipStreamSynchronize(c->stream));
   return FVM_OK;
 }
