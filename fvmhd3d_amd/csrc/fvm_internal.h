@@ -67,12 +67,16 @@ struct SubstepState {
   // host side of the sweep: the global adjacency (neighbour per face-end),
   // pinned mirrors of dti and the active flags in mesh order (acth is all
   // zero between substeps), and scratch
-  std::vector<int> hoff, hnbr, permh, hoth, chg_idx;
+  std::vector<int> hoff, hnbr;
   std::vector<unsigned char> changed;
   std::vector<unsigned long long> pending;  // one bit per cell
-  std::vector<double> sdh, chg_val;
   double *dtih = nullptr;       // pinned, n
   unsigned char *acth = nullptr;  // pinned, n
+  double *sdh = nullptr;        // pinned, n: the sweep's values per layout cell
+  int *permh = nullptr;         // pinned, n: the layout's cells
+  int *hoth = nullptr;          // pinned, n: the hot cells
+  int *chgi = nullptr;          // pinned, n: changed cells
+  double *chgv = nullptr;       // pinned, n: their values
 };
 
 struct fvm_ctx {

@@ -450,8 +450,9 @@ void fvm_substep_release(fvm_ctx *c) {
                   &s.hot, &s.hpos, &s.hlist, &s.chg, &s.dtig, &s.actg})
     fvm_release(*b);
   for (auto &b : s.g) fvm_release(b);
-  if (s.dtih) (void)hipHostFree(s.dtih);
-  if (s.acth) (void)hipHostFree(s.acth);
+  for (void *h : {(void *)s.dtih, (void *)s.acth, (void *)s.sdh, (void *)s.permh, (void *)s.hoth, (void *)s.chgi,
+                  (void *)s.chgv})
+    if (h) (void)hipHostFree(h);
   s = SubstepState{};
 }
 
@@ -636,8 +637,7 @@ int substep_sweep(fvm_ctx *c) {
   }
   SLAUNCH(c, nA, k_sub_compact, P<int>(s.hot), P<int>(s.hpos), nA, 0, P<int>(s.hlist));
   SLAUNCH(c, nhot, k_sub_map, P<int>(s.hlist), nhot, P<int>(s.perm));
-  s.hoth.resize(nhot);
-  SCHK(c, hipMemcpyAsync(s.hoth.data(), s.hlist.p, 4 * (size_t)nhot, hipMemcpyDeviceToHost, st));
+  SCHK(c, hipMemcpyAsync(s.hoth, s.hlist.p, 4 * (size_t)nhot, hipMemcpyDeviceToHost, st));
   const bool global = (long long)npl * 8 > n;
   if (global) {
     SLAUNCH(c, n, k_sub_fill_u8, P<unsigned char>(s.actg), n);
@@ -647,10 +647,8 @@ int substep_sweep(fvm_ctx *c) {
     SCHK(c, hipMemcpyAsync(s.acth, s.actg.p, (size_t)n, hipMemcpyDeviceToHost, st));
     SCHK(c, hipStreamSynchronize(st));
   } else {
-    s.sdh.resize(npl);
-    s.permh.resize(npl);
-    SCHK(c, hipMemcpyAsync(s.sdh.data(), s.sdti.p, 8 * (size_t)npl, hipMemcpyDeviceToHost, st));
-    SCHK(c, hipMemcpyAsync(s.permh.data(), s.perm.p, 4 * (size_t)npl, hipMemcpyDeviceToHost, st));
+    SCHK(c, hipMemcpyAsync(s.sdh, s.sdti.p, 8 * (size_t)npl, hipMemcpyDeviceToHost, st));
+    SCHK(c, hipMemcpyAsync(s.permh, s.perm.p, 4 * (size_t)npl, hipMemcpyDeviceToHost, st));
     SCHK(c, hipStreamSynchronize(st));
     for (int l = 0; l < npl; l++) {
       const int g = s.permh[l];
@@ -659,7 +657,7 @@ int substep_sweep(fvm_ctx *c) {
     }
   }
   std::vector<int> chg;
-  sparse_sweep(s.hoff.data(), s.hnbr.data(), s.acth, s.hoth.data(), nhot, s.dtih, s.pending.data(), s.pending.size(),
+  sparse_sweep(s.hoff.data(), s.hnbr.data(), s.acth, s.hoth, nhot, s.dtih, s.pending.data(), s.pending.size(),
                s.changed.data(), chg);
   // acth is all zero between substeps
   if (global)
@@ -668,13 +666,14 @@ int substep_sweep(fvm_ctx *c) {
     for (int l = 0; l < nA; l++) s.acth[s.permh[l]] = 0;
   const int nchg = (int)chg.size();
   if (nchg > 0) {
-    s.chg_val.resize(nchg);
-    for (int k = 0; k < nchg; k++) s.chg_val[k] = s.dtih[chg[k]];
-    if (int rc = fvm_ensure(c, s.chg, 12 * (size_t)nchg + 8)) return rc;  // reserved at setup for every cell
+    for (int k = 0; k < nchg; k++) {
+      s.chgi[k] = chg[k];
+      s.chgv[k] = s.dtih[chg[k]];
+    }
     int *di = P<int>(s.chg);
-    double *dv = (double *)((char *)s.chg.p + 4 * (((size_t)nchg + 1) & ~(size_t)1));
-    SCHK(c, hipMemcpyAsync(di, chg.data(), 4 * (size_t)nchg, hipMemcpyHostToDevice, st));
-    SCHK(c, hipMemcpyAsync(dv, s.chg_val.data(), 8 * (size_t)nchg, hipMemcpyHostToDevice, st));
+    double *dv = (double *)((char *)s.chg.p + 4 * (((size_t)nchg + 1) & ~(size_t)1));  // reserved at setup: 12n + 8 bytes
+    SCHK(c, hipMemcpyAsync(di, s.chgi, 4 * (size_t)nchg, hipMemcpyHostToDevice, st));
+    SCHK(c, hipMemcpyAsync(dv, s.chgv, 8 * (size_t)nchg, hipMemcpyHostToDevice, st));
     SLAUNCH(c, nchg, k_sub_apply_changes, di, dv, nchg, P<int>(s.g2l), P<double>(c->field[FVM_FIELD_DT_NEW]));
   }
   SCHK(c, hipStreamSynchronize(st));
@@ -728,6 +727,11 @@ int fvm_substep_setup(fvm_ctx *c, int ncell, int nface, const fvm_face *faces, c
     return rc;
   SCHK(c, hipHostMalloc((void **)&s.dtih, 8 * N, hipHostMallocDefault));
   SCHK(c, hipHostMalloc((void **)&s.acth, N, hipHostMallocDefault));
+  SCHK(c, hipHostMalloc((void **)&s.sdh, 8 * N, hipHostMallocDefault));
+  SCHK(c, hipHostMalloc((void **)&s.permh, 4 * N, hipHostMallocDefault));
+  SCHK(c, hipHostMalloc((void **)&s.hoth, 4 * N, hipHostMallocDefault));
+  SCHK(c, hipHostMalloc((void **)&s.chgi, 4 * N, hipHostMallocDefault));
+  SCHK(c, hipHostMalloc((void **)&s.chgv, 8 * N, hipHostMallocDefault));
   std::memset(s.acth, 0, N);
   try {
     s.hoff.assign(cell_face_off, cell_face_off + N + 1);
@@ -783,6 +787,21 @@ int fvm_substep_setup(fvm_ctx *c, int ncell, int nface, const fvm_face *faces, c
   if (nref > 0) SCHK(c, hipMemcpy(s.gface.p, cell_face, 4 * (size_t)nref, hipMemcpyHostToDevice));
   SCHK(c, hipMemcpy(s.gvol.p, cell_volume, 8 * N, hipMemcpyHostToDevice));
   SCHK(c, hipStreamSynchronize(c->stream));
+  // first use of the host handover buffers and of the copy paths happens
+  // here, once (about 6 ms on the first medium-sized copy), not in a substep
+  std::memset(s.sdh, 0, 8 * N);
+  std::memset(s.permh, 0, 4 * N);
+  std::memset(s.hoth, 0, 4 * N);
+  std::memset(s.chgi, 0, 4 * N);
+  std::memset(s.chgv, 0, 8 * N);
+  std::memset(s.dtih, 0, 8 * N);
+  const void *zero = s.g[FVM_FIELD_DT_NEW].p;  // 8n zero bytes
+  SCHK(c, hipMemcpy(s.dtih, zero, 8 * N, hipMemcpyDeviceToHost));
+  SCHK(c, hipMemcpy(s.sdh, zero, 8 * N, hipMemcpyDeviceToHost));
+  SCHK(c, hipMemcpy(s.permh, zero, 4 * N, hipMemcpyDeviceToHost));
+  SCHK(c, hipMemcpy(s.hoth, zero, 4 * N, hipMemcpyDeviceToHost));
+  SCHK(c, hipMemcpy(s.chg.p, s.chgi, 4 * N, hipMemcpyHostToDevice));
+  SCHK(c, hipMemcpy(s.chg.p, s.chgv, 8 * N, hipMemcpyHostToDevice));
   s.n = ncell;
   s.nface = nface;
   s.nref = nref;
