@@ -198,7 +198,8 @@ __global__ void k_sub_rungs(const double *__restrict__ dt_new, double *__restric
 
 
 
-// The context's per-cell fields of one substep, all in one launch.  Per field:
+// The context's per-cell fields of one substep, all in one launch (one row
+// of blocks per field).  Per field:
 enum : signed char {
   F_COPY = 0,     // kept field, copied for every local cell
   F_ACTIVE = 1,   // kept field, copied for active cells, zero for neighbours (fresh dU)
@@ -217,9 +218,10 @@ struct FieldTable {
 __global__ void k_sub_gather_all(FieldTable T, int n, int npl, int nA, const int *__restrict__ perm, double t_global,
                                  const double *__restrict__ gwt, const double *__restrict__ gtbeg) {
   const int i = gidx();
+  const int f = (int)blockIdx.y;  // one row of blocks per field
   if (i >= npl) return;
   const int g = perm[i];
-  for (int f = 0; f < FVM_NFIELDS; f++) {
+  {
     const int w = T.width[f];
     switch (T.mode[f]) {
       case F_COPY:
@@ -261,8 +263,9 @@ __global__ void k_sub_gather_all(FieldTable T, int n, int npl, int nA, const int
   }
 }
 
-// Write-back of one substep in one launch: the active cells' fields go to
-// their owners, the neighbours' fresh dU, gradPsi and divB are added.
+// Write-back of one substep in one launch (one row of blocks per field): the
+// active cells' fields go to their owners, the neighbours' fresh dU, gradPsi
+// and divB are added.
 struct BackTable {
   double *G[FVM_NFIELDS];
   const double *L[FVM_NFIELDS];
@@ -270,9 +273,10 @@ struct BackTable {
 };
 __global__ void k_sub_writeback(BackTable T, int n, int npl, int nA, const int *__restrict__ perm) {
   const int i = gidx();
+  const int f = (int)blockIdx.y;  // one row of blocks per field
   if (i >= npl) return;
   const int g = perm[i];
-  for (int f = 0; f < FVM_NFIELDS; f++) {
+  {
     const int w = T.width[f];
     if (i < nA && T.back[f]) {
       for (int k = 0; k < w; k++) SFLD(T.G[f], k, g, n) = SFLD(T.L[f], k, i, npl);
@@ -556,8 +560,9 @@ int build_layout(fvm_ctx *c, unsigned mask, double t_global, int check, int *cou
     else
       T.mode[f] = kInt[f] ? F_SKIP : F_ZERO;
   }
-  SLAUNCH(c, npl, k_sub_gather_all, T, n, npl, nA, P<int>(s.perm), t_global, P<double>(s.g[FVM_FIELD_DWDT]),
-          P<double>(s.g[FVM_FIELD_TBEG]));
+  hipLaunchKernelGGL(k_sub_gather_all, dim3(nblocks(npl), FVM_NFIELDS), dim3(kBlock), 0, c->stream, T, n, npl, nA,
+                     P<int>(s.perm), t_global, P<double>(s.g[FVM_FIELD_DWDT]), P<double>(s.g[FVM_FIELD_TBEG]));
+  SCHK(c, hipGetLastError());
   if (counts) {
     counts[0] = nA;
     counts[1] = nI;
@@ -611,7 +616,9 @@ int write_back(fvm_ctx *c, bool full) {
       T.L[f] = P<double>(c->field[f]);
     }
   }
-  SLAUNCH(c, npl, k_sub_writeback, T, n, npl, nA, P<int>(s.perm));
+  hipLaunchKernelGGL(k_sub_writeback, dim3(nblocks(npl), FVM_NFIELDS), dim3(kBlock), 0, c->stream, T, n, npl, nA,
+                     P<int>(s.perm));
+  SCHK(c, hipGetLastError());
   SCHK(c, hipStreamSynchronize(c->stream));
   return FVM_OK;
 }
