@@ -77,6 +77,7 @@ struct SubstepState {
   int *hoth = nullptr;          // pinned, n: the hot cells
   int *chgi = nullptr;          // pinned, n: changed cells
   double *chgv = nullptr;       // pinned, n: their values
+  int *errh = nullptr;          // pinned: the layout's error flag
 };
 
 struct fvm_ctx {

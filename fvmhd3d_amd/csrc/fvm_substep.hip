@@ -455,7 +455,7 @@ void fvm_substep_release(fvm_ctx *c) {
     fvm_release(*b);
   for (auto &b : s.g) fvm_release(b);
   for (void *h : {(void *)s.dtih, (void *)s.acth, (void *)s.sdh, (void *)s.permh, (void *)s.hoth, (void *)s.chgi,
-                  (void *)s.chgv})
+                  (void *)s.chgv, (void *)s.errh})
     if (h) (void)hipHostFree(h);
   s = SubstepState{};
 }
@@ -492,11 +492,11 @@ int build_layout(fvm_ctx *c, unsigned mask, double t_global, int check, int *cou
   SLAUNCH(c, n, k_sub_active, P<int>(s.rung), P<double>(s.g[FVM_FIELD_TEND]), n, mask, t_global, check, P<int>(s.act),
           P<int>(s.err));
   int nA = 0, nfl = 0, nI = 0, nref = 0;
+  // the flag's copy completes with the scan's own synchronisation
+  SCHK(c, hipMemcpyAsync(s.errh, s.err.p, sizeof(int), hipMemcpyDeviceToHost, st));
   if (fvm::exclusive_scan_flags(c->scan, P<int>(s.act), P<int>(s.apos), n, &nA, st))
     return fvm_fail(c, FVM_EHIP, "substep: scan of active cells failed");
-  int e = 0;
-  if (int rc = read_err(c, &e)) return rc;
-  if (e & 1) return fvm_fail(c, FVM_ESTATE, "substep: an active cell does not end its step at the current time");
+  if (*s.errh & 1) return fvm_fail(c, FVM_ESTATE, "substep: an active cell does not end its step at the current time");
   SLAUNCH(c, n, k_sub_compact, P<int>(s.act), P<int>(s.apos), n, 0, P<int>(s.perm));
   SLAUNCH(c, nface, k_sub_fill_i, P<int>(s.fmark), nface, 0);
   SLAUNCH(c, nA, k_sub_mark_faces, P<int>(s.perm), nA, P<int>(s.goff), P<int>(s.gface), P<int>(s.fmark));
@@ -619,8 +619,7 @@ int write_back(fvm_ctx *c, bool full) {
   hipLaunchKernelGGL(k_sub_writeback, dim3(nblocks(npl), FVM_NFIELDS), dim3(kBlock), 0, c->stream, T, n, npl, nA,
                      P<int>(s.perm));
   SCHK(c, hipGetLastError());
-  SCHK(c, hipStreamSynchronize(c->stream));
-  return FVM_OK;
+  return FVM_OK;  // completes with the rung histogram's synchronisation, which always follows
 }
 
 // the timestep limiter of a substep: initial values and hot cells on the
@@ -638,10 +637,7 @@ int substep_sweep(fvm_ctx *c) {
   int nhot = 0;
   if (fvm::exclusive_scan_flags(c->scan, P<int>(s.hot), P<int>(s.hpos), nA, &nhot, st))
     return fvm_fail(c, FVM_EHIP, "substep: scan of the limiter's cells failed");
-  if (nhot == 0) {
-    SCHK(c, hipStreamSynchronize(st));
-    return FVM_OK;
-  }
+  if (nhot == 0) return FVM_OK;
   SLAUNCH(c, nA, k_sub_compact, P<int>(s.hot), P<int>(s.hpos), nA, 0, P<int>(s.hlist));
   SLAUNCH(c, nhot, k_sub_map, P<int>(s.hlist), nhot, P<int>(s.perm));
   SCHK(c, hipMemcpyAsync(s.hoth, s.hlist.p, 4 * (size_t)nhot, hipMemcpyDeviceToHost, st));
@@ -739,6 +735,8 @@ int fvm_substep_setup(fvm_ctx *c, int ncell, int nface, const fvm_face *faces, c
   SCHK(c, hipHostMalloc((void **)&s.hoth, 4 * N, hipHostMallocDefault));
   SCHK(c, hipHostMalloc((void **)&s.chgi, 4 * N, hipHostMallocDefault));
   SCHK(c, hipHostMalloc((void **)&s.chgv, 8 * N, hipHostMallocDefault));
+  SCHK(c, hipHostMalloc((void **)&s.errh, sizeof(int), hipHostMallocDefault));
+  *s.errh = 0;
   std::memset(s.acth, 0, N);
   try {
     s.hoff.assign(cell_face_off, cell_face_off + N + 1);
