@@ -915,10 +915,36 @@ __global__ void k_end_fill(MeshView M, int *cursor, int *sent) {
 }
 // order each cell's face-ends by face_active_list position (insertion sort of
 // short segments) so that every sum runs in the reference's face order
+// Each cell's face-end entries (a << 1 | side, all distinct) in ascending
+// order.  Up to END_SORT_REG entries are sorted in registers by a fully
+// unrolled odd-even transposition network (static indices, padded with
+// INT_MAX); longer lists are insertion-sorted in place.  Distinct keys have
+// one sorted order, so both give the same list.
+constexpr int END_SORT_REG = 48;
 __global__ void k_end_sort(const int *soff, int *sent, int n) {
   const int c = gid();
   if (c >= n) return;
   const int b = soff[c], e = soff[c + 1];
+  const int deg = e - b;
+  if (deg <= 1) return;
+  if (deg <= END_SORT_REG) {
+    int v[END_SORT_REG];
+#pragma unroll
+    for (int k = 0; k < END_SORT_REG; k++) v[k] = k < deg ? sent[b + k] : INT_MAX;
+#pragma unroll
+    for (int pass = 0; pass < END_SORT_REG; pass++) {
+#pragma unroll
+      for (int k = pass & 1; k + 1 < END_SORT_REG; k += 2) {
+        const int lo = min(v[k], v[k + 1]), hi = max(v[k], v[k + 1]);
+        v[k] = lo;
+        v[k + 1] = hi;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < END_SORT_REG; k++)
+      if (k < deg) sent[b + k] = v[k];
+    return;
+  }
   for (int p = b + 1; p < e; p++) {
     const int v = sent[p];
     int q = p - 1;
