@@ -899,19 +899,53 @@ __global__ void k_face_compact(MeshView M, const int *flag, const int *pos, int 
   const int e = gid();
   if (e < nref && flag[e]) fact[pos[e]] = M.cface[e];
 }
+// Consecutive active faces mostly share the cell whose list introduced them,
+// so lanes with the same target add once per wavefront: the lowest such lane
+// adds the lane count, and each lane takes base + its rank among them.  Every
+// lane of the wavefront runs the loop (inactive ones with target -1).
+__device__ __forceinline__ int wave_add(int *ctr, int t, bool want_pos) {
+  const int lane = (int)(threadIdx.x & 63);
+  unsigned long long left = __ballot(t >= 0);
+  int pos = -1;
+  while (left) {
+    const int lead = __ffsll((long long)left) - 1;
+    const int tv = __shfl(t, lead);
+    const unsigned long long same = __ballot(t == tv);
+    int base = 0;
+    if (lane == lead) base = atomicAdd(&ctr[tv], __popcll(same));
+    if (want_pos) {
+      base = __shfl(base, lead);
+      if (t == tv) pos = base + __popcll(same & ((1ull << lane) - 1ull));
+    }
+    left &= ~same;
+  }
+  return pos;
+}
 __global__ void k_end_count(MeshView M, int *cnt) {
   const int a = gid();
-  if (a >= M.nfa) return;
-  const int f = M.fact[a];
-  atomicAdd(&cnt[M.s1[f]], 1);
-  atomicAdd(&cnt[M.s2[f]], 1);
+  int t1 = -1, t2 = -1;
+  if (a < M.nfa) {
+    const int f = M.fact[a];
+    t1 = M.s1[f];
+    t2 = M.s2[f];
+  }
+  wave_add(cnt, t1, false);
+  wave_add(cnt, t2, false);
 }
 __global__ void k_end_fill(MeshView M, int *cursor, int *sent) {
   const int a = gid();
-  if (a >= M.nfa) return;
-  const int f = M.fact[a];
-  sent[atomicAdd(&cursor[M.s1[f]], 1)] = (a << 1) | 0;
-  sent[atomicAdd(&cursor[M.s2[f]], 1)] = (a << 1) | 1;
+  int t1 = -1, t2 = -1;
+  if (a < M.nfa) {
+    const int f = M.fact[a];
+    t1 = M.s1[f];
+    t2 = M.s2[f];
+  }
+  const int p1 = wave_add(cursor, t1, true);
+  const int p2 = wave_add(cursor, t2, true);
+  if (a < M.nfa) {
+    sent[p1] = (a << 1) | 0;
+    sent[p2] = (a << 1) | 1;
+  }
 }
 // order each cell's face-ends by face_active_list position (insertion sort of
 // short segments) so that every sum runs in the reference's face order
