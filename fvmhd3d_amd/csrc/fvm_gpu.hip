@@ -895,6 +895,21 @@ __global__ void k_face_firstflag(MeshView M, const int *bnd, int nactive, const 
   const bool ok = bnd[i] == NO_BOUNDARY;
   for (int e = M.coff[i]; e < M.coff[i + 1]; e++) flag[e] = (ok && firstpos[M.cface[e]] == e) ? 1 : 0;
 }
+// The same flags when every active cell lists all its faces (each face under
+// s1 when s1 is active and under s2 when s2 is active): a face first appears
+// in the lower-indexed of its two cells that are active and not on a
+// boundary, since the lists are concatenated in cell order.
+__global__ void k_face_firstflag_direct(MeshView M, const int *bnd, int nactive, int *flag) {
+  const int i = gid();
+  if (i >= nactive) return;
+  const bool ok = bnd[i] == NO_BOUNDARY;
+  for (int e = M.coff[i]; e < M.coff[i + 1]; e++) {
+    const int f = M.cface[e];
+    const int j = (M.s1[f] == i) ? M.s2[f] : M.s1[f];
+    const bool earlier = j < i && j < nactive && bnd[j] == NO_BOUNDARY;
+    flag[e] = (ok && !earlier) ? 1 : 0;
+  }
+}
 __global__ void k_face_compact(MeshView M, const int *flag, const int *pos, int nref, int *fact) {
   const int e = gid();
   if (e < nref && flag[e]) fact[pos[e]] = M.cface[e];
@@ -1214,8 +1229,19 @@ int fvm_upload_mesh(fvm_ctx *c, int np, int nactive, int nimport_loc, int nface,
       const fvm_face &F = faces[cell_face[e]];
       if (F.s1 != i && F.s2 != i) return fail(c, FVM_EARG, "fvm_upload_mesh: cell lists a face it does not own");
     }
+  // complete lists: every face listed once under each of its active cells
+  bool complete = true;
+  try {
+    std::vector<int> seen(nface, 0);
+    for (int e = 0; e < nref; e++) seen[cell_face[e]]++;
+    for (int f = 0; f < nface && complete; f++)
+      complete = seen[f] == (faces[f].s1 < nactive) + (faces[f].s2 < nactive);
+  } catch (...) {
+    complete = false;
+  }
   (void)hipSetDevice(c->device);
   c->have_mesh = false;  // until the new mesh is complete
+  c->lists_complete = complete;
   const bool resize = np != c->np;
   c->np = np;
   c->nactive = nactive;
@@ -1263,6 +1289,7 @@ int fvm_upload_faces(fvm_ctx *c, int np, int nactive, int nimport_loc, int nface
       return fail(c, FVM_EARG, "fvm_upload_faces: face cell index out of range (s1 must be active)");
   (void)hipSetDevice(c->device);
   c->have_mesh = false;  // until the new mesh is complete
+  c->lists_complete = true;  // built below, every face under each active cell
   const bool resize = np != c->np;
   for (int f = 0; f < FVM_NFIELDS; f++) {
     const size_t bytes = field_bytes(c, f, np);
@@ -1400,9 +1427,13 @@ int fvm_get_active_faces(fvm_ctx *c, int *nface_active) {
     return rc;
   int *firstpos = (int *)c->tmp1.p, *flag = (int *)c->tmp2.p, *pos = (int *)c->tmp0.p;
   fvm::MeshView M = meshview(c);
-  LAUNCH(c, fvm::k_fill_int, c->nface, firstpos, INT_MAX, c->nface);
-  LAUNCH(c, fvm::k_face_firstpos, c->nactive, M, (const int *)c->field[FVM_FIELD_BND].p, c->nactive, firstpos);
-  LAUNCH(c, fvm::k_face_firstflag, c->nactive, M, (const int *)c->field[FVM_FIELD_BND].p, c->nactive, firstpos, flag);
+  if (c->lists_complete) {
+    LAUNCH(c, fvm::k_face_firstflag_direct, c->nactive, M, (const int *)c->field[FVM_FIELD_BND].p, c->nactive, flag);
+  } else {
+    LAUNCH(c, fvm::k_fill_int, c->nface, firstpos, INT_MAX, c->nface);
+    LAUNCH(c, fvm::k_face_firstpos, c->nactive, M, (const int *)c->field[FVM_FIELD_BND].p, c->nactive, firstpos);
+    LAUNCH(c, fvm::k_face_firstflag, c->nactive, M, (const int *)c->field[FVM_FIELD_BND].p, c->nactive, firstpos, flag);
+  }
   int total = 0;
   if (nref > 0) {
     rc = fvm::exclusive_scan_flags(c->scan, flag, pos, nref, &total, c->stream);

@@ -88,6 +88,7 @@ struct fvm_ctx {
   double t_global = 0.0;
   int np = 0, nactive = 0, nimport_loc = 0, nface = 0, nref = 0, nfa = -1;
   bool have_mesh = false;
+  bool lists_complete = false;  // every face listed under each of its active cells
   DBuf field[FVM_NFIELDS];
   DBuf m_s1, m_s2, m_c, m_n, m_coff, m_cface, m_cvol, m_fact, m_soff, m_sent;
   DBuf m_cn;  // per-face geometry record (centroid, n) of the same faces as m_c / m_n

@@ -538,6 +538,7 @@ int build_layout(fvm_ctx *c, unsigned mask, double t_global, int check, int *cou
   c->nface = nfl;
   c->nref = nref;
   c->nfa = -1;
+  c->lists_complete = true;  // active cells list all their faces
   c->have_mesh = true;
   c->t_global = t_global;
   // the context's fields, one launch
