@@ -1,11 +1,15 @@
 """Benchmark: Voronoi face-flux updates/s on one MI355X.
 
-Workload (default): the MHD blast on 128^3 = 2,097,152 generators drawn
-uniform-random (Rand48, seed 123) in the periodic unit cube, rho = 1,
-p = 100 inside r < 0.1 and 0.1 outside, B = (10/sqrt2)(1,1,0), v = 0,
-gamma = 5/3, Courant 0.4, static mesh.  One step = one pass of the path over
-every face: CFL + predictor + reconstruction, limiter, flux + FP64 apply,
-on the FP32 face path (FP64 flux sums, pressure recovery and CFL).
+Workload (default): the 3D Orszag-Tang-type vortex on 256^3 = 16,777,216
+generators on a jittered periodic lattice (Rand48, seed 123, +-0.1 dx),
+rho = 25/(36 pi), p = 5/(12 pi), v = (-sin 2pi y, sin 2pi x, 0.1 sin 2pi z),
+B = (-sin 2pi y, sin 4pi x, 0)/sqrt(4 pi), gamma = 5/3, Courant 0.4, smooth
+limiter, static mesh.  One step = one pass of the path over every face: CFL +
+predictor + reconstruction, limiter, flux + FP64 apply, on the FP32 face path
+(FP64 flux sums, pressure recovery and CFL).  Outside the timed region the
+same initial state is run for the configuration's 100 steps on the FP32 face
+path and on the plain FP64 path, and the relative L1 difference of every
+conserved component, the dt history and the diagnostic counts are compared.
 
 Prints ONE JSON line.  Everything else goes to stderr.
 """
@@ -243,12 +247,49 @@ def surrogate_bench(iters, n=128, nsteps=16, max_faces=4_000_000):
             "workload": f"mhd_blast_{n}^3 after {nsteps} steps, first {nf} active faces"}
 
 
+def verify_fp32_vs_fp64(fresh, nsteps):
+    """The FP32 face path and the plain FP64 path from the same initial state for
+    nsteps global steps: relative L1 difference of every conserved component and of
+    rho, whether both chose the same dt every step, and both paths' diagnostic
+    counts (face states where the reference would assert, entropy and mass
+    fallbacks).  Outside the timed region."""
+    from fvmhd3d_amd import gpu as G
+    res = {}
+    for prec, name in ((G.FUSED_FP32, "fp32"), (G.FUSED_FP64, "fp64")):
+        c = fresh(prec)
+        c.reset_diagnostics()
+        c.L.fvm_synchronize(c.h)
+        t = time.perf_counter()
+        c.fused_run(nsteps, 1.0)
+        c.L.fvm_synchronize(c.h)
+        wall = time.perf_counter() - t
+        res[name] = dict(W=c.download("w"), U=c.download("U"), dts=c.fused_dt_history(nsteps),
+                         diag=c.diagnostics(), ms=wall * 1e3 / nsteps)
+        c.close()
+        log(f"[bench] {name} path: {nsteps} steps, {res[name]['ms']:.2f} ms/step, diag {res[name]['diag']}")
+    a, b = res["fp32"], res["fp64"]
+    l1 = lambda x, y: float(np.sum(np.abs(x - y)) / max(np.sum(np.abs(y)), 1e-300))
+    names = ["mass", "momx", "momy", "momz", "energy", "Bx", "By", "Bz", "psi_mass", "entropy_mass"]
+    per = {n: l1(a["U"][:, k], b["U"][:, k]) for k, n in enumerate(names)}
+    out = {"fp64_path_ms_per_step": round(b["ms"], 4),
+           "l1_rel_vs_fp64": {"rho": l1(a["W"][:, 0], b["W"][:, 0]), "E": per["energy"],
+                              "ethm": l1(a["W"][:, 4], b["W"][:, 4]), "steps": nsteps,
+                              "conserved": per},
+           "same_dt_every_step": bool(np.array_equal(np.asarray(a["dts"]), np.asarray(b["dts"]))),
+           "diagnostics": {"fp32": a["diag"], "fp64": b["diag"], "equal": a["diag"] == b["diag"]}}
+    log(f"[bench] FP32 vs FP64 after {nsteps} steps: {out['l1_rel_vs_fp64']}; same dt "
+        f"{out['same_dt_every_step']}; diagnostics equal {out['diagnostics']['equal']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="mhd_blast_128^3", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="orszag_tang_256^3", choices=sorted(WORKLOADS))
+    ap.add_argument("--verify-steps", type=int, default=0,
+                    help="steps of the FP32-vs-FP64 comparison (default: the configuration's step count)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--copy-only", action="store_true", help="only the stream-copy probe (counter calibration)")
@@ -340,21 +381,7 @@ def main():
                                   "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
 
     if not args.no_verify:
-        c64 = fresh(G.FUSED_FP64)
-        c64.fused_run(args.warmup, 1.0)
-        c64.L.fvm_synchronize(c64.h)
-        t64 = time.perf_counter()
-        c64.fused_run(args.steps, 1.0)
-        c64.L.fvm_synchronize(c64.h)
-        extra["fp64_path_ms_per_step"] = round((time.perf_counter() - t64) * 1e3 / args.steps, 4)
-        W64 = c64.download("w")
-        U64 = c64.download("U")
-        c64.close()
-        l1 = lambda a, b: float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
-        extra["l1_rel_vs_fp64"] = {"rho": l1(W32[:, 0], W64[:, 0]), "E": l1(U32[:, 4], U64[:, 4]),
-                                   "ethm": l1(W32[:, 4], W64[:, 4]), "steps": args.warmup + args.steps}
-        log(f"[bench] FP32 vs FP64 after {args.warmup + args.steps} steps: {extra['l1_rel_vs_fp64']}")
-
+        extra.update(verify_fp32_vs_fp64(fresh, args.verify_steps or WORKLOADS[args.workload][4]))
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(st, lim)
@@ -368,7 +395,7 @@ def main():
         "data": "synthetic (Rand48-seeded generators, analytic initial state)",
         "config": {"workload": args.workload, "cells": m.ncell, "faces": m.nface,
                    "generators": gen, "limiter": ["shipped", "minmod", "smooth"][lim], "mesh": "static"},
-        "roofline": roofline, "cpu_baseline": cpu, **extra,
+        "roofline": roofline, "cpu_baseline": cpu, "diagnostics_timed": diag, **extra,
     }
     print(json.dumps(out), flush=True)
 

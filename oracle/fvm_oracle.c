@@ -171,6 +171,9 @@ static inline real Phi(int mode, real x) {
   }
 }
 
+/* Phi itself, for the closed-form checks of each mode */
+API double o_limiter_phi(int mode, double x) { return Phi(mode, x); }
+
 /* limiter of computeReconstruction.cpp:19-40 (multiplies by 1/dWi) */
 static inline real limiter_rec(int mode, real dWi, real dWj_min, real dWj_max) {
   if (fabs(dWi) <= 1.0e-16 * smax(fabs(dWj_max), fabs(dWj_min))) return 1.0;

@@ -45,11 +45,15 @@ def build():
 _libs = {}
 
 
-def _lib(mode):
-    if mode not in _libs:
-        path = os.path.join(HERE, f"liboracle_{mode}.so")
-        if not os.path.exists(path):
-            build()
+def _lib(mode, path=None):
+    """The oracle library of ``mode``; ``path`` loads another build of the same
+    source instead (the tests' planted-mistake builds)."""
+    key = path or mode
+    if key not in _libs:
+        if path is None:
+            path = os.path.join(HERE, f"liboracle_{mode}.so")
+            if not os.path.exists(path):
+                build()
         L = ctypes.CDLL(path)
         for fn in ("o_compute_reconstruction", "o_slope_limiter", "o_compute_fluid_update",
                    "o_compute_timestep", "o_compute_timestep_limiter"):
@@ -67,19 +71,21 @@ def _lib(mode):
         L.o_step_global.restype = ctypes.c_double
         L.o_rung_dt.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_double]
         L.o_rung_dt.restype = ctypes.c_double
+        L.o_limiter_phi.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.o_limiter_phi.restype = ctypes.c_double
         L.o_sizeof_state.restype = ctypes.c_int
         assert L.o_sizeof_state() == ctypes.sizeof(OState), "o_state layout mismatch"
-        _libs[mode] = L
-    return _libs[mode]
+        _libs[key] = L
+    return _libs[key]
 
 
 class Oracle:
     """Runs the reference functions on a fvmhd3d_amd.state.CellState in place."""
 
-    def __init__(self, mode="f64", limiter=0):
+    def __init__(self, mode="f64", limiter=0, path=None):
         assert mode in ("f64", "shipped")
         self.mode = mode
-        self.L = _lib(mode)
+        self.L = _lib(mode, path)
         self.sdtype = np.float64 if mode == "f64" else np.float32
         self.limiter = limiter
         self.diag = dict(positivity=0, closure=0, ethm_failed=0, mass_failed=0, dsl=0)
@@ -169,6 +175,10 @@ class Oracle:
         st.t_global = o.t_global
         self._unpack(st, o, keep)
         return dts
+
+    def phi(self, mode, x):
+        """The limiter function Phi of ``mode`` at x."""
+        return self.L.o_limiter_phi(int(mode), float(x))
 
     def rung_dt(self, dtmax, min_rung, dt):
         return self.L.o_rung_dt(dtmax, min_rung, dt)
