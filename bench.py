@@ -86,18 +86,34 @@ def limit_bytes(ncell, nfe):
     return ncell * per_cell + nfe * 12
 
 
-def cpu_baseline(st, lim, budget_s=30.0):
-    """The oracle (CPU restatement, one core) on one global step of the same mesh."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(st, lim):
+    """The oracle's global step on the host cores (OpenMP over every thread the
+    process is given: OMP_NUM_THREADS, else the CPUs it may run on), one step of the
+    same mesh and state: CFL, predictor, reconstruction, limiter, flux scatter and
+    primitives.  The threaded passes are bit-identical to the serial oracle."""
     from oracle.oracle import Oracle
-    o = Oracle("f64", limiter=lim)
+    o = Oracle("f64_omp", limiter=lim)
     s = st.copy()
     o.get_active_faces(s)
     t = time.perf_counter()
-    o.step_global(s, 1.0, 1)
+    o.step_global_par(s, 1.0, 1)
     dt = time.perf_counter() - t
-    return {"value": len(s.face_act) / dt, "unit": "face-flux updates/s", "cores": 1, "kind": "port",
-            "sample": f"1 global step of the same mesh ({st.np} cells, {len(s.face_act)} faces), "
-                      f"oracle/fvm_oracle.c FP64, {dt:.1f} s"}
+    threads = o.threads()
+    return {"value": len(s.face_act) / dt, "unit": "face-flux updates/s", "cores": threads, "kind": "port",
+            "sample": f"1 global step of the same mesh and state ({st.np} cells, {len(s.face_act)} faces), "
+                      f"oracle/fvm_oracle.c FP64 o_step_global_par on {threads} OpenMP threads "
+                      f"(host: {len(os.sched_getaffinity(0))} usable CPUs, {_cpu_model()}), {dt:.1f} s"}
 
 
 def substep_cpu_baseline(m, W0, dtmax, nsub, budget_s=15.0):

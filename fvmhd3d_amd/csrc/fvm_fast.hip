@@ -44,13 +44,15 @@ int fvm_fused_setup(fvm_ctx *c, int precision, int moving_mesh) {
     return fvm_fail(c, FVM_EARG, "host allocation failed");
   }
   if (rc) return rc;
+  c->fz.np = c->np;
   c->fz.ready = true;
   return FVM_OK;
 }
 
 int fvm_fused_run(fvm_ctx *c, int nsteps, double dtmax, double *t_out) {
   if (!c) return FVM_EARG;
-  if (!c->fz.ready) return fvm_fail(c, FVM_ESTATE, "fvm_fused_run before fvm_fused_setup");
+  if (!c->fz.ready || !c->have_mesh || c->np != c->fz.np)
+    return fvm_fail(c, FVM_ESTATE, "fvm_fused_run without a fvm_fused_setup on the current mesh");
   if (nsteps < 0 || !(dtmax > 0.0)) return fvm_fail(c, FVM_EARG, "bad nsteps/dtmax");
   (void)hipSetDevice(c->device);
   FusedState &z = c->fz;
@@ -76,7 +78,8 @@ int fvm_fused_dt_history(fvm_ctx *c, double *dts, int n) {
 
 int fvm_fused_sync(fvm_ctx *c) {
   if (!c) return FVM_EARG;
-  if (!c->fz.ready) return fvm_fail(c, FVM_ESTATE, "no fused state");
+  if (!c->fz.ready || !c->have_mesh || c->np != c->fz.np)
+    return fvm_fail(c, FVM_ESTATE, "fvm_fused_sync without a fvm_fused_setup on the current mesh");
   (void)hipSetDevice(c->device);
   return c->fz.precision == FVM_FUSED_FP64 ? fvm_fused_sync_f64(c) : fvm_fused_sync_f32(c);
 }

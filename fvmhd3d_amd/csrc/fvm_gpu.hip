@@ -1206,6 +1206,30 @@ int fvm_set_time(fvm_ctx *c, double t) {
   return FVM_OK;
 }
 
+namespace fvm {
+// Complete cell lists: every face appears exactly once in the list of its s1
+// when s1 is active and exactly once in the list of its s2 when s2 is active
+// (localMesh.cpp:586-589).  Checked per (cell, face) pair; the lists must
+// already be known to hold only faces their cell owns.
+bool lists_complete(int nactive, int nface, const fvm_face *faces, const int *cell_face_off, const int *cell_face) {
+  try {
+    std::vector<unsigned char> under1(nface, 0), under2(nface, 0);
+    for (int i = 0; i < nactive; i++)
+      for (int e = cell_face_off[i]; e < cell_face_off[i + 1]; e++) {
+        const int f = cell_face[e];
+        unsigned char &u = faces[f].s1 == i ? under1[f] : under2[f];
+        if (u) return false;  // listed twice under the same cell
+        u = 1;
+      }
+    for (int f = 0; f < nface; f++)
+      if (under1[f] != (faces[f].s1 < nactive) || under2[f] != (faces[f].s2 < nactive)) return false;
+  } catch (...) {
+    return false;
+  }
+  return true;
+}
+}  // namespace fvm
+
 int fvm_upload_mesh(fvm_ctx *c, int np, int nactive, int nimport_loc, int nface, const fvm_face *faces,
                     const int *cell_face_off, const int *cell_face, const double *cell_volume) {
   if (!c) return FVM_EARG;
@@ -1229,17 +1253,9 @@ int fvm_upload_mesh(fvm_ctx *c, int np, int nactive, int nimport_loc, int nface,
       const fvm_face &F = faces[cell_face[e]];
       if (F.s1 != i && F.s2 != i) return fail(c, FVM_EARG, "fvm_upload_mesh: cell lists a face it does not own");
     }
-  // complete lists: every face listed once under each of its active cells
-  bool complete = true;
-  try {
-    std::vector<int> seen(nface, 0);
-    for (int e = 0; e < nref; e++) seen[cell_face[e]]++;
-    for (int f = 0; f < nface && complete; f++)
-      complete = seen[f] == (faces[f].s1 < nactive) + (faces[f].s2 < nactive);
-  } catch (...) {
-    complete = false;
-  }
+  bool complete = fvm::lists_complete(nactive, nface, faces, cell_face_off, cell_face);
   (void)hipSetDevice(c->device);
+  fvm_fast_release(c);   // the fused state belongs to the previous mesh
   c->have_mesh = false;  // until the new mesh is complete
   c->lists_complete = complete;
   const bool resize = np != c->np;
@@ -1288,6 +1304,7 @@ int fvm_upload_faces(fvm_ctx *c, int np, int nactive, int nimport_loc, int nface
     if (faces[f].s1 < 0 || faces[f].s1 >= nactive || faces[f].s2 < 0 || faces[f].s2 >= np || faces[f].s1 == faces[f].s2)
       return fail(c, FVM_EARG, "fvm_upload_faces: face cell index out of range (s1 must be active)");
   (void)hipSetDevice(c->device);
+  fvm_fast_release(c);   // the fused state belongs to the previous mesh
   c->have_mesh = false;  // until the new mesh is complete
   c->lists_complete = true;  // built below, every face under each active cell
   const bool resize = np != c->np;

@@ -25,6 +25,9 @@ int exclusive_scan(ScanScratch &s, const int *in, int *out, int n, hipStream_t s
 int exclusive_scan_flags(ScanScratch &s, const int *flags, int *out, int n, int *total, hipStream_t st);
 void scan_release(ScanScratch &s);
 
+// every face listed once under each of its active cells (fvm_gpu.hip)
+bool lists_complete(int nactive, int nface, const fvm_face *faces, const int *cell_face_off, const int *cell_face);
+
 }  // namespace fvm
 
 struct DBuf {
@@ -35,6 +38,7 @@ struct DBuf {
 // device-resident stepping state (fvm_fast.hip)
 struct FusedState {
   bool ready = false;
+  int np = 0;  // cells of the mesh the state was set up on
   int precision = 0;  // FVM_FUSED_FP64 / FVM_FUSED_FP32
   int moving = 0;
   int E = 0;
@@ -57,6 +61,7 @@ struct FusedState {
 // scratch of one substep
 struct SubstepState {
   bool ready = false;
+  bool complete = false;  // the global cell lists list every face under each of its cells
   int n = 0, nface = 0, nref = 0;
   DBuf gs1, gs2, gc, gn, goff, gface, gvol;  // global mesh (SoA)
   DBuf g[FVM_NFIELDS];                        // global fields, SoA [k*n + i]

@@ -514,6 +514,7 @@ int build_layout(fvm_ctx *c, unsigned mask, double t_global, int check, int *cou
   // the context's mesh (capacity for the whole mesh was reserved at setup);
   // not valid until the layout is complete
   c->have_mesh = false;
+  fvm_fast_release(c);
   SLAUNCH(c, nfl, k_sub_local_faces, P<int>(s.fl), nfl, P<int>(s.gs1), P<int>(s.gs2), P<double>(s.gc), P<double>(s.gn),
           nface, P<double>(s.g[FVM_FIELD_POS]), n, P<int>(s.act), P<int>(s.g2l), c->prm.domain_size[0],
           c->prm.domain_size[1], c->prm.domain_size[2], P<int>(c->m_s1), P<int>(c->m_s2), P<double>(c->m_c),
@@ -538,7 +539,7 @@ int build_layout(fvm_ctx *c, unsigned mask, double t_global, int check, int *cou
   c->nface = nfl;
   c->nref = nref;
   c->nfa = -1;
-  c->lists_complete = true;  // active cells list all their faces
+  c->lists_complete = s.complete;  // active cells list all their faces when the global lists are complete
   c->have_mesh = true;
   c->t_global = t_global;
   // the context's fields, one launch
@@ -716,9 +717,12 @@ int fvm_substep_setup(fvm_ctx *c, int ncell, int nface, const fvm_face *faces, c
     for (int e = cell_face_off[i]; e < cell_face_off[i + 1]; e++)
       if (faces[cell_face[e]].s1 != i && faces[cell_face[e]].s2 != i)
         return fvm_fail(c, FVM_EARG, "fvm_substep_setup: cell lists a face it does not own");
+  const bool complete = fvm::lists_complete(ncell, nface, faces, cell_face_off, cell_face);
   (void)hipSetDevice(c->device);
   fvm_substep_release(c);
+  fvm_fast_release(c);  // the context's mesh is replaced by the substep layouts
   SubstepState &s = c->sub;
+  s.complete = complete;
   const size_t N = (size_t)ncell, F = (size_t)nface;
   int rc;
   if ((rc = ensure_all(c, {{&s.gs1, 4 * F}, {&s.gs2, 4 * F}, {&s.gc, 24 * F}, {&s.gn, 24 * F}, {&s.goff, 4 * (N + 1)},

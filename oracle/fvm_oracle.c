@@ -39,6 +39,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #ifndef SREAL
 #define SREAL double
@@ -57,6 +60,11 @@ enum { NO_BOUNDARY = 0, REFLECTING = 1, OUTFLOW = 2, DIOD = 3, INFLOW = 4 };
 enum { LIM_SHIPPED = 0, LIM_MINMOD = 1, LIM_SMOOTH = 2 };
 
 #define API __attribute__((visibility("default")))
+/* diagnostic counters may be bumped from threads of the parallel global step */
+#define DIAG_INC(x) \
+  do {                \
+    _Pragma("omp atomic") (x)++; \
+  } while (0)
 
 typedef struct {
   int s1, s2;
@@ -506,12 +514,27 @@ API int o_get_active_faces(o_state *s, int *face_act_out) {
 /* ------------------------------------------------------------------ */
 /* computeReconstruction.cpp:42-264                                     */
 /* ------------------------------------------------------------------ */
-API void o_compute_reconstruction(o_state *s) {
+/* per-face scratch of one reconstruction (grown on demand) */
+typedef struct {
+  real *centroid_list, *dtij_list;
+  int cap;
+} o_rec_buf;
+
+static void rec_buf_init(o_rec_buf *b) {
+  b->cap = 64;
+  b->centroid_list = (real *)malloc(sizeof(real) * 3 * b->cap);
+  b->dtij_list = (real *)malloc(sizeof(real) * b->cap);
+}
+
+static void rec_buf_free(o_rec_buf *b) {
+  free(b->centroid_list);
+  free(b->dtij_list);
+}
+
+/* the body of the cell loop of computeReconstruction.cpp:57-262 for cell i */
+static void reconstruction_cell(o_state *s, int i, o_rec_buf *buf) {
   const double *L = s->L;
-  int cap = 64;
-  real *centroid_list = (real *)malloc(sizeof(real) * 3 * cap);
-  real *dtij_list = (real *)malloc(sizeof(real) * cap);
-  for (int i = 0; i < s->nactive; i++) {
+  {
     real Wi[NF];
     for (int k = 0; k < NF; k++) Wi[k] = s->w[i * NF + k];
     const real *ipos = s->pos + 3 * i;
@@ -524,7 +547,7 @@ API void o_compute_reconstruction(o_state *s) {
         s->z[i * NF + k] = 0.0;
         s->t[i * NF + k] = 0.0;
       }
-      continue;
+      return;
     }
 
     real Wx[NF], Wy[NF], Wz[NF];
@@ -532,11 +555,12 @@ API void o_compute_reconstruction(o_state *s) {
     for (int k = 0; k < NF; k++) Wx[k] = Wy[k] = Wz[k] = 0.0, dWj_min[k] = dWj_max[k] = 0.0;
 
     const int fb = s->cell_off[i], nface = s->cell_off[i + 1] - fb;
-    if (nface > cap) {
-      cap = 2 * nface;
-      centroid_list = (real *)realloc(centroid_list, sizeof(real) * 3 * cap);
-      dtij_list = (real *)realloc(dtij_list, sizeof(real) * cap);
+    if (nface > buf->cap) {
+      buf->cap = 2 * nface;
+      buf->centroid_list = (real *)realloc(buf->centroid_list, sizeof(real) * 3 * buf->cap);
+      buf->dtij_list = (real *)realloc(buf->dtij_list, sizeof(real) * buf->cap);
     }
+    real *centroid_list = buf->centroid_list, *dtij_list = buf->dtij_list;
     int zero_flag = 0;
     const real dtI = s->tend[i] - s->t_global;
 
@@ -597,7 +621,7 @@ API void o_compute_reconstruction(o_state *s) {
         s->z[i * NF + k] = 0.0;
         s->t[i * NF + k] = 0.0;
       }
-      continue;
+      return;
     }
 
     {
@@ -648,55 +672,60 @@ API void o_compute_reconstruction(o_state *s) {
       }
     }
   }
-  free(centroid_list);
-  free(dtij_list);
+}
+
+API void o_compute_reconstruction(o_state *s) {
+  o_rec_buf buf;
+  rec_buf_init(&buf);
+  for (int i = 0; i < s->nactive; i++) reconstruction_cell(s, i, &buf);
+  rec_buf_free(&buf);
 }
 
 /* ------------------------------------------------------------------ */
 /* slopeLimiter.cpp:27-237 (compute) and computeFluidUpdateI            */
 /* (computeFluidUpdate.cpp:25-75, apply)                                */
 /* ------------------------------------------------------------------ */
-API void o_slope_limiter(o_state *s) {
-  const int np = s->np;
-  real *area_test = (real *)calloc((size_t)np * 3, sizeof(real));
-  /* slopeLimiter_all / import limiters are 1 after localMesh_import (localMesh.cpp:57, 267) */
-  for (int i = 0; i < np * NF; i++) s->lim[i] = 1.0;
+/* one active face of slopeLimiter.cpp:40-216: the limiter minimum and the
+ * area-closure sums of the sides in `sides` (bit 0: s1, bit 1: s2); the
+ * dsl check is counted with the s1 side */
+static void slope_face(o_state *s, const o_face *face, int sides, real *area_test) {
+  const int i = face->s1;
+  const int j = face->s2;
+  const real dtI = s->tend[i] - s->t_global;
+  const real dtJ = s->tend[j] - s->t_global;
+  const real dth = 0.5 * smin(dtI, dtJ);
 
-  for (int fa = 0; fa < s->nface_act; fa++) {
-    const o_face *face = &s->faces[s->face_act[fa]];
-    const int i = face->s1;
-    const int j = face->s2;
-    const real dtI = s->tend[i] - s->t_global;
-    const real dtJ = s->tend[j] - s->t_global;
-    const real dth = 0.5 * smin(dtI, dtJ);
+  const real *ip = s->pos + 3 * i;
+  const real dri[3] = {face->centroid[0] - ip[0], face->centroid[1] - ip[1],
+                       face->centroid[2] - ip[2]};
+  const real *fn = face->n;
+  const real area = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+  const real ia = 1.0 / area;
+  const real normal[3] = {fn[0] * ia, fn[1] * ia, fn[2] * ia};
+  const real dsh = dri[0] * normal[0] + dri[1] * normal[1] + dri[2] * normal[2];
+  const real dsl = 2.0 * dsh;
+  if (!(dsl > 0.0) && (sides & 1)) DIAG_INC(s->n_dsl_viol);
+  const real drj[3] = {dri[0] - normal[0] * dsl, dri[1] - normal[1] * dsl,
+                       dri[2] - normal[2] * dsl};
 
-    const real *ip = s->pos + 3 * i;
-    const real dri[3] = {face->centroid[0] - ip[0], face->centroid[1] - ip[1],
-                         face->centroid[2] - ip[2]};
-    const real *fn = face->n;
-    const real area = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
-    const real ia = 1.0 / area;
-    const real normal[3] = {fn[0] * ia, fn[1] * ia, fn[2] * ia};
-    const real dsh = dri[0] * normal[0] + dri[1] * normal[1] + dri[2] * normal[2];
-    const real dsl = 2.0 * dsh;
-    if (!(dsl > 0.0)) s->n_dsl_viol++;
-    const real drj[3] = {dri[0] - normal[0] * dsl, dri[1] - normal[1] * dsl,
-                         dri[2] - normal[2] * dsl};
+  for (int d = 0; d < 3; d++) {
+    const real an = normal[d] * area;
+    if (sides & 1) area_test[3 * i + d] = area_test[3 * i + d] + an;
+    if (sides & 2) area_test[3 * j + d] = area_test[3 * j + d] - an;
+  }
 
-    for (int d = 0; d < 3; d++) {
-      const real an = normal[d] * area;
-      area_test[3 * i + d] = area_test[3 * i + d] + an;
-      area_test[3 * j + d] = area_test[3 * j + d] - an;
-    }
-
-    if (s->bnd[i] != NO_BOUNDARY || s->bnd[j] != NO_BOUNDARY) {
-      for (int k = 0; k < NF; k++) s->lim[i * NF + k] = s->lim[j * NF + k] = 0;
-      continue;
-    }
-
+  if (s->bnd[i] != NO_BOUNDARY || s->bnd[j] != NO_BOUNDARY) {
     for (int k = 0; k < NF; k++) {
-      const real dWit = (real)s->t[i * NF + k] * (dtI - dth);
-      const real dWjt = (real)s->t[j * NF + k] * (dtJ - dth);
+      if (sides & 1) s->lim[i * NF + k] = 0;
+      if (sides & 2) s->lim[j * NF + k] = 0;
+    }
+    return;
+  }
+
+  for (int k = 0; k < NF; k++) {
+    const real dWit = (real)s->t[i * NF + k] * (dtI - dth);
+    const real dWjt = (real)s->t[j * NF + k] * (dtJ - dth);
+    if (sides & 1) {
       const real dWix = (real)s->x[i * NF + k] * dri[0] + (real)s->y[i * NF + k] * dri[1] +
                         (real)s->z[i * NF + k] * dri[2];
       const real dWi = dWix + dWit;
@@ -704,7 +733,8 @@ API void o_slope_limiter(o_state *s) {
           (real)s->lim[i * NF + k],
           limiter_sl(s->limiter_mode, dWi, smin((real)s->mm_lo[i * NF + k] - s->w[i * NF + k], 0.0),
                      smax((real)s->mm_hi[i * NF + k] - s->w[i * NF + k], 0.0)));
-
+    }
+    if (sides & 2) {
       const real dWjx = (real)s->x[j * NF + k] * drj[0] + (real)s->y[j * NF + k] * drj[1] +
                         (real)s->z[j * NF + k] * drj[2];
       const real dWj = dWjx + dWjt;
@@ -714,27 +744,38 @@ API void o_slope_limiter(o_state *s) {
                      smax((real)s->mm_hi[j * NF + k] - s->w[j * NF + k], 0.0)));
     }
   }
+}
 
-  for (int i = 0; i < s->nactive; i++) {
-    if (s->bnd[i] != NO_BOUNDARY) continue;
-    const real *a = area_test + 3 * i;
-    const real aabs = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
-    if (!(aabs < 1.0e-10 * pow(s->cvol[i], 2.0 / 3.0))) s->n_closure_viol++;
+/* slopeLimiter.cpp:218-237: area closure of active cell i */
+static void closure_cell(o_state *s, int i, const real *area_test) {
+  if (s->bnd[i] != NO_BOUNDARY) return;
+  const real *a = area_test + 3 * i;
+  const real aabs = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  if (!(aabs < 1.0e-10 * pow(s->cvol[i], 2.0 / 3.0))) DIAG_INC(s->n_closure_viol);
+}
+
+/* computeFluidUpdateI (computeFluidUpdate.cpp:25-75) for cell i */
+static void limiter_apply_cell(o_state *s, int i) {
+  if (!(i < s->nactive || s->is_ngb[i])) return;
+  const real f0 = 1.0;
+  for (int k = 0; k < NF; k++) {
+    const real f = f0 * (real)s->lim[i * NF + k];
+    s->x[i * NF + k] = (sreal)((real)s->x[i * NF + k] * f);
+    s->y[i * NF + k] = (sreal)((real)s->y[i * NF + k] * f);
+    s->z[i * NF + k] = (sreal)((real)s->z[i * NF + k] * f);
+    s->t[i * NF + k] = (sreal)((real)s->t[i * NF + k] * f);
   }
+}
+
+API void o_slope_limiter(o_state *s) {
+  const int np = s->np;
+  real *area_test = (real *)calloc((size_t)np * 3, sizeof(real));
+  /* slopeLimiter_all / import limiters are 1 after localMesh_import (localMesh.cpp:57, 267) */
+  for (int i = 0; i < np * NF; i++) s->lim[i] = 1.0;
+  for (int fa = 0; fa < s->nface_act; fa++) slope_face(s, &s->faces[s->face_act[fa]], 3, area_test);
+  for (int i = 0; i < s->nactive; i++) closure_cell(s, i, area_test);
   free(area_test);
-
-  /* computeFluidUpdateI: apply to active cells and to neighbour ghosts */
-  for (int i = 0; i < np; i++) {
-    if (!(i < s->nactive || s->is_ngb[i])) continue;
-    const real f0 = 1.0;
-    for (int k = 0; k < NF; k++) {
-      const real f = f0 * (real)s->lim[i * NF + k];
-      s->x[i * NF + k] = (sreal)((real)s->x[i * NF + k] * f);
-      s->y[i * NF + k] = (sreal)((real)s->y[i * NF + k] * f);
-      s->z[i * NF + k] = (sreal)((real)s->z[i * NF + k] * f);
-      s->t[i * NF + k] = (sreal)((real)s->t[i * NF + k] * f);
-    }
-  }
+  for (int i = 0; i < np; i++) limiter_apply_cell(s, i);
 }
 
 /* ------------------------------------------------------------------ */
@@ -746,97 +787,119 @@ typedef struct {
   sreal divB;
 } o_fluidD;
 
+/* one active face of computeFluidUpdate.cpp:101-234: after the boundary swap
+ * the face adds -ft, +dPsi, +bt to cell *pi and +ft, -dPsi, -bt to cell *pj,
+ * with ft = flux*dt*A, dPsi = n psi dt A, bt = Bn dt A */
+static void fluid_face(o_state *s, const o_face *face, int *pi, int *pj, real *ft, real *dPsi, real *bt) {
+  int i = face->s1;
+  int j = face->s2;
+
+  const real *fn = face->n;
+  const real area = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+  const real ia = 1.0 / area;
+  real normal[3] = {fn[0] * ia, fn[1] * ia, fn[2] * ia};
+
+  const real *ip = s->pos + 3 * i;
+  real dri[3] = {face->centroid[0] - ip[0], face->centroid[1] - ip[1],
+                 face->centroid[2] - ip[2]};
+
+  if (s->bnd[i] != NO_BOUNDARY) {
+    const int tmp = i;
+    i = j;
+    j = tmp;
+    for (int d = 0; d < 3; d++) {
+      normal[d] = normal[d] * -1.0;
+      dri[d] = dri[d] * -1.0;
+    }
+  }
+
+  const real dtI = s->tend[i] - s->t_global;
+  const real dtJ = s->tend[j] - s->t_global;
+  const real dt = smin(dtI, dtJ);
+  const real dti = dtI - 0.5 * dt;
+  const real dtj = dtJ - 0.5 * dt;
+
+  const real dsh = dri[0] * normal[0] + dri[1] * normal[1] + dri[2] * normal[2];
+  const real dsl = 2.0 * dsh;
+  if (!(dsl > 0.0)) DIAG_INC(s->n_dsl_viol);
+
+  const real drj[3] = {dri[0] - normal[0] * dsl, dri[1] - normal[1] * dsl,
+                       dri[2] - normal[2] * dsl};
+
+  real Wi[NF], Wj[NF];
+  for (int k = 0; k < NF; k++) {
+    Wi[k] = s->w[(size_t)i * NF + k];
+    Wj[k] = s->w[(size_t)j * NF + k];
+  }
+  const int jbnd = s->bnd[j];
+  if (jbnd == NO_BOUNDARY) {
+    for (int k = 0; k < NF; k++) {
+      const size_t ik = (size_t)i * NF + k, jk = (size_t)j * NF + k;
+      Wi[k] += (real)s->x[ik] * dri[0] + (real)s->y[ik] * dri[1] + (real)s->z[ik] * dri[2] +
+               (real)s->t[ik] * dti;
+      Wj[k] += (real)s->x[jk] * drj[0] + (real)s->y[jk] * drj[1] + (real)s->z[jk] * drj[2] +
+               (real)s->t[jk] * dtj;
+    }
+  }
+  if (!(Wi[DENS] > 0.0) || !(Wi[ETHM] > 0.0) || !(Wj[DENS] > 0.0) || !(Wj[ETHM] > 0.0))
+    DIAG_INC(s->n_pos_viol);
+
+  const real vi[3] = {(real)s->vel[3 * i], (real)s->vel[3 * i + 1], (real)s->vel[3 * i + 2]};
+  const real vj[3] = {(real)s->vel[3 * j], (real)s->vel[3 * j + 1], (real)s->vel[3 * j + 2]};
+  const real vij[3] = {vj[0] + vi[0], vj[1] + vi[1], vj[2] + vi[2]};
+  const real dvij[3] = {vj[0] - vi[0], vj[1] - vi[1], vj[2] - vi[2]};
+  const real fij[3] = {dri[0] - normal[0] * dsh, dri[1] - normal[1] * dsh,
+                       dri[2] - normal[2] * dsh};
+  const real fdv = dvij[0] * fij[0] + dvij[1] * fij[1] + dvij[2] * fij[2];
+  const real rji[3] = {normal[0] * dsl, normal[1] * dsl, normal[2] * dsl};
+  const real r = 1.0 / sqr(dsl);
+  const real dwij[3] = {rji[0] * fdv * r, rji[1] * fdv * r, rji[2] * fdv * r};
+  const real wij[3] = {vij[0] * 0.5 - dwij[0], vij[1] * 0.5 - dwij[1], vij[2] * 0.5 - dwij[2]};
+
+  real psi, Bn, flux[NF];
+  compute_flux(s, wij, Wi, Wj, jbnd, normal, &psi, &Bn, flux);
+
+  const real tarea = dt * area;
+  dPsi[0] = normal[0] * (psi * tarea);
+  dPsi[1] = normal[1] * (psi * tarea);
+  dPsi[2] = normal[2] * (psi * tarea);
+  for (int k = 0; k < NF; k++) ft[k] = flux[k] * tarea;
+  *bt = Bn * tarea;
+  *pi = i;
+  *pj = j;
+}
+
+/* the FluidD accumulation of cell c for one face (computeFluidUpdate.cpp:221-234) */
+static inline void fluid_acc(o_fluidD *acc, int minus, const real *ft, const real *dPsi, real bt) {
+  if (minus) {
+    for (int k = 0; k < NF; k++) acc->U[k] = (sreal)((real)acc->U[k] - ft[k]);
+    for (int d = 0; d < 3; d++) acc->gradPsi[d] = acc->gradPsi[d] + (sreal)dPsi[d];
+    acc->divB = (sreal)((real)acc->divB + bt);
+  } else {
+    for (int k = 0; k < NF; k++) acc->U[k] = (sreal)((real)acc->U[k] + ft[k]);
+    for (int d = 0; d < 3; d++) acc->gradPsi[d] = acc->gradPsi[d] - (sreal)dPsi[d];
+    acc->divB = (sreal)((real)acc->divB - bt);
+  }
+}
+
+/* computeFluidUpdate.cpp:237-245: dU_act[i] += FluidD of cell i */
+static inline void fluid_store(o_state *s, int i, const o_fluidD *acc) {
+  s->ddivB[i] = s->ddivB[i] + acc->divB;
+  for (int d = 0; d < 3; d++) s->dgp[3 * i + d] = s->dgp[3 * i + d] + acc->gradPsi[d];
+  for (int k = 0; k < NF; k++) s->dU[i * NF + k] = s->dU[i * NF + k] + acc->U[k];
+}
+
 API void o_compute_fluid_update(o_state *s) {
   const int np = s->np;
   o_fluidD *acc = (o_fluidD *)calloc((size_t)np, sizeof(o_fluidD));
   for (int fa = 0; fa < s->nface_act; fa++) {
-    const o_face *face = &s->faces[s->face_act[fa]];
-    int i = face->s1;
-    int j = face->s2;
-
-    const real *fn = face->n;
-    const real area = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
-    const real ia = 1.0 / area;
-    real normal[3] = {fn[0] * ia, fn[1] * ia, fn[2] * ia};
-
-    const real *ip = s->pos + 3 * i;
-    real dri[3] = {face->centroid[0] - ip[0], face->centroid[1] - ip[1],
-                   face->centroid[2] - ip[2]};
-
-    if (s->bnd[i] != NO_BOUNDARY) {
-      const int tmp = i;
-      i = j;
-      j = tmp;
-      for (int d = 0; d < 3; d++) {
-        normal[d] = normal[d] * -1.0;
-        dri[d] = dri[d] * -1.0;
-      }
-    }
-
-    const real dtI = s->tend[i] - s->t_global;
-    const real dtJ = s->tend[j] - s->t_global;
-    const real dt = smin(dtI, dtJ);
-    const real dti = dtI - 0.5 * dt;
-    const real dtj = dtJ - 0.5 * dt;
-
-    const real dsh = dri[0] * normal[0] + dri[1] * normal[1] + dri[2] * normal[2];
-    const real dsl = 2.0 * dsh;
-    if (!(dsl > 0.0)) s->n_dsl_viol++;
-
-    const real drj[3] = {dri[0] - normal[0] * dsl, dri[1] - normal[1] * dsl,
-                         dri[2] - normal[2] * dsl};
-
-    real Wi[NF], Wj[NF];
-    for (int k = 0; k < NF; k++) {
-      Wi[k] = s->w[(size_t)i * NF + k];
-      Wj[k] = s->w[(size_t)j * NF + k];
-    }
-    const int jbnd = s->bnd[j];
-    if (jbnd == NO_BOUNDARY) {
-      for (int k = 0; k < NF; k++) {
-        const size_t ik = (size_t)i * NF + k, jk = (size_t)j * NF + k;
-        Wi[k] += (real)s->x[ik] * dri[0] + (real)s->y[ik] * dri[1] + (real)s->z[ik] * dri[2] +
-                 (real)s->t[ik] * dti;
-        Wj[k] += (real)s->x[jk] * drj[0] + (real)s->y[jk] * drj[1] + (real)s->z[jk] * drj[2] +
-                 (real)s->t[jk] * dtj;
-      }
-    }
-    if (!(Wi[DENS] > 0.0) || !(Wi[ETHM] > 0.0) || !(Wj[DENS] > 0.0) || !(Wj[ETHM] > 0.0))
-      s->n_pos_viol++;
-
-    const real vi[3] = {(real)s->vel[3 * i], (real)s->vel[3 * i + 1], (real)s->vel[3 * i + 2]};
-    const real vj[3] = {(real)s->vel[3 * j], (real)s->vel[3 * j + 1], (real)s->vel[3 * j + 2]};
-    const real vij[3] = {vj[0] + vi[0], vj[1] + vi[1], vj[2] + vi[2]};
-    const real dvij[3] = {vj[0] - vi[0], vj[1] - vi[1], vj[2] - vi[2]};
-    const real fij[3] = {dri[0] - normal[0] * dsh, dri[1] - normal[1] * dsh,
-                         dri[2] - normal[2] * dsh};
-    const real fdv = dvij[0] * fij[0] + dvij[1] * fij[1] + dvij[2] * fij[2];
-    const real rji[3] = {normal[0] * dsl, normal[1] * dsl, normal[2] * dsl};
-    const real r = 1.0 / sqr(dsl);
-    const real dwij[3] = {rji[0] * fdv * r, rji[1] * fdv * r, rji[2] * fdv * r};
-    const real wij[3] = {vij[0] * 0.5 - dwij[0], vij[1] * 0.5 - dwij[1], vij[2] * 0.5 - dwij[2]};
-
-    real psi, Bn, flux[NF];
-    compute_flux(s, wij, Wi, Wj, jbnd, normal, &psi, &Bn, flux);
-
-    const real tarea = dt * area;
-    const real dPsi[3] = {normal[0] * (psi * tarea), normal[1] * (psi * tarea),
-                          normal[2] * (psi * tarea)};
-
-    for (int k = 0; k < NF; k++) acc[i].U[k] = (sreal)((real)acc[i].U[k] - flux[k] * tarea);
-    for (int d = 0; d < 3; d++) acc[i].gradPsi[d] = acc[i].gradPsi[d] + (sreal)dPsi[d];
-    acc[i].divB = (sreal)((real)acc[i].divB + Bn * tarea);
-
-    for (int k = 0; k < NF; k++) acc[j].U[k] = (sreal)((real)acc[j].U[k] + flux[k] * tarea);
-    for (int d = 0; d < 3; d++) acc[j].gradPsi[d] = acc[j].gradPsi[d] - (sreal)dPsi[d];
-    acc[j].divB = (sreal)((real)acc[j].divB - Bn * tarea);
+    int i, j;
+    real ft[NF], dPsi[3], bt;
+    fluid_face(s, &s->faces[s->face_act[fa]], &i, &j, ft, dPsi, &bt);
+    fluid_acc(&acc[i], 1, ft, dPsi, bt);
+    fluid_acc(&acc[j], 0, ft, dPsi, bt);
   }
-
-  for (int i = 0; i < np; i++) {
-    s->ddivB[i] = s->ddivB[i] + acc[i].divB;
-    for (int d = 0; d < 3; d++) s->dgp[3 * i + d] = s->dgp[3 * i + d] + acc[i].gradPsi[d];
-    for (int k = 0; k < NF; k++) s->dU[i * NF + k] = s->dU[i * NF + k] + acc[i].U[k];
-  }
+  for (int i = 0; i < np; i++) fluid_store(s, i, &acc[i]);
   free(acc);
 }
 
@@ -852,172 +915,181 @@ API void o_compute_flux_batch(o_state *s, int n, const double *wij, const double
 /* ------------------------------------------------------------------ */
 /* computePrimitives, computeFluidUpdate.cpp:347-542                    */
 /* ------------------------------------------------------------------ */
-API void o_compute_primitives(o_state *s, int predictor_flag) {
+/* the cell loop of computeFluidUpdate.cpp:356-502 for active cell i; returns
+ * 1 when the mass is not positive (the cell is left for the fallback) */
+static int primitives_cell(o_state *s, int i, int predictor_flag) {
   const real gamma_gas = s->gamma_gas;
-  int *failed = (int *)malloc(sizeof(int) * (size_t)(s->nactive + 1));
-  int nfailed = 0;
-  for (int i = 0; i < s->nactive; i++) {
-    real *Ui = s->U + (size_t)i * NF;
-    real *wi = s->w + (size_t)i * NF;
-    if (s->bnd[i] != NO_BOUNDARY) {
-      to_primitive(Ui, s->mvol[i], wi);
-      for (int k = 0; k < NF; k++)
-        s->x[i * NF + k] = s->y[i * NF + k] = s->z[i * NF + k] = s->t[i * NF + k] = 0.0;
-      s->mvol[i] = s->cvol[i];
-      to_conservative(wi, s->mvol[i], Ui);
-      continue;
-    }
+  real *Ui = s->U + (size_t)i * NF;
+  real *wi = s->w + (size_t)i * NF;
+  if (s->bnd[i] != NO_BOUNDARY) {
+    to_primitive(Ui, s->mvol[i], wi);
+    for (int k = 0; k < NF; k++)
+      s->x[i * NF + k] = s->y[i * NF + k] = s->z[i * NF + k] = s->t[i * NF + k] = 0.0;
+    s->mvol[i] = s->cvol[i];
+    to_conservative(wi, s->mvol[i], Ui);
+    return 0;
+  }
 
-    real Uc[NF];
-    for (int k = 0; k < NF; k++) Uc[k] = Ui[k] + (real)s->dU[i * NF + k];
+  real Uc[NF];
+  for (int k = 0; k < NF; k++) Uc[k] = Ui[k] + (real)s->dU[i * NF + k];
 
-    const real divB = (real)s->ddivB[i];
-    const real dPsi[3] = {(real)s->dgp[3 * i], (real)s->dgp[3 * i + 1], (real)s->dgp[3 * i + 2]};
+  const real divB = (real)s->ddivB[i];
+  const real dPsi[3] = {(real)s->dgp[3 * i], (real)s->dgp[3 * i + 1], (real)s->dgp[3 * i + 2]};
 
-    const real dt = s->t_global - s->tbeg[i];
-    s->divBi[i] = dt > 0.0 ? divB / dt / s->cvol[i] : divB;
+  const real dt = s->t_global - s->tbeg[i];
+  s->divBi[i] = dt > 0.0 ? divB / dt / s->cvol[i] : divB;
 
-    if (Uc[MASS] <= 0.0) {
-      failed[nfailed++] = i;
-      s->n_mass_failed++;
-      continue;
+  if (Uc[MASS] <= 0.0) {
+    DIAG_INC(s->n_mass_failed);
+    return 1;
+  }
+
+  {
+    const real dth = 0.5 * dt;
+    real Wh[NF];
+    for (int k = 0; k < NF; k++) Wh[k] = wi[k] + (real)s->t[i * NF + k] * dth;
+
+    Uc[WBX] -= Wh[VELX] * divB;
+    Uc[WBY] -= Wh[VELY] * divB;
+    Uc[WBZ] -= Wh[VELZ] * divB;
+
+    /* DEDNER (computeFluidUpdate.cpp:3, 416-430) */
+    const real pres = P_pressure(s, Wh);
+    const real B2 = Wh[BX] * Wh[BX] + Wh[BY] * Wh[BY] + Wh[BZ] * Wh[BZ];
+    const real dcs2 = gamma_gas * pres + B2;
+    const real ch_sig = sqrt(dcs2 / Wh[DENS]);
+    Uc[WBX] -= dPsi[0];
+    Uc[WBY] -= dPsi[1];
+    Uc[WBZ] -= dPsi[2];
+    Uc[MPSI] -= dcs2 * divB;
+
+    if (!predictor_flag) {
+      const real *a0 = s->acc0 + 3 * i, *a1 = s->acc1 + 3 * i;
+      const real M0 = Ui[MASS], M1 = Uc[MASS];
+      real dmom[3];
+      for (int d = 0; d < 3; d++) dmom[d] = ((a0[d] * M0 + a1[d] * M1) * 0.5) * dt;
+      Uc[MOMX] += dmom[0];
+      Uc[MOMY] += dmom[1];
+      Uc[MOMZ] += dmom[2];
+      const real mom0[3] = {Ui[MOMX], Ui[MOMY], Ui[MOMZ]};
+      const real mom1[3] = {Uc[MOMX], Uc[MOMY], Uc[MOMZ]};
+      Uc[ENER] += ((mom0[0] * a0[0] + mom0[1] * a0[1] + mom0[2] * a0[2]) +
+                   (mom1[0] * a1[0] + mom1[1] * a1[1] + mom1[2] * a1[2])) *
+                  dt * 0.5;
     }
 
     {
-      const real dth = 0.5 * dt;
-      real Wh[NF];
-      for (int k = 0; k < NF; k++) Wh[k] = wi[k] + (real)s->t[i * NF + k] * dth;
-
-      Uc[WBX] -= Wh[VELX] * divB;
-      Uc[WBY] -= Wh[VELY] * divB;
-      Uc[WBZ] -= Wh[VELZ] * divB;
-
-      /* DEDNER (computeFluidUpdate.cpp:3, 416-430) */
-      const real pres = P_pressure(s, Wh);
-      const real B2 = Wh[BX] * Wh[BX] + Wh[BY] * Wh[BY] + Wh[BZ] * Wh[BZ];
-      const real dcs2 = gamma_gas * pres + B2;
-      const real ch_sig = sqrt(dcs2 / Wh[DENS]);
-      Uc[WBX] -= dPsi[0];
-      Uc[WBY] -= dPsi[1];
-      Uc[WBZ] -= dPsi[2];
-      Uc[MPSI] -= dcs2 * divB;
-
-      if (!predictor_flag) {
-        const real *a0 = s->acc0 + 3 * i, *a1 = s->acc1 + 3 * i;
-        const real M0 = Ui[MASS], M1 = Uc[MASS];
-        real dmom[3];
-        for (int d = 0; d < 3; d++) dmom[d] = ((a0[d] * M0 + a1[d] * M1) * 0.5) * dt;
-        Uc[MOMX] += dmom[0];
-        Uc[MOMY] += dmom[1];
-        Uc[MOMZ] += dmom[2];
-        const real mom0[3] = {Ui[MOMX], Ui[MOMY], Ui[MOMZ]};
-        const real mom1[3] = {Uc[MOMX], Uc[MOMY], Uc[MOMZ]};
-        Uc[ENER] += ((mom0[0] * a0[0] + mom0[1] * a0[1] + mom0[2] * a0[2]) +
-                     (mom1[0] * a1[0] + mom1[1] * a1[1] + mom1[2] * a1[2])) *
-                    dt * 0.5;
-      }
-
-      {
-        const double cr = 0.5;
-        const real l_min = pow(s->cvol[i] / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
-        Uc[MPSI] *= exp(-cr * ch_sig / l_min * dt);
-      }
-    }
-
-    real W1[NF];
-    to_primitive(Uc, s->cvol[i], W1);
-    /* Problem_compute_ethm_update returns W[ETHM] (Problem.cpp:228-232) */
-    if (W1[ETHM] <= 0.0) {
-      W1[ETHM] = P_ethm_from_entropy(s, W1);
-      s->n_ethm_failed++;
-    } else {
-      W1[ENTR] = P_entropy_from_ethm(s, W1);
-    }
-
-    if (predictor_flag) {
-      if (dt > 0.0)
-        for (int k = 0; k < NF; k++) s->t[i * NF + k] = (sreal)((W1[k] - wi[k]) / dt);
-      else
-        for (int k = 0; k < NF; k++) s->t[i * NF + k] = 0.0;
-    }
-
-    if (!predictor_flag) {
-      s->mvol[i] = s->cvol[i];
-      for (int k = 0; k < NF; k++) wi[k] = W1[k];
-      to_conservative(W1, s->cvol[i], Ui);
-      for (int k = 0; k < NF; k++) s->dU[i * NF + k] = 0.0;
-      s->dgp[3 * i] = s->dgp[3 * i + 1] = s->dgp[3 * i + 2] = 0.0;
-      s->ddivB[i] = 0.0;
+      const double cr = 0.5;
+      const real l_min = pow(s->cvol[i] / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
+      Uc[MPSI] *= exp(-cr * ch_sig / l_min * dt);
     }
   }
 
-  for (int f = 0; f < nfailed; f++) {
-    const int i = failed[f];
-    real *Ui = s->U + (size_t)i * NF;
-    real *wi = s->w + (size_t)i * NF;
-    const real dt = s->t_global - s->tbeg[i];
-    real W1[NF], Ut[NF];
-    for (int k = 0; k < NF; k++) W1[k] = wi[k] + (real)s->t[i * NF + k] * dt;
-    to_conservative(W1, s->mvol[i], Ut);
-    to_primitive(Ut, s->cvol[i], W1);
+  real W1[NF];
+  to_primitive(Uc, s->cvol[i], W1);
+  /* Problem_compute_ethm_update returns W[ETHM] (Problem.cpp:228-232) */
+  if (W1[ETHM] <= 0.0) {
+    W1[ETHM] = P_ethm_from_entropy(s, W1);
+    DIAG_INC(s->n_ethm_failed);
+  } else {
+    W1[ENTR] = P_entropy_from_ethm(s, W1);
+  }
+
+  if (predictor_flag) {
+    if (dt > 0.0)
+      for (int k = 0; k < NF; k++) s->t[i * NF + k] = (sreal)((W1[k] - wi[k]) / dt);
+    else
+      for (int k = 0; k < NF; k++) s->t[i * NF + k] = 0.0;
+  }
+
+  if (!predictor_flag) {
     s->mvol[i] = s->cvol[i];
-    if (W1[ETHM] <= 0.0) {
-      W1[ETHM] = P_ethm_from_entropy(s, W1);
-    } else {
-      W1[ENTR] = P_entropy_from_ethm(s, W1);
-    }
     for (int k = 0; k < NF; k++) wi[k] = W1[k];
     to_conservative(W1, s->cvol[i], Ui);
     for (int k = 0; k < NF; k++) s->dU[i * NF + k] = 0.0;
     s->dgp[3 * i] = s->dgp[3 * i + 1] = s->dgp[3 * i + 2] = 0.0;
     s->ddivB[i] = 0.0;
   }
+  return 0;
+}
+
+/* the failed-mass fallback of computeFluidUpdate.cpp:504-540 for cell i */
+static void primitives_failed_cell(o_state *s, int i) {
+  real *Ui = s->U + (size_t)i * NF;
+  real *wi = s->w + (size_t)i * NF;
+  const real dt = s->t_global - s->tbeg[i];
+  real W1[NF], Ut[NF];
+  for (int k = 0; k < NF; k++) W1[k] = wi[k] + (real)s->t[i * NF + k] * dt;
+  to_conservative(W1, s->mvol[i], Ut);
+  to_primitive(Ut, s->cvol[i], W1);
+  s->mvol[i] = s->cvol[i];
+  if (W1[ETHM] <= 0.0) {
+    W1[ETHM] = P_ethm_from_entropy(s, W1);
+  } else {
+    W1[ENTR] = P_entropy_from_ethm(s, W1);
+  }
+  for (int k = 0; k < NF; k++) wi[k] = W1[k];
+  to_conservative(W1, s->cvol[i], Ui);
+  for (int k = 0; k < NF; k++) s->dU[i * NF + k] = 0.0;
+  s->dgp[3 * i] = s->dgp[3 * i + 1] = s->dgp[3 * i + 2] = 0.0;
+  s->ddivB[i] = 0.0;
+}
+
+API void o_compute_primitives(o_state *s, int predictor_flag) {
+  int *failed = (int *)malloc(sizeof(int) * (size_t)(s->nactive + 1));
+  int nfailed = 0;
+  for (int i = 0; i < s->nactive; i++)
+    if (primitives_cell(s, i, predictor_flag)) failed[nfailed++] = i;
+  for (int f = 0; f < nfailed; f++) primitives_failed_cell(s, failed[f]);
   free(failed);
 }
 
 /* ------------------------------------------------------------------ */
 /* computeTimestep.cpp:6-62                                             */
 /* ------------------------------------------------------------------ */
-API void o_compute_timestep(o_state *s) {
+/* the cell loop of computeTimestep.cpp:13-60 for active cell i */
+static void timestep_cell(o_state *s, int i) {
   const real gamma_gas = s->gamma_gas;
-  for (int i = 0; i < s->nactive; i++) {
-    const real Volume = s->mvol[i];
-    real Wi[NF];
-    to_primitive(s->U + (size_t)i * NF, Volume, Wi);
-    const real h = pow(Volume / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
-    const real *pv = s->mvel + 3 * i;
+  const real Volume = s->mvol[i];
+  real Wi[NF];
+  to_primitive(s->U + (size_t)i * NF, Volume, Wi);
+  const real h = pow(Volume / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
+  const real *pv = s->mvel + 3 * i;
 
-    real csi = 0.0, vrel = 0.0;
-    const int isb = s->bnd[i] != NO_BOUNDARY;
-    if (!isb) {
-      const real Pi = P_pressure(s, Wi) * gamma_gas;
-      const real B2i = Wi[BX] * Wi[BX] + Wi[BY] * Wi[BY] + Wi[BZ] * Wi[BZ];
-      csi = sqrt((Pi + B2i) / Wi[DENS]);
-      const real dv[3] = {Wi[VELX] - pv[0], Wi[VELY] - pv[1], Wi[VELZ] - pv[2]};
-      vrel = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
-    }
-
-    real dti = h / (csi + vrel + 1.0 / REF_HUGE);
-    const real fac = isb ? 0.0 : 1.0;
-    const real vi[3] = {Wi[VELX], Wi[VELY], Wi[VELZ]};
-    const real wi[3] = {pv[0], pv[1], pv[2]};
-    for (int k = s->cell_off[i]; k < s->cell_off[i + 1]; k++) {
-      const o_face *face = &s->faces[s->cell_face[k]];
-      const int j = (i == face->s1) ? face->s2 : face->s1;
-      const real *wj_ = s->w + (size_t)j * NF;
-      const real vj[3] = {wj_[VELX], wj_[VELY], wj_[VELZ]};
-      const real dvi[3] = {vj[0] - vi[0], vj[1] - vi[1], vj[2] - vi[2]};
-      const real wj[3] = {(real)s->vel[3 * j], (real)s->vel[3 * j + 1], (real)s->vel[3 * j + 2]};
-      const real dwi[3] = {wj[0] - wi[0], wj[1] - wi[1], wj[2] - wi[2]};
-      const real dvmax =
-          sqrt((dwi[0] * dwi[0] + dwi[1] * dwi[1] + dwi[2] * dwi[2]) +
-               fac * (dvi[0] * dvi[0] + dvi[1] * dvi[1] + dvi[2] * dvi[2]) + 1.0 / REF_HUGE);
-      dti = smin(dti, h / (csi + dvmax));
-    }
-    /* Problem_extra_timestep_criterion returns HUGE (Problem.cpp:223-226) */
-    dti = smin(dti, REF_HUGE);
-    s->dt_new[i] = dti * s->courant_no;
+  real csi = 0.0, vrel = 0.0;
+  const int isb = s->bnd[i] != NO_BOUNDARY;
+  if (!isb) {
+    const real Pi = P_pressure(s, Wi) * gamma_gas;
+    const real B2i = Wi[BX] * Wi[BX] + Wi[BY] * Wi[BY] + Wi[BZ] * Wi[BZ];
+    csi = sqrt((Pi + B2i) / Wi[DENS]);
+    const real dv[3] = {Wi[VELX] - pv[0], Wi[VELY] - pv[1], Wi[VELZ] - pv[2]};
+    vrel = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
   }
+
+  real dti = h / (csi + vrel + 1.0 / REF_HUGE);
+  const real fac = isb ? 0.0 : 1.0;
+  const real vi[3] = {Wi[VELX], Wi[VELY], Wi[VELZ]};
+  const real wi[3] = {pv[0], pv[1], pv[2]};
+  for (int k = s->cell_off[i]; k < s->cell_off[i + 1]; k++) {
+    const o_face *face = &s->faces[s->cell_face[k]];
+    const int j = (i == face->s1) ? face->s2 : face->s1;
+    const real *wj_ = s->w + (size_t)j * NF;
+    const real vj[3] = {wj_[VELX], wj_[VELY], wj_[VELZ]};
+    const real dvi[3] = {vj[0] - vi[0], vj[1] - vi[1], vj[2] - vi[2]};
+    const real wj[3] = {(real)s->vel[3 * j], (real)s->vel[3 * j + 1], (real)s->vel[3 * j + 2]};
+    const real dwi[3] = {wj[0] - wi[0], wj[1] - wi[1], wj[2] - wi[2]};
+    const real dvmax =
+        sqrt((dwi[0] * dwi[0] + dwi[1] * dwi[1] + dwi[2] * dwi[2]) +
+             fac * (dvi[0] * dvi[0] + dvi[1] * dvi[1] + dvi[2] * dvi[2]) + 1.0 / REF_HUGE);
+    dti = smin(dti, h / (csi + dvmax));
+  }
+  /* Problem_extra_timestep_criterion returns HUGE (Problem.cpp:223-226) */
+  dti = smin(dti, REF_HUGE);
+  s->dt_new[i] = dti * s->courant_no;
+}
+
+API void o_compute_timestep(o_state *s) {
+  for (int i = 0; i < s->nactive; i++) timestep_cell(s, i);
 }
 
 /* computeTimestep.cpp:64-110 */
@@ -1066,121 +1138,140 @@ API double o_rung_dt(double dtmax, int min_rung, double dt) {
   }
 }
 
-API void o_compute_predictor(o_state *s, double dtmax, int min_rung) {
+/* one active face of computePredictor.cpp:47-140: after the boundary swap,
+ * fl = flux*A, gp = n psi A, bA = Bn A and the face's volume rate dV, added
+ * to cell *pi with - (and to *pj with +) */
+static void predictor_face(o_state *s, const o_face *face, int *pi, int *pj, int *pjbnd, real *fl, real *gp,
+                           real *bA, real *pdV) {
+  int i = face->s1;
+  int j = face->s2;
+  const real *fn = face->n;
+  const real area = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+  const real ia = 1.0 / area;
+  real normal[3] = {fn[0] * ia, fn[1] * ia, fn[2] * ia};
+  const real *ip = s->pos + 3 * i;
+  real dri[3] = {face->centroid[0] - ip[0], face->centroid[1] - ip[1],
+                 face->centroid[2] - ip[2]};
+  if (s->bnd[i] != NO_BOUNDARY) {
+    const int tmp = i;
+    i = j;
+    j = tmp;
+    for (int d = 0; d < 3; d++) {
+      normal[d] = normal[d] * -1.0;
+      dri[d] = dri[d] * -1.0;
+    }
+  }
+  const real dsh = dri[0] * normal[0] + dri[1] * normal[1] + dri[2] * normal[2];
+  const real dsl = 2.0 * dsh;
+  if (!(dsl > 0.0)) DIAG_INC(s->n_dsl_viol);
+
+  const int jbnd = s->bnd[j];
+  const real *Wi = s->w + (size_t)i * NF;
+  const real *Wj = (jbnd == NO_BOUNDARY || jbnd == INFLOW) ? s->w + (size_t)j * NF : Wi;
+
+  const real vi[3] = {(real)s->vel[3 * i], (real)s->vel[3 * i + 1], (real)s->vel[3 * i + 2]};
+  const real vj[3] = {(real)s->vel[3 * j], (real)s->vel[3 * j + 1], (real)s->vel[3 * j + 2]};
+  const real vij[3] = {vj[0] + vi[0], vj[1] + vi[1], vj[2] + vi[2]};
+  const real dvij[3] = {vj[0] - vi[0], vj[1] - vi[1], vj[2] - vi[2]};
+  const real fij[3] = {dri[0] - normal[0] * dsh, dri[1] - normal[1] * dsh,
+                       dri[2] - normal[2] * dsh};
+  const real fdv = dvij[0] * fij[0] + dvij[1] * fij[1] + dvij[2] * fij[2];
+  const real rij[3] = {normal[0] * dsl, normal[1] * dsl, normal[2] * dsl};
+  const real r = 1.0 / sqr(dsl);
+  const real dwij[3] = {rij[0] * fdv * r, rij[1] * fdv * r, rij[2] * fdv * r};
+  const real wij[3] = {vij[0] * 0.5 - dwij[0], vij[1] * 0.5 - dwij[1], vij[2] * 0.5 - dwij[2]};
+
+  real psi, Bn, flux[NF];
+  compute_flux(s, wij, Wi, Wj, jbnd, normal, &psi, &Bn, flux);
+
+  for (int k = 0; k < NF; k++) fl[k] = flux[k] * area;
+  for (int d = 0; d < 3; d++) gp[d] = normal[d] * (psi * area);
+  *bA = Bn * area;
+  *pdV = area * (rij[0] * wij[0] + rij[1] * wij[1] + rij[2] * wij[2]) / dsl;
+  *pi = i;
+  *pj = j;
+  *pjbnd = jbnd;
+}
+
+/* the dUdt/dVdt accumulation of one face into cell c (computePredictor.cpp:120-140) */
+static inline void predictor_acc(o_fluidD *d, real *dv, int minus, const real *fl, const real *gp, real bA, real dV) {
+  if (minus) {
+    for (int k = 0; k < NF; k++) d->U[k] = (sreal)((real)d->U[k] - fl[k]);
+    for (int q = 0; q < 3; q++) d->gradPsi[q] = d->gradPsi[q] + (sreal)gp[q];
+    d->divB = (sreal)((real)d->divB + bA);
+    *dv += dV;
+  } else {
+    *dv -= dV;
+    for (int k = 0; k < NF; k++) d->U[k] = (sreal)((real)d->U[k] + fl[k]);
+    for (int q = 0; q < 3; q++) d->gradPsi[q] = d->gradPsi[q] - (sreal)gp[q];
+    d->divB = (sreal)((real)d->divB - bA);
+  }
+}
+
+/* computePredictor.cpp:142-389 for active cell i, from its face sums */
+static void predictor_cell(o_state *s, int i, double dtmax, int min_rung, const o_fluidD *dUdti, real dVdti) {
   const real gamma_gas = s->gamma_gas;
+  if (s->bnd[i] != NO_BOUNDARY) return;
+  const sreal *dotUi = dUdti->U;
+  const real divB = (real)dUdti->divB; /* a float as shipped (computePredictor.cpp:147) */
+  const real dPsi[3] = {(real)dUdti->gradPsi[0], (real)dUdti->gradPsi[1],
+                        (real)dUdti->gradPsi[2]};
+  const real *U0 = s->U + (size_t)i * NF;
+  real W0[NF];
+  to_primitive(U0, s->cvol[i], W0);
+
+  /* the dt-halving loop of computePredictor.cpp:156-164 on mass */
+  while (U0[MASS] + (real)dotUi[MASS] * s->dt_new[i] <= 0.0) s->dt_new[i] *= 0.5;
+
+  const real dt = o_rung_dt(dtmax, min_rung, s->dt_new[i]) / 128;
+  real Uc[NF];
+  for (int k = 0; k < NF; k++) Uc[k] = U0[k] + (real)dotUi[k] * dt;
+
+  Uc[WBX] -= W0[VELX] * divB * dt;
+  Uc[WBY] -= W0[VELY] * divB * dt;
+  Uc[WBZ] -= W0[VELZ] * divB * dt;
+  {
+    const real pres = P_pressure(s, W0);
+    const real B2 = W0[BX] * W0[BX] + W0[BY] * W0[BY] + W0[BZ] * W0[BZ];
+    const real dcs2 = gamma_gas * pres + B2;
+    const real ch_sig = sqrt(dcs2 / W0[DENS]);
+    Uc[WBX] -= dPsi[0] * dt;
+    Uc[WBY] -= dPsi[1] * dt;
+    Uc[WBZ] -= dPsi[2] * dt;
+    Uc[MPSI] -= dcs2 * divB * dt;
+    const double cr = 0.5;
+    const real l_min = pow(s->cvol[i] / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
+    Uc[MPSI] *= exp(-cr * ch_sig / l_min * dt);
+  }
+
+  real W1[NF];
+  to_primitive(Uc, s->cvol[i] * exp(dVdti * dt / s->cvol[i]), W1);
+  if (W1[ETHM] <= 0.0) {
+    W1[ETHM] = P_ethm_from_entropy(s, W1);
+    DIAG_INC(s->n_ethm_failed);
+  } else {
+    W1[ENTR] = P_entropy_from_ethm(s, W1);
+  }
+  {
+    const real idt = 1.0 / dt;
+    for (int k = 0; k < NF; k++) s->t[i * NF + k] = (sreal)((W1[k] - W0[k]) * idt);
+  }
+}
+
+API void o_compute_predictor(o_state *s, double dtmax, int min_rung) {
   const int nactive = s->nactive;
   o_fluidD *dUdt = (o_fluidD *)calloc((size_t)nactive + 1, sizeof(o_fluidD));
   real *dVdt = (real *)calloc((size_t)nactive + 1, sizeof(real));
-
   for (int fa = 0; fa < s->nface_act; fa++) {
-    const o_face *face = &s->faces[s->face_act[fa]];
-    int i = face->s1;
-    int j = face->s2;
-    const real *fn = face->n;
-    const real area = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
-    const real ia = 1.0 / area;
-    real normal[3] = {fn[0] * ia, fn[1] * ia, fn[2] * ia};
-    const real *ip = s->pos + 3 * i;
-    real dri[3] = {face->centroid[0] - ip[0], face->centroid[1] - ip[1],
-                   face->centroid[2] - ip[2]};
-    if (s->bnd[i] != NO_BOUNDARY) {
-      const int tmp = i;
-      i = j;
-      j = tmp;
-      for (int d = 0; d < 3; d++) {
-        normal[d] = normal[d] * -1.0;
-        dri[d] = dri[d] * -1.0;
-      }
-    }
-    const real dsh = dri[0] * normal[0] + dri[1] * normal[1] + dri[2] * normal[2];
-    const real dsl = 2.0 * dsh;
-    if (!(dsl > 0.0)) s->n_dsl_viol++;
-
-    const int jbnd = s->bnd[j];
-    const real *Wi = s->w + (size_t)i * NF;
-    const real *Wj = (jbnd == NO_BOUNDARY || jbnd == INFLOW) ? s->w + (size_t)j * NF : Wi;
-
-    const real vi[3] = {(real)s->vel[3 * i], (real)s->vel[3 * i + 1], (real)s->vel[3 * i + 2]};
-    const real vj[3] = {(real)s->vel[3 * j], (real)s->vel[3 * j + 1], (real)s->vel[3 * j + 2]};
-    const real vij[3] = {vj[0] + vi[0], vj[1] + vi[1], vj[2] + vi[2]};
-    const real dvij[3] = {vj[0] - vi[0], vj[1] - vi[1], vj[2] - vi[2]};
-    const real fij[3] = {dri[0] - normal[0] * dsh, dri[1] - normal[1] * dsh,
-                         dri[2] - normal[2] * dsh};
-    const real fdv = dvij[0] * fij[0] + dvij[1] * fij[1] + dvij[2] * fij[2];
-    const real rij[3] = {normal[0] * dsl, normal[1] * dsl, normal[2] * dsl};
-    const real r = 1.0 / sqr(dsl);
-    const real dwij[3] = {rij[0] * fdv * r, rij[1] * fdv * r, rij[2] * fdv * r};
-    const real wij[3] = {vij[0] * 0.5 - dwij[0], vij[1] * 0.5 - dwij[1], vij[2] * 0.5 - dwij[2]};
-
-    real psi, Bn, flux[NF];
-    compute_flux(s, wij, Wi, Wj, jbnd, normal, &psi, &Bn, flux);
-
+    int i, j, jbnd;
+    real fl[NF], gp[3], bA, dV;
+    predictor_face(s, &s->faces[s->face_act[fa]], &i, &j, &jbnd, fl, gp, &bA, &dV);
     /* dUdt lives only for [0, nactive) (computePredictor.cpp:40-41) */
-    if (i < nactive) {
-      for (int k = 0; k < NF; k++) dUdt[i].U[k] = (sreal)((real)dUdt[i].U[k] - flux[k] * area);
-      for (int d = 0; d < 3; d++)
-        dUdt[i].gradPsi[d] = dUdt[i].gradPsi[d] + (sreal)(normal[d] * (psi * area));
-      dUdt[i].divB = (sreal)((real)dUdt[i].divB + Bn * area);
-    }
-    const real dV = area * (rij[0] * wij[0] + rij[1] * wij[1] + rij[2] * wij[2]) / dsl;
-    if (i < nactive) dVdt[i] += dV;
-
+    if (i < nactive) predictor_acc(&dUdt[i], &dVdt[i], 1, fl, gp, bA, dV);
     if (j >= nactive || jbnd != NO_BOUNDARY) continue;
-
-    dVdt[j] -= dV;
-    for (int k = 0; k < NF; k++) dUdt[j].U[k] = (sreal)((real)dUdt[j].U[k] + flux[k] * area);
-    for (int d = 0; d < 3; d++)
-      dUdt[j].gradPsi[d] = dUdt[j].gradPsi[d] - (sreal)(normal[d] * (psi * area));
-    dUdt[j].divB = (sreal)((real)dUdt[j].divB - Bn * area);
+    predictor_acc(&dUdt[j], &dVdt[j], 0, fl, gp, bA, dV);
   }
-
-  for (int i = 0; i < nactive; i++) {
-    if (s->bnd[i] != NO_BOUNDARY) continue;
-    const sreal *dotUi = dUdt[i].U;
-    const real divB = (real)dUdt[i].divB; /* a float as shipped (computePredictor.cpp:147) */
-    const real dPsi[3] = {(real)dUdt[i].gradPsi[0], (real)dUdt[i].gradPsi[1],
-                          (real)dUdt[i].gradPsi[2]};
-    const real *U0 = s->U + (size_t)i * NF;
-    real W0[NF];
-    to_primitive(U0, s->cvol[i], W0);
-
-    /* the dt-halving loop of computePredictor.cpp:156-164 on mass */
-    while (U0[MASS] + (real)dotUi[MASS] * s->dt_new[i] <= 0.0) s->dt_new[i] *= 0.5;
-
-    const real dt = o_rung_dt(dtmax, min_rung, s->dt_new[i]) / 128;
-    real Uc[NF];
-    for (int k = 0; k < NF; k++) Uc[k] = U0[k] + (real)dotUi[k] * dt;
-
-    Uc[WBX] -= W0[VELX] * divB * dt;
-    Uc[WBY] -= W0[VELY] * divB * dt;
-    Uc[WBZ] -= W0[VELZ] * divB * dt;
-    {
-      const real pres = P_pressure(s, W0);
-      const real B2 = W0[BX] * W0[BX] + W0[BY] * W0[BY] + W0[BZ] * W0[BZ];
-      const real dcs2 = gamma_gas * pres + B2;
-      const real ch_sig = sqrt(dcs2 / W0[DENS]);
-      Uc[WBX] -= dPsi[0] * dt;
-      Uc[WBY] -= dPsi[1] * dt;
-      Uc[WBZ] -= dPsi[2] * dt;
-      Uc[MPSI] -= dcs2 * divB * dt;
-      const double cr = 0.5;
-      const real l_min = pow(s->cvol[i] / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
-      Uc[MPSI] *= exp(-cr * ch_sig / l_min * dt);
-    }
-
-    real W1[NF];
-    to_primitive(Uc, s->cvol[i] * exp(dVdt[i] * dt / s->cvol[i]), W1);
-    if (W1[ETHM] <= 0.0) {
-      W1[ETHM] = P_ethm_from_entropy(s, W1);
-      s->n_ethm_failed++;
-    } else {
-      W1[ENTR] = P_entropy_from_ethm(s, W1);
-    }
-    {
-      const real idt = 1.0 / dt;
-      for (int k = 0; k < NF; k++) s->t[i * NF + k] = (sreal)((W1[k] - W0[k]) * idt);
-    }
-  }
+  for (int i = 0; i < nactive; i++) predictor_cell(s, i, dtmax, min_rung, &dUdt[i], dVdt[i]);
   free(dUdt);
   free(dVdt);
 }
@@ -1215,4 +1306,177 @@ API double o_step_global(o_state *s, double dtmax) {
     o_compute_primitives(s, 0);
     return dt;
   }
+}
+
+/* ------------------------------------------------------------------ */
+/* The same global step on every host core (OpenMP; the CPU baseline).  */
+/* The reference runs one chare per core (fvmhd3d.cpp:56-73); here the  */
+/* cell loops run in parallel as they are, and each face loop runs in   */
+/* two passes: the per-face work in parallel into one record per face,  */
+/* then each cell adds its faces' records in ascending face_act order,  */
+/* the order of the serial loops, so every cell's sums and therefore    */
+/* the results are bit-identical to o_step_global.  Needs every cell    */
+/* active and no boundary cells (the global step); otherwise it runs    */
+/* o_step_global.  Built single-threaded it runs the same passes.       */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int i, j, jbnd, pad;
+  real v[NF + 3 + 2]; /* flux term, psi term, Bn term, dV */
+} o_face_rec;
+
+API int o_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+API double o_step_global_par(o_state *s, double dtmax) {
+  const int n = s->np, nfa = s->nface_act;
+  int ok = s->nactive == n;
+  for (int i = 0; i < n && ok; i++) ok = s->bnd[i] == NO_BOUNDARY;
+  if (!ok) return o_step_global(s, dtmax);
+
+  /* each cell's active faces by ascending face_act position */
+  int *posf = (int *)malloc(sizeof(int) * ((size_t)s->nface + 1));
+  int *cpos = (int *)malloc(sizeof(int) * ((size_t)s->cell_off[n] + 1));
+  int *ccnt = (int *)malloc(sizeof(int) * ((size_t)n + 1));
+  o_face_rec *rec = (o_face_rec *)malloc(sizeof(o_face_rec) * ((size_t)nfa + 1));
+  int *failed = (int *)calloc((size_t)n + 1, sizeof(int));
+#pragma omp parallel for schedule(static)
+  for (int f = 0; f < s->nface; f++) posf[f] = -1;
+#pragma omp parallel for schedule(static)
+  for (int a = 0; a < nfa; a++) posf[s->face_act[a]] = a;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < n; c++) {
+    int *p = cpos + s->cell_off[c], m = 0;
+    for (int k = s->cell_off[c]; k < s->cell_off[c + 1]; k++) {
+      const int a = posf[s->cell_face[k]];
+      if (a < 0) continue;
+      int q = m++;
+      while (q > 0 && p[q - 1] > a) {
+        p[q] = p[q - 1];
+        q--;
+      }
+      p[q] = a;
+    }
+    ccnt[c] = m;
+  }
+  /* every active face must be in the lists of both its cells (else the serial step) */
+  {
+    long long listed = 0;
+#pragma omp parallel for schedule(static) reduction(+ : listed)
+    for (int c = 0; c < n; c++) {
+      const int *p = cpos + s->cell_off[c];
+      for (int q = 0; q < ccnt[c]; q++) {
+        const o_face *face = &s->faces[s->face_act[p[q]]];
+        listed += (face->s1 == c) + (face->s2 == c) * 2 + (q > 0 && p[q] == p[q - 1]) * 4;
+      }
+    }
+    if (listed != 3LL * nfa) {
+      free(posf);
+      free(cpos);
+      free(ccnt);
+      free(rec);
+      free(failed);
+      return o_step_global(s, dtmax);
+    }
+  }
+
+  /* computeTimestep */
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) timestep_cell(s, i);
+
+  /* computePredictor: faces, then each cell's sums and update */
+#pragma omp parallel for schedule(static)
+  for (int a = 0; a < nfa; a++) {
+    o_face_rec *r = &rec[a];
+    predictor_face(s, &s->faces[s->face_act[a]], &r->i, &r->j, &r->jbnd, r->v, r->v + NF, r->v + NF + 3,
+                   r->v + NF + 4);
+  }
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < n; c++) {
+    o_fluidD d;
+    real dv = 0.0;
+    memset(&d, 0, sizeof d);
+    const int *p = cpos + s->cell_off[c];
+    for (int q = 0; q < ccnt[c]; q++) {
+      const o_face_rec *r = &rec[p[q]];
+      if (r->i == c) predictor_acc(&d, &dv, 1, r->v, r->v + NF, r->v[NF + 3], r->v[NF + 4]);
+      else if (r->jbnd == NO_BOUNDARY) predictor_acc(&d, &dv, 0, r->v, r->v + NF, r->v[NF + 3], r->v[NF + 4]);
+    }
+    predictor_cell(s, c, dtmax, 0, &d, dv);
+  }
+
+  double dtmin = REF_HUGE;
+#pragma omp parallel for schedule(static) reduction(min : dtmin)
+  for (int i = 0; i < n; i++) dtmin = smin(dtmin, s->dt_new[i]);
+  const double dt = o_rung_dt(dtmax, 0, dtmin);
+  const double t0 = s->t_global;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    s->tbeg[i] = t0;
+    s->tend[i] = t0 + dt;
+  }
+
+  /* computeReconstruction */
+#pragma omp parallel
+  {
+    o_rec_buf buf;
+    rec_buf_init(&buf);
+#pragma omp for schedule(static)
+    for (int i = 0; i < n; i++) reconstruction_cell(s, i, &buf);
+    rec_buf_free(&buf);
+  }
+
+  /* slopeLimiter + computeFluidUpdateI: each cell takes its own side of its faces */
+  {
+    real *area_test = (real *)calloc((size_t)n * 3, sizeof(real));
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < n; c++) {
+      for (int k = 0; k < NF; k++) s->lim[c * NF + k] = 1.0;
+      const int *p = cpos + s->cell_off[c];
+      for (int q = 0; q < ccnt[c]; q++) {
+        const o_face *face = &s->faces[s->face_act[p[q]]];
+        slope_face(s, face, face->s1 == c ? 1 : 2, area_test);
+      }
+      closure_cell(s, c, area_test);
+      limiter_apply_cell(s, c);
+    }
+    free(area_test);
+  }
+
+  /* computeFluidUpdate: faces, then each cell's FluidD */
+#pragma omp parallel for schedule(static)
+  for (int a = 0; a < nfa; a++) {
+    o_face_rec *r = &rec[a];
+    fluid_face(s, &s->faces[s->face_act[a]], &r->i, &r->j, r->v, r->v + NF, r->v + NF + 3);
+  }
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < n; c++) {
+    o_fluidD acc;
+    memset(&acc, 0, sizeof acc);
+    const int *p = cpos + s->cell_off[c];
+    for (int q = 0; q < ccnt[c]; q++) {
+      const o_face_rec *r = &rec[p[q]];
+      fluid_acc(&acc, r->i == c, r->v, r->v + NF, r->v[NF + 3]);
+    }
+    fluid_store(s, c, &acc);
+  }
+
+  /* computePrimitives at t + dt */
+  s->t_global = t0 + dt;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) failed[i] = primitives_cell(s, i, 0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++)
+    if (failed[i]) primitives_failed_cell(s, i);
+
+  free(posf);
+  free(cpos);
+  free(ccnt);
+  free(rec);
+  free(failed);
+  return dt;
 }

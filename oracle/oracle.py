@@ -71,6 +71,10 @@ def _lib(mode, path=None):
         L.o_step_global.restype = ctypes.c_double
         L.o_rung_dt.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_double]
         L.o_rung_dt.restype = ctypes.c_double
+        if hasattr(L, "o_step_global_par"):
+            L.o_step_global_par.argtypes = [ctypes.POINTER(OState), ctypes.c_double]
+            L.o_step_global_par.restype = ctypes.c_double
+            L.o_threads.restype = ctypes.c_int
         L.o_limiter_phi.argtypes = [ctypes.c_int, ctypes.c_double]
         L.o_limiter_phi.restype = ctypes.c_double
         L.o_sizeof_state.restype = ctypes.c_int
@@ -83,10 +87,12 @@ class Oracle:
     """Runs the reference functions on a fvmhd3d_amd.state.CellState in place."""
 
     def __init__(self, mode="f64", limiter=0, path=None):
-        assert mode in ("f64", "shipped")
+        """mode: "f64" (all-FP64), "shipped" (float storage) or "f64_omp" (all-FP64
+        built with OpenMP: step_global_par runs on every thread it is given)."""
+        assert mode in ("f64", "shipped", "f64_omp")
         self.mode = mode
         self.L = _lib(mode, path)
-        self.sdtype = np.float64 if mode == "f64" else np.float32
+        self.sdtype = np.float32 if mode == "shipped" else np.float64
         self.limiter = limiter
         self.diag = dict(positivity=0, closure=0, ethm_failed=0, mass_failed=0, dsl=0)
 
@@ -179,6 +185,20 @@ class Oracle:
     def phi(self, mode, x):
         """The limiter function Phi of ``mode`` at x."""
         return self.L.o_limiter_phi(int(mode), float(x))
+
+    def step_global_par(self, st, dtmax, nsteps=1):
+        """nsteps of the global-timestep driver with the threaded passes (bit-identical
+        to step_global); returns the list of dt."""
+        o, keep = self._pack(st)
+        dts = []
+        for _ in range(nsteps):
+            dts.append(self.L.o_step_global_par(ctypes.byref(o), float(dtmax)))
+        st.t_global = o.t_global
+        self._unpack(st, o, keep)
+        return dts
+
+    def threads(self):
+        return self.L.o_threads()
 
     def rung_dt(self, dtmax, min_rung, dt):
         return self.L.o_rung_dt(dtmax, min_rung, dt)

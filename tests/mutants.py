@@ -16,7 +16,7 @@ import tempfile
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "fvm_oracle.c")
 CFLAGS = ["-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-fvisibility=hidden",
-          "-Wl,-Bsymbolic", "-DSREAL=double"]
+          "-Wno-unknown-pragmas", "-Wl,-Bsymbolic", "-DSREAL=double"]
 
 _TMP = tempfile.mkdtemp(prefix="oracle_mutants_")
 _built = {}

@@ -282,5 +282,25 @@ FU_MUTANTS = {
 
 @pytest.mark.parametrize("tag", sorted(FU_MUTANTS))
 def test_extrapolation_pin_catches_mutant(tag):
-    path = build_mutant(tag, "o_compute_fluid_update", FU_MUTANTS[tag])
+    path = build_mutant(tag, "fluid_face", FU_MUTANTS[tag])
     assert fails(check_extrapolation, path), tag
+
+
+# ------------------------------------------- substep visiting order ------
+def test_limiter_depends_on_the_active_order():
+    """The reference's active list is the Scheduler's (Scheduler.h
+    pull_active_list: finest rung bucket first, each bucket in push order);
+    this build's substep layouts list active cells in ascending mesh order.
+    The sequential sweep (computeTimestep.cpp:64-110) is order dependent, so
+    the two orders can give different dt_new: on the chain 0-1-2 with
+    d = (100, 100, 1), ascending order reaches cell 0 with 100 (one backward
+    hop), while the finest-first order (2, 1, 0) carries the small step to
+    cell 0 as 4."""
+    d = [100.0, 100.0, 1.0]
+    st = graph_state(3, 3, [(0, 1), (1, 2)], d)
+    Oracle("f64").compute_timestep_limiter(st)
+    assert np.array_equal(st.dt_new, [100.0, 2.0, 1.0])
+    # the same cells in the finest-first order: local 0 = cell 2, 1 = cell 1, 2 = cell 0
+    st = graph_state(3, 3, [(0, 1), (1, 2)], d[::-1])
+    Oracle("f64").compute_timestep_limiter(st)
+    assert np.array_equal(st.dt_new[::-1], [4.0, 2.0, 1.0])
