@@ -47,9 +47,7 @@ def build_workload(name):
     from fvmhd3d_amd.state import global_step_state
     gen, n, ic, lim, _ = WORKLOADS[name]
     t = time.perf_counter()
-    pos = M.uniform_random(n ** 3, seed=123) if gen == "random" else M.jittered_lattice(n, seed=123)
-    pos = pos[M.morton_order(pos, (1.0, 1.0, 1.0))]
-    m = M.build_periodic(pos)
+    m = M.workload_mesh(gen, n, seed=123)
     W = {"blast": M.ic_blast, "ot": M.ic_orszag_tang, "alfven": M.ic_alfven}[ic](m.pos)
     st = global_step_state(m, W, dt=0.0)
     log(f"[bench] mesh {name}: {m.ncell} cells, {m.nface} faces, built in {time.perf_counter() - t:.1f} s")
@@ -230,37 +228,65 @@ def substep_bench(nsub, n=2_097_152, dtmax=1.0 / 128, host_driver=True, cpu=True
     return out
 
 
-def surrogate_bench(iters, n=128, nsteps=16, max_faces=4_000_000):
-    """Surrogate + FP64 fallback against the FP64 and FP32 HLLD on the computeFlux
-    inputs of the 128^3 MHD blast after 16 steps (held out from training), with
-    the trained parameters fvmhd3d_amd/data/hlld_surrogate.npz."""
+def surrogate_bench(workload, nsteps=0):
+    """The learned HLLD surrogate in the step against the plain FP64 path.
+
+    From the same initial state: the plain FP64 path (the reference's
+    arithmetic, FP64 HLLD), the FP32 face path (FP32 HLLD from both sides of
+    every face) and the FP32 face path with the surrogate in its flux step
+    (every face once, network where the gate accepts it, FP32 HLLD where it
+    does not), each for the configuration's step count.  Reports face-flux
+    updates/s of each (wall time of the whole run, synchronised), the relative
+    L1 of density and total energy of the surrogate run against the FP64 run,
+    the fraction of faces the gate sent to the HLLD, and the per-kernel times of
+    the surrogate step.  Parameters: fvmhd3d_amd/data/hlld_surrogate.npz."""
     from fvmhd3d_amd import gpu as G
-    from fvmhd3d_amd import mesh as M
-    from fvmhd3d_amd import surrogate as S
-    pos = M.uniform_random(n ** 3, seed=123)
-    m = M.build_periodic(pos[M.morton_order(pos, (1.0, 1.0, 1.0))])
-    W, t = S.evolve(m, M.ic_blast(m.pos), G.LIMITER_SMOOTH, nsteps)
-    faces = S.face_states(m, W, G.LIMITER_SMOOTH, t_global=t, count=max_faces)
-    nf = len(faces[1])
-    sg = S.Surrogate.load()
-    ctx = G.Context()
-    _, _, F = ctx.compute_flux(*faces)
-    ctx.close()
-    _, _, F2, nfb = sg.compute_flux(*faces)
-    ans = np.any(F2 != F, axis=1)
-    l1 = float(np.sum(np.abs(F2[:, :8] - F[:, :8])) / np.sum(np.abs(F[:, :8])))
-    per = (np.sum(np.abs(F2[:, :8] - F[:, :8]), 0) / np.sum(np.abs(F[:, :8]), 0)).tolist()
-    ms, _ = sg.time_flux(*faces, iters=max(iters, 1))
+    from fvmhd3d_amd.surrogate import Surrogate
+    m, st, lim = build_workload(workload)
+    nsteps = nsteps or WORKLOADS[workload][4]
+    sg = Surrogate.load()
+
+    def run(precision, use_sg):
+        c = G.Context(gamma=st.gamma, courant=st.courant, box=st.box, limiter=lim)
+        c.upload_state(st)
+        c.fused_setup(precision)
+        if use_sg:
+            c.fused_set_surrogate(sg)
+        c.reset_diagnostics()
+        c.fused_timing(True)
+        c.L.fvm_synchronize(c.h)
+        t = time.perf_counter()
+        c.fused_run(nsteps, 1.0)
+        c.L.fvm_synchronize(c.h)
+        wall = time.perf_counter() - t
+        ms, nl = c.fused_kernel_ms()
+        out = dict(W=c.download("w"), U=c.download("U"), dts=c.fused_dt_history(nsteps), diag=c.diagnostics(),
+                   ms=wall * 1e3 / nsteps, kms=[round(ms[k] / max(nl[k], 1), 4) for k in range(4)])
+        c.close()
+        log(f"[surrogate] {'FP64' if precision == G.FUSED_FP64 else 'FP32'}{' + surrogate' if use_sg else ''}: "
+            f"{out['ms']:.2f} ms/step, kernels {out['kms']}, diag {out['diag']}")
+        return out
+
+    r64 = run(G.FUSED_FP64, False)
+    r32 = run(G.FUSED_FP32, False)
+    rsg = run(G.FUSED_FP32, True)
     sg.close()
-    bytes_face = (3 + 10 + 10 + 3) * 8 + 4 + 12 * 8
-    return {"metric": "computeFlux faces/s: learned surrogate + FP64 fallback vs FP64 HLLD", "unit": "faces/s",
-            "value": nf / (ms[0] * 1e-3), "faces": nf, "fallback_fraction": nfb / nf,
-            "answered_by_network": int(ans.sum()), "l1_rel_vs_fp64": l1, "l1_rel_per_component": per,
-            "ms": {"surrogate_plus_fallback": ms[0], "hlld_fp64": ms[1], "hlld_fp32": ms[2]},
-            "faces_per_s": {"hlld_fp64": nf / (ms[1] * 1e-3), "hlld_fp32": nf / (ms[2] * 1e-3)},
-            "speedup_vs_fp64": ms[1] / ms[0],
-            "hbm_GBps_fp64_hlld": nf * bytes_face / (ms[1] * 1e-3) / 1e9,
-            "workload": f"mhd_blast_{n}^3 after {nsteps} steps, first {nf} active faces"}
+    l1 = lambda x, y: float(np.sum(np.abs(x - y)) / max(np.sum(np.abs(y)), 1e-300))
+    fps = lambda r: m.nface / (r["ms"] * 1e-3)
+    out = {"metric": "Voronoi face-flux updates/s (1 GPU): learned HLLD surrogate + HLLD fallback in the step "
+                     "vs the plain FP64 path", "unit": "face-flux updates/s", "value": fps(rsg),
+           "workload": workload, "steps": nsteps, "cells": m.ncell, "faces": m.nface,
+           "ms_per_step": {"surrogate_fp32": rsg["ms"], "plain_fp64": r64["ms"], "fp32_hlld": r32["ms"]},
+           "faces_per_s": {"surrogate_fp32": fps(rsg), "plain_fp64": fps(r64), "fp32_hlld": fps(r32)},
+           "speedup_vs_plain_fp64": r64["ms"] / rsg["ms"],
+           "l1_rel_vs_fp64": {"rho": l1(rsg["W"][:, 0], r64["W"][:, 0]), "E": l1(rsg["U"][:, 4], r64["U"][:, 4])},
+           "l1_rel_fp32_hlld_vs_fp64": {"rho": l1(r32["W"][:, 0], r64["W"][:, 0]), "E": l1(r32["U"][:, 4], r64["U"][:, 4])},
+           "fallback_fraction": rsg["diag"]["sg_fallback"] / float(m.nface * nsteps),
+           "same_dt_every_step": bool(np.array_equal(rsg["dts"], r64["dts"])),
+           "surrogate_step_kernel_ms": {"cfl_predict_recon": rsg["kms"][0], "limit": rsg["kms"][1],
+                                        "faces_plus_cells": rsg["kms"][2]},
+           "diagnostics": {"surrogate": rsg["diag"], "fp64": r64["diag"]}}
+    return out
 
 
 def verify_fp32_vs_fp64(fresh, nsteps):
@@ -310,7 +336,8 @@ def main():
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--copy-only", action="store_true", help="only the stream-copy probe (counter calibration)")
     ap.add_argument("--surrogate", action="store_true",
-                    help="the learned HLLD surrogate against the FP64/FP32 HLLD on harvested blast faces")
+                    help="the learned HLLD surrogate in the step against the plain FP64 path, the "
+                         "configuration's step count (--verify-steps overrides it)")
     ap.add_argument("--substeps", type=int, default=0,
                     help="individual timesteps: this many substeps of the 2M-generator disk (6 rungs)")
     ap.add_argument("--no-host-driver", action="store_true",
@@ -323,7 +350,7 @@ def main():
 
     from fvmhd3d_amd import gpu as G
     if args.surrogate:
-        print(json.dumps(surrogate_bench(args.steps)), flush=True)
+        print(json.dumps(surrogate_bench(args.workload, args.verify_steps)), flush=True)
         return
     if args.substeps:
         print(json.dumps(substep_bench(args.substeps, host_driver=not args.no_host_driver,

@@ -11,8 +11,10 @@
 void fvm_fast_release(fvm_ctx *c) {
   FusedState &z = c->fz;
   for (DBuf *b : {&z.perm, &z.inv, &z.deg, &z.gbase, &z.fe, &z.fdr, &z.lmin, &z.U, &z.W64, &z.cvol, &z.mvel, &z.rec[0], &z.rec[1], &z.mm,
-                  &z.dt_new, &z.dtmin, &z.tdev, &z.hist})
+                  &z.dt_new, &z.dtmin, &z.tdev, &z.hist, &z.sg_faceL, &z.sg_face, &z.sg_fid, &z.sg_flux})
     fvm_release(*b);
+  z.sg = nullptr;
+  z.sg_nface = 0;
   for (auto &e : z.ev)
     if (e) (void)hipEventDestroy(e);
   z.ev.clear();
@@ -84,6 +86,31 @@ int fvm_fused_sync(fvm_ctx *c) {
   return c->fz.precision == FVM_FUSED_FP64 ? fvm_fused_sync_f64(c) : fvm_fused_sync_f32(c);
 }
 
+int fvm_fused_set_surrogate(fvm_ctx *c, fvm_sg *sg) {
+  if (!c) return FVM_EARG;
+  if (!sg) {
+    c->fz.sg = nullptr;
+    return FVM_OK;
+  }
+  if (!c->fz.ready || !c->have_mesh || c->np != c->fz.np)
+    return fvm_fail(c, FVM_ESTATE, "fvm_fused_set_surrogate without a fvm_fused_setup on the current mesh");
+  if (c->fz.precision != FVM_FUSED_FP32) return fvm_fail(c, FVM_EARG, "the surrogate step runs on the FP32 face path");
+  if (c->fz.moving) return fvm_fail(c, FVM_EARG, "the surrogate step takes static meshes (face speed 0)");
+  if (fvm_sg_device_of(sg) != c->device) return fvm_fail(c, FVM_EARG, "the surrogate lives on another device");
+  (void)hipSetDevice(c->device);
+  if (c->fz.sg_nface == 0) {
+    int rc;
+    try {
+      rc = fvm_fused_sg_setup_f32(c);
+    } catch (...) {
+      return fvm_fail(c, FVM_EARG, "host allocation failed");
+    }
+    if (rc) return rc;
+  }
+  c->fz.sg = sg;
+  return FVM_OK;
+}
+
 int fvm_fused_timing(fvm_ctx *c, int enable) {
   if (!c) return FVM_EARG;
   c->fz.timing = enable != 0;
@@ -108,10 +135,21 @@ int fvm_fused_kernel_ms(fvm_ctx *c, double *ms, long long *launches, int n) {
 // data-sheet peak; also the calibration kernel for counter-based traffic)
 // ===========================================================================
 namespace fvm {
-__global__ void __launch_bounds__(256) k_stream_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+typedef float f4v_t __attribute__((ext_vector_type(4)));
+// each thread moves UNR 16-byte words per iteration, all loads issued before
+// the stores; non-temporal, so the stream does not linger in L2
+constexpr int COPY_UNR = 4;
+__global__ void __launch_bounds__(256) k_stream_copy(const f4v_t *__restrict__ src, f4v_t *__restrict__ dst, size_t n) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) dst[i] = src[i];
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + (COPY_UNR - 1) * stride < n; i += COPY_UNR * stride) {
+    f4v_t v[COPY_UNR];
+#pragma unroll
+    for (int u = 0; u < COPY_UNR; u++) v[u] = __builtin_nontemporal_load(src + i + u * stride);
+#pragma unroll
+    for (int u = 0; u < COPY_UNR; u++) __builtin_nontemporal_store(v[u], dst + i + u * stride);
+  }
+  for (; i < n; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 }  // namespace fvm
 
@@ -130,10 +168,10 @@ int fvm_measure_copy_bandwidth(fvm_ctx *c, long long bytes, int iters, double *g
   (void)hipEventCreate(&e1);
   (void)hipMemsetAsync(a, 0, n * 16, c->stream);
   const unsigned grid = 256 * 8 * 4;
-  hipLaunchKernelGGL(fvm::k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const float4 *)a, (float4 *)b, n);
+  hipLaunchKernelGGL(fvm::k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const fvm::f4v_t *)a, (fvm::f4v_t *)b, n);
   (void)hipEventRecord(e0, c->stream);
   for (int k = 0; k < iters; k++)
-    hipLaunchKernelGGL(fvm::k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const float4 *)a, (float4 *)b, n);
+    hipLaunchKernelGGL(fvm::k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const fvm::f4v_t *)a, (fvm::f4v_t *)b, n);
   (void)hipEventRecord(e1, c->stream);
   hipError_t err = hipEventSynchronize(e1);
   float ms = 0.0f;

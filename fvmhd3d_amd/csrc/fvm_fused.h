@@ -247,6 +247,15 @@ __device__ __forceinline__ void wave_count(unsigned long long *ctr, bool pred) {
   if (m && __lane_id() == (__ffsll((long long)m) - 1)) atomicAdd(ctr, (unsigned long long)__popcll(m));
 }
 
+// adds the sum over the wavefront's active lanes of v (0 <= v < 2^16), from
+// ballots only, so lanes that have left the kernel contribute nothing
+__device__ __forceinline__ void wave_add(unsigned long long *ctr, int v) {
+  if (__ballot(v != 0) == 0) return;
+  unsigned long long tot = 0;
+  for (int b = 0; b < 16; b++) tot += (unsigned long long)__popcll(__ballot((v >> b) & 1)) << b;
+  if (__lane_id() == __ffsll((long long)__ballot(1)) - 1) atomicAdd(ctr, tot);
+}
+
 __device__ __forceinline__ double ethm_from_entropy_d(double g, const double *W) {
   return W[ENTR] * pow(W[DENS], g) / (g - 1.0);
 }
@@ -696,13 +705,16 @@ __global__ void __launch_bounds__(FB) k_limit(FusedArgs<T> A) {
 // into the own cell (computeFluidUpdate.cpp:209-307)
 template <typename T, bool MOVING>
 __device__ __forceinline__ void face_update(const FusedArgs<T> &A, int c, const FE<T> &fe, const T *Wn, const T *Wo,
-                                            double dtf, double *dU, double *gp, double &dv, bool &bad) {
+                                            double dtf, double *dU, double *gp, double &dv, int &nbad) {
   T WL[NF], WR[NF];
   for (int k = 0; k < NF; k++) {
     WL[k] = fe.isL ? Wo[k] : Wn[k];
     WR[k] = fe.isL ? Wn[k] : Wo[k];
   }
-  bad = bad || !(WL[DENS] > T(0.0)) || !(WL[ETHM] > T(0.0)) || !(WR[DENS] > T(0.0)) || !(WR[ETHM] > T(0.0));
+  // a face state where the reference asserts (computeFluidUpdate.cpp:179-189),
+  // counted once per face: by its L cell
+  nbad += (fe.isL && (!(WL[DENS] > T(0.0)) || !(WL[ETHM] > T(0.0)) || !(WR[DENS] > T(0.0)) || !(WR[ETHM] > T(0.0))))
+              ? 1 : 0;
   T wij[3], dV;
   face_velocity<T, MOVING>(A, fe.isL ? c : fe.j, fe.isL ? fe.j : c, fe, wij, dV);
   T psi, Bn, flux[NF];
@@ -826,7 +838,7 @@ __global__ void __launch_bounds__(FB) k_flux_apply(FusedArgs<T> A) {
   double dU[NF], gp[3], dv = 0.0;
   for (int k = 0; k < NF; k++) dU[k] = 0.0;
   gp[0] = gp[1] = gp[2] = 0.0;
-  bool bad = false;
+  int nbad = 0;
 
   const int deg = A.deg[c];
   const int e0 = A.gbase[c >> 6] + (c & 63);
@@ -854,9 +866,9 @@ __global__ void __launch_bounds__(FB) k_flux_apply(FusedArgs<T> A) {
       const T *dr = fe.isL ? fe.drL : fe.drR;
       for (int k = 0; k < NF; k++) Wo[k] = Wc[k] + (gc[k] * dr[0] + gc[NF + k] * dr[1] + gc[2 * NF + k] * dr[2] + tc[k] * dti);
     }
-    face_update<T, MOVING>(A, c, fe, Wn, Wo, st.dtf, dU, gp, dv, bad);
+    face_update<T, MOVING>(A, c, fe, Wn, Wo, st.dtf, dU, gp, dv, nbad);
   }
-  wave_count(&A.diag[FVM_DIAG_POSITIVITY], bad);
+  wave_add(&A.diag[FVM_DIAG_POSITIVITY], nbad);
   cell_update<T>(A, c, st.t0, st.dt_glob, dU, gp, dv, tc);
 }
 
@@ -964,7 +976,7 @@ __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
   double dU[NF], gp[3], dv = 0.0;
   for (int k = 0; k < NF; k++) dU[k] = 0.0;
   gp[0] = gp[1] = gp[2] = 0.0;
-  bool bad = false;
+  int nbad = 0;
 
   for (int q = 0, e = e0; q < maxdeg; q++, e += 64) {
     __builtin_amdgcn_s_waitcnt(WAIT_ALL);  // step q's records are in LDS
@@ -996,10 +1008,10 @@ __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
       T Wo[NF];
       const T *dr = fe.isL ? fe.drL : fe.drR;
       for (int k = 0; k < NF; k++) Wo[k] = face_state32(Wc[k], gc[k], gc[NF + k], gc[2 * NF + k], dr);
-      face_update<T, MOVING>(A, c, fe, Wn, Wo, st.dtf, dU, gp, dv, bad);
+      face_update<T, MOVING>(A, c, fe, Wn, Wo, st.dtf, dU, gp, dv, nbad);
     }
   }
-  wave_count(&A.diag[FVM_DIAG_POSITIVITY], bad);
+  wave_add(&A.diag[FVM_DIAG_POSITIVITY], nbad);
   if (live) {
     T tc[NF];  // reloaded rather than held through the face loop
     ldv<NF>(A.rec + (size_t)c * REC + R_T, tc);
@@ -1251,7 +1263,10 @@ template <typename T> int fused_setup_t(fvm_ctx *c) {
   return FVM_OK;
 }
 
-template <typename T, bool MOVING, int LIM> void launch_step(fvm_ctx *c, const FusedArgs<T> &A, hipEvent_t *ev) {
+// the surrogate flux step of the FP32 path (fvm_sg_step.h, FP32 translation unit only)
+int sg_step_launch(fvm_ctx *c, const FusedArgs<float> &A);
+
+template <typename T, bool MOVING, int LIM> int launch_step(fvm_ctx *c, const FusedArgs<T> &A, hipEvent_t *ev) {
   const unsigned grid = fused_grid(c->np);
   if (ev) (void)hipEventRecord(ev[0], c->stream);
   hipLaunchKernelGGL((k_cfl_predict_recon<T, MOVING, LIM>), dim3(grid), dim3(FB), 0, c->stream, A);
@@ -1262,7 +1277,12 @@ template <typename T, bool MOVING, int LIM> void launch_step(fvm_ctx *c, const F
     // FVM_FUSED_ADDR64 set: always the 64-bit addressing (it serves meshes past
     // 2^24 cells; the switch lets small meshes exercise it)
     const bool addr64 = std::getenv("FVM_FUSED_ADDR64") != nullptr;
-    if (!addr64 && (size_t)c->np * REC * sizeof(float) <= (size_t(1) << 32))
+    if (c->fz.sg) {
+      if constexpr (!MOVING) {
+        const int rc = sg_step_launch(c, A);
+        if (rc) return rc;
+      }
+    } else if (!addr64 && (size_t)c->np * REC * sizeof(float) <= (size_t(1) << 32))
       hipLaunchKernelGGL((k_flux_apply_coop<MOVING, true>), dim3(grid), dim3(FB), 0, c->stream, A);
     else
       hipLaunchKernelGGL((k_flux_apply_coop<MOVING, false>), dim3(grid), dim3(FB), 0, c->stream, A);
@@ -1270,6 +1290,7 @@ template <typename T, bool MOVING, int LIM> void launch_step(fvm_ctx *c, const F
   else
     hipLaunchKernelGGL((k_flux_apply<T, MOVING>), dim3(grid), dim3(FB), 0, c->stream, A);
   if (ev) (void)hipEventRecord(ev[3], c->stream);
+  return FVM_OK;
 }
 
 template <typename T, bool MOVING> int fused_run_t(fvm_ctx *c, int nsteps, double dtmax) {
@@ -1285,9 +1306,11 @@ template <typename T, bool MOVING> int fused_run_t(fvm_ctx *c, int nsteps, doubl
     FusedArgs<T> A = fused_args<T>(c, dtmax);
     A.write_w64 = (s == nsteps - 1) ? 1 : 0;
     hipEvent_t *ev = z.timing ? &z.ev[5 * s] : nullptr;
-    if (lim == LIM_MINMOD) launch_step<T, MOVING, LIM_MINMOD>(c, A, ev);
-    else if (lim == LIM_SMOOTH) launch_step<T, MOVING, LIM_SMOOTH>(c, A, ev);
-    else launch_step<T, MOVING, LIM_SHIPPED>(c, A, ev);
+    int rc;
+    if (lim == LIM_MINMOD) rc = launch_step<T, MOVING, LIM_MINMOD>(c, A, ev);
+    else if (lim == LIM_SMOOTH) rc = launch_step<T, MOVING, LIM_SMOOTH>(c, A, ev);
+    else rc = launch_step<T, MOVING, LIM_SHIPPED>(c, A, ev);
+    if (rc) return rc;
     FVM_HIPCHK(c, hipGetLastError());
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, (double *)z.tdev.p, (unsigned long long *)z.dtmin.p,
                        dtmax, (double *)z.hist.p, s);

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/fvm_gpu.h"
+#include "../../include/fvm_surrogate.h"
 
 namespace fvm {
 
@@ -50,6 +51,11 @@ struct FusedState {
   DBuf lmin;                // (3V/4pi)^(1/3) per cell (the volume is fixed on this path)
   DBuf rec[2], mm, dt_new, dtmin, tdev, hist;
   size_t slots = 0;
+  // the learned HLLD surrogate in the flux step (FP32 path, fvm_sg_step.h):
+  // every face once, L face-end records, face of each slot, flux records
+  fvm_sg *sg = nullptr;
+  int sg_nface = 0;
+  DBuf sg_faceL, sg_face, sg_fid, sg_flux;
   bool timing = false;
   std::vector<hipEvent_t> ev;  // 5 per timed step
   double ms[4] = {0, 0, 0, 0};
@@ -120,6 +126,13 @@ int fvm_fused_run_f32(fvm_ctx *c, int nsteps, double dtmax);
 int fvm_fused_run_f64(fvm_ctx *c, int nsteps, double dtmax);
 int fvm_fused_sync_f32(fvm_ctx *c);
 int fvm_fused_sync_f64(fvm_ctx *c);
+// face list of the surrogate flux step of the FP32 path (fvm_fused_f32.hip)
+int fvm_fused_sg_setup_f32(fvm_ctx *c);
+// the surrogate's packed weights on its device (packed on demand, stream
+// synchronised) and its gate {max |ln rho jump|, max |ln p jump|, max |v_n|/c_s,
+// max log10 error}; the device it lives on (fvm_surrogate.hip)
+int fvm_sg_device_weights(fvm_sg *s, const void **frag, const float **bias, float gate[4]);
+int fvm_sg_device_of(const fvm_sg *s);
 
 #define FVM_HIPCHK(c, call)                                                                \
   do {                                                                                     \

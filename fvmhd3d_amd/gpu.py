@@ -29,7 +29,7 @@ FIELDS = {
 }
 SUBSTEP_RUNG = len(FIELDS)  # rung per cell in fvm_substep_upload / _download
 NDIAG = 8
-DIAG_NAMES = ["positivity", "closure", "ethm_failed", "mass_failed", "dsl"]
+DIAG_NAMES = ["positivity", "closure", "ethm_failed", "mass_failed", "dsl", "sg_fallback"]
 
 
 class FvmParams(ctypes.Structure):
@@ -103,6 +103,7 @@ SIGNATURES = {
     "fvm_sg_gradients": [_P, _I, _P, _P, _D, _P, _P, ctypes.POINTER(ctypes.c_double)],
     "fvm_sg_forward": [_P, _I, _P, _P],
     "fvm_sg_compute_flux": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_int)],
+    "fvm_fused_set_surrogate": [_P, _P],
     "fvm_sg_compute_flux_device": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_int)],
     "fvm_sg_time_flux": [_P, _I, _P, _P, _P, _P, _P, _I, _P, ctypes.POINTER(ctypes.c_int)],
 }
@@ -310,6 +311,12 @@ class Context:
     # device-resident stepping
     def fused_setup(self, precision=FUSED_FP32, moving=False):
         self._chk(self.L.fvm_fused_setup(self.h, int(precision), int(bool(moving))))
+
+    def fused_set_surrogate(self, sg):
+        """The learned HLLD surrogate (a surrogate.Surrogate, or None) in the
+        flux step of the FP32 face path (fvm_fused_set_surrogate)."""
+        self._chk(self.L.fvm_fused_set_surrogate(self.h, sg.h if sg is not None else None))
+        self._sg = sg  # kept alive while set
 
     def fused_run(self, nsteps, dtmax):
         t = ctypes.c_double()

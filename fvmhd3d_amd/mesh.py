@@ -118,6 +118,38 @@ def build_periodic(pos: np.ndarray, box=(1.0, 1.0, 1.0), nthreads: int = 0) -> M
     return Mesh(faces, cell_off, cell_face, volume, centroid, pos, box)
 
 
+_MESH_FIELDS = ("faces", "cell_off", "cell_face", "volume", "centroid", "pos", "box")
+
+
+def workload_mesh(generators: str, n: int, seed: int = 123) -> Mesh:
+    """The periodic unit-cube mesh of a workload: ``generators`` is "lattice" (n^3
+    jittered lattice sites) or "random" (n^3 uniform sites), Rand48-seeded, sorted
+    along the Morton curve.  When the environment variable FVM_MESH_CACHE names a
+    directory, the mesh is stored there as .npy files on first use and loaded from
+    there afterwards (a 256^3 mesh takes minutes to build and ~9 GB to store)."""
+    cache = os.environ.get("FVM_MESH_CACHE")
+    tag = os.path.join(cache, f"{generators}_{n}_{seed}") if cache else None
+    if tag and os.path.exists(os.path.join(tag, "done")):
+        return Mesh(*(np.load(os.path.join(tag, f + ".npy"), allow_pickle=False) for f in _MESH_FIELDS))
+    if generators == "lattice":
+        pos = jittered_lattice(n, seed=seed)
+    elif generators == "random":
+        pos = uniform_random(n ** 3, seed=seed)
+    else:
+        raise ValueError(f"unknown generators {generators!r}")
+    m = build_periodic(pos[morton_order(pos, (1.0, 1.0, 1.0))])
+    if tag:
+        try:
+            os.makedirs(tag, exist_ok=True)
+            for f in _MESH_FIELDS:
+                np.save(os.path.join(tag, f + ".npy"), getattr(m, f), allow_pickle=False)
+            open(os.path.join(tag, "done"), "w").close()
+        except OSError:  # a full disk only loses the cache, not the mesh
+            import shutil
+            shutil.rmtree(tag, ignore_errors=True)
+    return m
+
+
 # ----------------------------------------------------------------------------
 # generator positions
 # ----------------------------------------------------------------------------

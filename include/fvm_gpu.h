@@ -108,6 +108,7 @@ enum fvm_diag {
   FVM_DIAG_ETHM_FAILED = 2, /* entropy fallback, computeFluidUpdate.cpp:462-466 */
   FVM_DIAG_MASS_FAILED = 3, /* failed-mass fallback, computeFluidUpdate.cpp:377-386 */
   FVM_DIAG_DSL = 4,        /* dsl > 0, slopeLimiter.cpp:81, computeFluidUpdate.cpp:143-144 */
+  FVM_DIAG_SG_FALLBACK = 5, /* surrogate flux step: faces the gate sent to the HLLD (no reference counterpart) */
   FVM_NDIAG = 8
 };
 

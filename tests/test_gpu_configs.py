@@ -15,9 +15,12 @@ same bits as its serial step (tests/test_oracle_parallel.py).
 
 Bars: FP64 path <= 1e-12 (the reference's arithmetic, operation for
 operation).  FP32 face path: relative L1 <= 1e-4 for mass and energy (the
-stated bar) and for every other conserved component except the two passive
-scalars psi*M and s*M, whose bar is 1e-3 (psi starts at zero and is created
-by the divergence cleaning, so its L1 norm is small)."""
+stated bar) and for every other conserved component.  Vector components are
+measured against the vector's L1 norm (sum over cells of |momentum| or |B|),
+not their own: B_z is zero in the blast's and the vortex's initial states, so
+its own norm is small and says nothing about the field's accuracy.  The two
+passive scalars psi*M and s*M have the bar 1e-3 against their own norm (psi
+starts at zero and is created by the divergence cleaning)."""
 import numpy as np
 import pytest
 
@@ -34,13 +37,12 @@ L1_BAR = {n: (1e-3 if n in ("psi_mass", "entropy_mass") else 1e-4) for n in NAME
 
 def _workload(kind):
     if kind == "alfven_32":
-        pos, ic, lim, nsteps = M.jittered_lattice(32, seed=123), M.ic_alfven, G.LIMITER_SHIPPED, 1
+        gen, n, ic, lim, nsteps = "lattice", 32, M.ic_alfven, G.LIMITER_SHIPPED, 1
     elif kind == "blast_128":
-        pos, ic, lim, nsteps = M.uniform_random(128 ** 3, seed=123), M.ic_blast, G.LIMITER_SMOOTH, 20
+        gen, n, ic, lim, nsteps = "random", 128, M.ic_blast, G.LIMITER_SMOOTH, 20
     else:
-        pos, ic, lim, nsteps = M.jittered_lattice(256, seed=123), M.ic_orszag_tang, G.LIMITER_SMOOTH, 1
-    pos = pos[M.morton_order(pos, (1, 1, 1))]
-    m = M.build_periodic(pos)
+        gen, n, ic, lim, nsteps = "lattice", 256, M.ic_orszag_tang, G.LIMITER_SMOOTH, 1
+    m = M.workload_mesh(gen, n, seed=123)
     return m, global_step_state(m, ic(m.pos), dt=0.0), lim, nsteps
 
 
@@ -55,6 +57,7 @@ def _gpu(st, lim, precision, nsteps):
     return out
 
 
+@pytest.mark.timeout(900)  # the 256^3 mesh build alone takes a minute or two
 @pytest.mark.parametrize("kind", ["alfven_32", "blast_128", "ot_256"])
 def test_workload_against_oracle(kind):
     m, st, lim, nsteps = _workload(kind)
@@ -73,11 +76,16 @@ def test_workload_against_oracle(kind):
         err = np.max(np.abs(a - b) / scale)
         print(f"FP64 path {name}: max rel err {err:.3e}")
         assert err <= 1e-12, (name, err)
-    assert g64["diag"]["positivity"] == o.diag["positivity"]
+    # face states where the reference asserts (positivity, counted per face) and
+    # the entropy / mass fallbacks (per cell), as the oracle counts them
+    print(f"diagnostics: FP64 path {g64['diag']}, FP32 path {g32['diag']}")
+    for key in ("positivity", "ethm_failed", "mass_failed"):
+        assert g64["diag"][key] == o.diag[key], (key, g64["diag"], o.diag)
     # FP32 face path against the FP64 oracle, every conserved component
     assert np.array_equal(g32["dts"], g64["dts"])
+    vec = {1: slice(1, 4), 2: slice(1, 4), 3: slice(1, 4), 5: slice(5, 8), 6: slice(5, 8), 7: slice(5, 8)}
     for k, name in enumerate(NAMES):
-        den = np.sum(np.abs(ref.U[:, k]))
+        den = np.sum(np.linalg.norm(ref.U[:, vec[k]], axis=1)) if k in vec else np.sum(np.abs(ref.U[:, k]))
         l1 = float(np.sum(np.abs(g32["U"][:, k] - ref.U[:, k])) / den) if den > 0 else 0.0
         print(f"FP32 path {name}: L1 rel {l1:.2e}")
         assert l1 <= L1_BAR[name], (name, l1)
