@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1008,6 +1009,41 @@ __global__ void k_end_sort(const int *soff, int *sent, int n) {
 }  // namespace fvm
 
 // ===========================================================================
+// staging memory of fvm_compute_flux_noalloc
+// ===========================================================================
+// A file-scope device array: every GPU the library runs on holds its own copy
+// (at its own address) until the process ends.  In host code the name
+// `scratch` is only a host-side stand-in; the device address of each GPU's
+// copy comes from hipGetSymbolAddress, called once per device by fvm_create
+// with that device current, and only that address is ever passed to kernels
+// or copies.
+static __device__ __align__(256) unsigned char scratch[FVM_FLUX_NOALLOC_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];  // device address of `scratch` per device, or null
+}  // namespace
+
+// device address of this device's copy of `scratch` (the caller has made the
+// device current); looked up once per device
+static int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return FVM_EARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return FVM_OK;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return FVM_EHIP;
+  g_scratch_ptr[device] = p;
+  return FVM_OK;
+}
+
+static void *noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
+// ===========================================================================
 // context
 // ===========================================================================
 int fvm_fail(fvm_ctx *c, int code, const std::string &msg) {
@@ -1167,6 +1203,10 @@ int fvm_create(fvm_ctx **out, int device, const fvm_params *params) {
     return FVM_EHIP;
   }
   (void)hipMemsetAsync(c->diag.p, 0, c->diag.bytes, c->stream);
+  if (noalloc_scratch_init(device) != FVM_OK) {
+    (void)fvm_destroy(c);
+    return FVM_EHIP;
+  }
   *out = c;
   return FVM_OK;
 }
@@ -1667,18 +1707,11 @@ int fvm_compute_predictor(fvm_ctx *c, double dtmax, int min_rung) {
   return DISPATCH(c, do_predictor, c, dtmax, min_rung);
 }
 
-int fvm_compute_flux(fvm_ctx *c, int n, const double *wij, const double *Wi, const double *Wj, const int *jbnd,
-                     const double *normal, double *psi, double *Bn, double *flux) {
-  if (!c) return FVM_EARG;
-  if (n < 0) return fail(c, FVM_EARG, "n < 0");
-  if (n == 0) return FVM_OK;
-  if (!wij || !Wi || !Wj || !jbnd || !normal || !psi || !Bn || !flux) return fail(c, FVM_EARG, "NULL array");
-  (void)hipSetDevice(c->device);
+// the batched computeFlux through device staging at base (room for n faces:
+// FVM_FLUX_BYTES_PER_FACE each): inputs in, one launch, outputs out, synchronous
+static int flux_batch_staged(fvm_ctx *c, char *base, int n, const double *wij, const double *Wi, const double *Wj,
+                             const int *jbnd, const double *normal, double *psi, double *Bn, double *flux) {
   const size_t N = (size_t)n;
-  const size_t bytes = sizeof(double) * (3 + 10 + 10 + 3 + 1 + 1 + 10) * N + sizeof(int) * N;
-  int rc = ensure(c, c->stage, bytes);
-  if (rc) return rc;
-  char *base = (char *)c->stage.p;
   double *d_wij = (double *)base, *d_Wi = d_wij + 3 * N, *d_Wj = d_Wi + 10 * N, *d_n = d_Wj + 10 * N, *d_psi = d_n + 3 * N,
          *d_Bn = d_psi + N, *d_flux = d_Bn + N;
   int *d_jb = (int *)(d_flux + 10 * N);
@@ -1693,6 +1726,33 @@ int fvm_compute_flux(fvm_ctx *c, int n, const double *wij, const double *Wi, con
   HIPCHK(c, hipMemcpyAsync(flux, d_flux, sizeof(double) * 10 * N, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return FVM_OK;
+}
+
+int fvm_compute_flux(fvm_ctx *c, int n, const double *wij, const double *Wi, const double *Wj, const int *jbnd,
+                     const double *normal, double *psi, double *Bn, double *flux) {
+  if (!c) return FVM_EARG;
+  if (n < 0) return fail(c, FVM_EARG, "n < 0");
+  if (n == 0) return FVM_OK;
+  if (!wij || !Wi || !Wj || !jbnd || !normal || !psi || !Bn || !flux) return fail(c, FVM_EARG, "NULL array");
+  (void)hipSetDevice(c->device);
+  int rc = ensure(c, c->stage, (size_t)FVM_FLUX_BYTES_PER_FACE * (size_t)n);
+  if (rc) return rc;
+  return flux_batch_staged(c, (char *)c->stage.p, n, wij, Wi, Wj, jbnd, normal, psi, Bn, flux);
+}
+
+int fvm_compute_flux_noalloc(fvm_ctx *c, int n, const double *wij, const double *Wi, const double *Wj,
+                             const int *jbnd, const double *normal, double *psi, double *Bn, double *flux) {
+  if (!c) return FVM_EARG;
+  if (n < 0) return fail(c, FVM_EARG, "n < 0");
+  if (n > FVM_FLUX_NOALLOC_MAX)
+    return fail(c, FVM_EARG, "fvm_compute_flux_noalloc: n = " + std::to_string(n) + " exceeds FVM_FLUX_NOALLOC_MAX = " +
+                                 std::to_string(FVM_FLUX_NOALLOC_MAX) + " faces");
+  if (n == 0) return FVM_OK;
+  if (!wij || !Wi || !Wj || !jbnd || !normal || !psi || !Bn || !flux) return fail(c, FVM_EARG, "NULL array");
+  (void)hipSetDevice(c->device);
+  char *base = (char *)noalloc_scratch(c->device);
+  if (!base) return fail(c, FVM_ESTATE, "fvm_compute_flux_noalloc: no staging address for this device");
+  return flux_batch_staged(c, base, n, wij, Wi, Wj, jbnd, normal, psi, Bn, flux);
 }
 
 int fvm_get_diagnostics(fvm_ctx *c, long long *out, int n) {

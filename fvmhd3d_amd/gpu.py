@@ -29,6 +29,7 @@ FIELDS = {
 }
 SUBSTEP_RUNG = len(FIELDS)  # rung per cell in fvm_substep_upload / _download
 NDIAG = 8
+FLUX_NOALLOC_MAX = 65536  # FVM_FLUX_NOALLOC_MAX of include/fvm_gpu.h
 DIAG_NAMES = ["positivity", "closure", "ethm_failed", "mass_failed", "dsl", "sg_fallback"]
 
 
@@ -67,6 +68,7 @@ SIGNATURES = {
     "fvm_compute_timestep_limiter": [_P],
     "fvm_compute_predictor": [_P, _D, _I],
     "fvm_compute_flux": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P],
+    "fvm_compute_flux_noalloc": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P],
     "fvm_get_diagnostics": [_P, ctypes.POINTER(ctypes.c_longlong), _I],
     "fvm_reset_diagnostics": [_P],
     "fvm_synchronize": [_P],
@@ -263,7 +265,9 @@ class Context:
     def compute_predictor(self, dtmax, min_rung=0):
         self._chk(self.L.fvm_compute_predictor(self.h, float(dtmax), int(min_rung)))
 
-    def compute_flux(self, wij, Wi, Wj, jbnd, normal):
+    def compute_flux(self, wij, Wi, Wj, jbnd, normal, noalloc=False):
+        """fvm_compute_flux (noalloc: fvm_compute_flux_noalloc, staged through the
+        library's fixed per-GPU array, at most FLUX_NOALLOC_MAX faces)."""
         n = len(Wi)
         wij = np.ascontiguousarray(wij, dtype=np.float64)
         Wi = np.ascontiguousarray(Wi, dtype=np.float64)
@@ -273,8 +277,8 @@ class Context:
         psi = np.empty(n)
         Bn = np.empty(n)
         flux = np.empty((n, 10))
-        self._chk(self.L.fvm_compute_flux(self.h, n, _ptr(wij), _ptr(Wi), _ptr(Wj), _ptr(jbnd), _ptr(normal),
-                                          _ptr(psi), _ptr(Bn), _ptr(flux)))
+        fn = self.L.fvm_compute_flux_noalloc if noalloc else self.L.fvm_compute_flux
+        self._chk(fn(self.h, n, _ptr(wij), _ptr(Wi), _ptr(Wj), _ptr(jbnd), _ptr(normal), _ptr(psi), _ptr(Bn), _ptr(flux)))
         return psi, Bn, flux
 
     def face_states(self, first, count):

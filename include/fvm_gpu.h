@@ -168,6 +168,19 @@ int fvm_compute_predictor(fvm_ctx *ctx, double dtmax, int min_rung);
  * arrays: wij[n][3], Wi/Wj[n][10], jbnd[n], normal[n][3] -> psi[n], Bn[n], flux[n][10] */
 int fvm_compute_flux(fvm_ctx *ctx, int n, const double *wij, const double *Wi, const double *Wj,
                      const int *jbnd, const double *normal, double *psi, double *Bn, double *flux);
+/* fvm_compute_flux without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as fvm_compute_flux; the device staging is a
+ * fixed array of FVM_FLUX_NOALLOC_BYTES in the library, one per GPU, held
+ * until the process ends.  n > FVM_FLUX_NOALLOC_MAX is refused with FVM_EARG.
+ * Calls that overlap on one GPU (from any context on that device) share that
+ * array and race: serialise them. */
+#define FVM_FLUX_BYTES_PER_FACE 308 /* wij 3, Wi 10, Wj 10, normal 3, psi, Bn, flux 10 doubles + jbnd int */
+#define FVM_FLUX_NOALLOC_MAX 65536
+#define FVM_FLUX_NOALLOC_BYTES (FVM_FLUX_BYTES_PER_FACE * FVM_FLUX_NOALLOC_MAX)
+int fvm_compute_flux_noalloc(fvm_ctx *ctx, int n, const double *wij, const double *Wi, const double *Wj,
+                             const int *jbnd, const double *normal, double *psi, double *Bn, double *flux);
 /* The arguments computeFluidUpdate passes to computeFlux (computeFluidUpdate.cpp:99-211:
  * boundary swap, space-time extrapolated face states, face velocity) for active
  * faces [first, first + count) of the active list, in fvm_compute_flux's layout.
