@@ -74,15 +74,20 @@ struct SubstepState {
   DBuf rung;                                  // int per cell
   DBuf act, apos, fmark, fpos, fl, cmark, iflag, ipos, g2l, perm, len, hist, err;
   DBuf sdti, hot, hpos, hlist, chg;  // timestep-limiter sweep: local dti, hot cells, changes
-  DBuf dtig, actg;                   // the same in mesh order (large layouts)
+  DBuf dtig, rankg;                  // the same in mesh order (large layouts)
+  // the Scheduler's order of the active list (Scheduler.h): a push stamp per
+  // cell, the next stamp, and the sort of one substep's active cells
+  DBuf seq, okey, okey2, oval2, otmp;
+  size_t otmp_bytes = 0;
+  unsigned long long next_seq = 0;
   // host side of the sweep: the global adjacency (neighbour per face-end),
-  // pinned mirrors of dti and the active flags in mesh order (acth is all
-  // zero between substeps), and scratch
+  // pinned mirrors of dti and of each active cell's place in the active list
+  // in mesh order (-1 elsewhere between substeps), and scratch
   std::vector<int> hoff, hnbr;
   std::vector<unsigned char> changed;
   std::vector<unsigned long long> pending;  // one bit per cell
   double *dtih = nullptr;       // pinned, n
-  unsigned char *acth = nullptr;  // pinned, n
+  int *rankh = nullptr;         // pinned, n
   double *sdh = nullptr;        // pinned, n: the sweep's values per layout cell
   int *permh = nullptr;         // pinned, n: the layout's cells
   int *hoth = nullptr;          // pinned, n: the hot cells

@@ -55,11 +55,24 @@ constexpr double J_EPS = 1e-6;
 __device__ __forceinline__ float tlog(float x) { return __logf(x); }
 __device__ __forceinline__ double tlog(double x) { return log(x); }
 
-// Scales, boost and the 16 features (15: the size J of the normalised jump) of
-// a face, and the central flux Fc of its normalised moving-frame states, in
-// precision TC (float for inference: the features are rounded to bf16 anyway;
-// double for training targets, which divide by J); returns whether the face is
-// inside the domain part of the gate.  sc = {rs, cs, ut1, ut2} for the inverse map.
+// Scales, boost and the 16 features of a face, and the base flux Fb of its
+// normalised moving-frame states, in precision TC (float for inference: the
+// features are rounded to bf16 anyway; double for training targets, which
+// divide by J); returns whether the face is inside the domain part of the gate.
+// sc = {rs, cs, ut1, ut2} for the inverse map.
+//
+// Features: the mean state of the two sides (v_n, p, B_n, B_t1, B_t2; the
+// scaling fixes the mean density to 1 and the boost the mean tangential
+// velocity to 0), the unit direction d/J of the jump d = R - L over (rho, v_n,
+// v_t1, v_t2, p, B_t1, B_t2) and its size J, the log density and pressure
+// ratios, and the larger fast speed.  A smooth-flow jump is a small difference
+// of O(1) values that bf16 cannot hold; its direction and size can.
+//
+// Base flux: the HLL flux of the two normalised states at the face (Harten,
+// Lax & van Leer 1983) with the wave speeds riemannSolver_HLLD starts from
+// (computeFlux.cpp:218-229): S_L = min(v_nL, v_nR) - c_f,max, S_R = max(...) +
+// c_f,max.  The network answers (G' - Fb)/J, what the HLLD's intermediate
+// states add per unit jump.
 template <typename TC, typename RS>
 __device__ __forceinline__ bool features(double gamma64, const FrameT<RS> &s, int jbnd, const Gate &G, float *x, TC *Fc,
                                          TC &J, TC *sc) {
@@ -79,33 +92,47 @@ __device__ __forceinline__ bool features(double gamma64, const FrameT<RS> &s, in
   const TC ic = frcp(cs), iP = frcp(Ps), iB = frcp(Bs), irs = frcp(rs);
   const TC vnL = (vL[0] - wn) * ic, vnR = (vR[0] - wn) * ic;
   const TC eL = gamma64 > 1.0 ? (TC)s.eL * iP : pL * iP, eR = gamma64 > 1.0 ? (TC)s.eR * iP : pR * iP;
+  // normalised sides: rho, v_n, v_t1, v_t2, p, e (thermal energy), B_n, B_t1, B_t2
   const TC L[9] = {rL * irs, vnL, (vL[1] - ut1) * ic, (vL[2] - ut2) * ic, pL * iP, eL, Bn * iB, BL[1] * iB, BL[2] * iB};
   const TC R[9] = {rR * irs, vnR, (vR[1] - ut1) * ic, (vR[2] - ut2) * ic, pR * iP, eR, Bn * iB, BR[1] * iB, BR[2] * iB};
   TC FL[7], FR[7];
   physical_flux<TC>(L[0], L[1], L[2], L[3], L[4], L[5], L[6], L[7], L[8], FL);
   physical_flux<TC>(R[0], R[1], R[2], R[3], R[4], R[5], R[6], R[7], R[8], FR);
-  for (int k = 0; k < 7; k++) Fc[k] = h * (FL[k] + FR[k]);
+  // HLL at the face (at rest in the moving frame)
+  const TC b2L = L[6] * L[6] + L[7] * L[7] + L[8] * L[8], b2R = R[6] * R[6] + R[7] * R[7] + R[8] * R[8];
+  const TC cfm = dsqrt(smax(fdiv(gamma * L[4] + b2L, L[0]), fdiv(gamma * R[4] + b2R, R[0])));
+  const TC SL = smin(L[1], R[1]) - cfm, SR = smax(L[1], R[1]) + cfm;
+  if (SL >= TC(0)) {
+    for (int k = 0; k < 7; k++) Fc[k] = FL[k];
+  } else if (SR <= TC(0)) {
+    for (int k = 0; k < 7; k++) Fc[k] = FR[k];
+  } else {
+    const TC EL = L[5] + h * L[0] * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]) + h * b2L;
+    const TC ER = R[5] + h * R[0] * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3]) + h * b2R;
+    const TC UL[7] = {L[0], L[0] * L[1], L[0] * L[2], L[0] * L[3], EL, L[7], L[8]};
+    const TC UR[7] = {R[0], R[0] * R[1], R[0] * R[2], R[0] * R[3], ER, R[7], R[8]};
+    const TC idS = frcp(SR - SL), SLR = SL * SR;
+    for (int k = 0; k < 7; k++) Fc[k] = (SR * FL[k] - SL * FR[k] + SLR * (UR[k] - UL[k])) * idS;
+  }
   const int jk[7] = {0, 1, 2, 3, 4, 7, 8};
-  TC j2 = TC(0);
-  for (int k = 0; k < 7; k++) j2 += sqr(R[jk[k]] - L[jk[k]]);
+  TC d[7], j2 = TC(0);
+  for (int k = 0; k < 7; k++) {
+    d[k] = R[jk[k]] - L[jk[k]];
+    j2 += sqr(d[k]);
+  }
   J = dsqrt(j2) + (TC)J_EPS;
-  x[0] = (float)(L[0] - TC(1));
-  x[1] = (float)L[4];
-  x[2] = (float)R[4];
-  x[3] = (float)vnL;
-  x[4] = (float)vnR;
-  x[5] = (float)L[2];
-  x[6] = (float)L[3];
-  x[7] = (float)L[6];
-  x[8] = (float)L[7];
-  x[9] = (float)R[7];
-  x[10] = (float)L[8];
-  x[11] = (float)R[8];
+  const TC iJ = frcp(J);
+  x[0] = (float)(h * (L[1] + R[1]));
+  x[1] = (float)(h * (L[4] + R[4]));
+  x[2] = (float)L[6];
+  x[3] = (float)(h * (L[7] + R[7]));
+  x[4] = (float)(h * (L[8] + R[8]));
+  for (int k = 0; k < 7; k++) x[5 + k] = (float)(d[k] * iJ);
+  x[12] = (float)J;
   const TC lr = tlog(fdiv(rR, rL)), lp = tlog(fdiv(pR, pL));
-  x[12] = (float)lr;
-  x[13] = (float)lp;
-  x[14] = (float)(vnR - vnL);
-  x[15] = (float)J;
+  x[13] = (float)lr;
+  x[14] = (float)lp;
+  x[15] = (float)cfm;
   const bool pos = s.rL > 0.0 && s.rR > 0.0 && s.pL > 0.0 && s.pR > 0.0;
   bool ok = jbnd == NO_BOUNDARY && pos;
   ok = ok && dabs(lr) <= (TC)G.max_log_rho && dabs(lp) <= (TC)G.max_log_p;

@@ -20,6 +20,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "../../include/fvm_gpu.h"
 #include "fvm_internal.h"
 
@@ -357,60 +359,84 @@ __global__ void k_sub_apply_changes(const int *__restrict__ gidx_, const double 
 
 // the sweep's initial values and active flags in mesh order (the flags of
 // cells outside the layout were cleared before)
+// the sweep's values and each active cell's place in the visiting order, in mesh order
 __global__ void k_sub_sweep_global(const double *__restrict__ sdti, const int *__restrict__ perm, int npl, int nA,
-                                   double *__restrict__ dtig, unsigned char *__restrict__ actg) {
+                                   double *__restrict__ dtig, int *__restrict__ rankg) {
   const int l = gidx();
   if (l >= npl) return;
   const int g = perm[l];
   dtig[g] = sdti[l];
-  actg[g] = l < nA;
+  rankg[g] = l < nA ? l : -1;
 }
 
-__global__ void k_sub_fill_u8(unsigned char *__restrict__ p, long long n) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+// sort keys of the active list: finest rung first, then push order
+// (Scheduler.h pull_active_list); the rungs are below 32 and the stamps below 2^58
+__global__ void k_sub_order_keys(const int *__restrict__ perm, int nA, const int *__restrict__ rung,
+                                 const unsigned long long *__restrict__ seq, unsigned long long *__restrict__ key) {
+  const int k = gidx();
+  if (k >= nA) return;
+  const int g = perm[k];
+  key[k] = ((unsigned long long)(kRungMax - 1 - rung[g]) << 58) | seq[g];
+}
+
+// position of each active cell in the active list
+__global__ void k_sub_apos(const int *__restrict__ perm, int nA, int *__restrict__ apos) {
+  const int k = gidx();
+  if (k >= nA) return;
+  apos[perm[k]] = k;
+}
+
+// the active cells are pushed again, in the order of the active list
+// (Iterate.cpp:146-168): their push stamps follow the last one given
+__global__ void k_sub_stamp(const int *__restrict__ perm, int nA, unsigned long long base,
+                            unsigned long long *__restrict__ seq) {
+  const int k = gidx();
+  if (k >= nA) return;
+  seq[perm[k]] = base + (unsigned long long)k;
+}
+
+__global__ void k_sub_iota64(unsigned long long *__restrict__ p, int n) {
+  const int i = gidx();
+  if (i < n) p[i] = (unsigned long long)i;
 }
 
 // hot cells as mesh cells
-__global__ void k_sub_map(int *__restrict__ list, int n, const int *__restrict__ perm) {
-  const int k = gidx();
-  if (k >= n) return;
-  list[k] = perm[list[k]];
-}
-
 // The sequential sweep of computeTimestep.cpp:64-110 over a whole mesh:
-// active cells in ascending index, each cell's neighbours in list order,
+// active cells in the order of the active list, each cell's neighbours in
+// list order,
 //   dti[j] = min(dti[j], 2 dti[i]);  dti[i] = min(dti[i], 2 dti[j]).
 // A visit changes something only if one of the visited cell's faces exceeds
-// ratio 2 at the values of that moment.  So the sweep visits, in ascending
-// order, the cells with such a face at the initial values ("hot"), every
-// active cell after the current one whose own value changed, and every one
-// whose face with a changed cell then exceeds ratio 2; every other visit is a
-// no-op.  The result equals the full sweep's bit for bit.
-// off/nbr: neighbour of every face-end; act: 1 for active cells; hot:
-// ascending hot cells; pending (one bit per cell) and changed: zero on entry
-// and on return; changed_list receives the cells whose value changed.
-// Every cell queued lies after the one being visited, so the queue is a
-// bitset read forward from a word cursor.
-void sparse_sweep(const int *off, const int *nbr, const unsigned char *act, const int *hot, int nhot, double *dti,
-                  unsigned long long *pending, size_t nwords, unsigned char *changed, std::vector<int> &changed_list) {
-  auto push = [&](int a) { pending[(size_t)a >> 6] |= 1ull << (a & 63); };
+// ratio 2 at the values of that moment.  So the sweep visits, in list order,
+// the cells with such a face at the initial values ("hot"), every active cell
+// after the current one whose own value changed, and every one whose face
+// with a changed cell then exceeds ratio 2; every other visit is a no-op.
+// The result equals the full sweep's bit for bit.
+// off/nbr: neighbour of every face-end (mesh cells); rank: the place of each
+// active cell in the list, -1 for the others; order: the list (rank -> cell);
+// hot: the ranks of the hot cells, ascending; pending (one bit per rank) and
+// changed: zero on entry and on return; changed_list receives the cells whose
+// value changed.  Every rank queued lies after the one being visited, so the
+// queue is a bitset read forward from a word cursor.
+void sparse_sweep(const int *off, const int *nbr, const int *rank, const int *order, const int *hot, int nhot,
+                  double *dti, unsigned long long *pending, size_t nwords, unsigned char *changed,
+                  std::vector<int> &changed_list) {
+  auto push = [&](int r) { pending[(size_t)r >> 6] |= 1ull << (r & 63); };
   const double scale_fac = 2.0;
   // Invariant: every active cell after the one being visited that is not
   // pending has all its faces within ratio 2 at the current values, so its
   // visit would change nothing.  When the value of c changes while visiting
-  // g, c itself (if active and after g) is queued, and of its active
-  // neighbours after g those whose face with c now exceeds ratio 2.
-  auto touched = [&](int c, int g) {
+  // the cell of rank rg, c itself (if active and after it) is queued, and of
+  // its active neighbours after it those whose face with c now exceeds ratio 2.
+  auto touched = [&](int c, int rg) {
     if (!changed[c]) {
       changed[c] = 1;
       changed_list.push_back(c);
     }
-    if (act[c] && c > g) push(c);
+    if (rank[c] > rg) push(rank[c]);
     const double dc = dti[c];
     for (int e = off[c]; e < off[c + 1]; e++) {
       const int a = nbr[e];
-      if (act[a] && a > g && (scale_fac * dti[a] < dc || scale_fac * dc < dti[a])) push(a);
+      if (rank[a] > rg && (scale_fac * dti[a] < dc || scale_fac * dc < dti[a])) push(rank[a]);
     }
   };
   for (int k = 0; k < nhot; k++) push(hot[k]);
@@ -420,16 +446,17 @@ void sparse_sweep(const int *off, const int *nbr, const unsigned char *act, cons
     if (w >= nwords) break;
     const int b = __builtin_ctzll(pending[w]);
     pending[w] &= pending[w] - 1;
-    const int g = (int)(w * 64 + (size_t)b);
+    const int rg = (int)(w * 64 + (size_t)b);
+    const int g = order[rg];
     for (int e = off[g]; e < off[g + 1]; e++) {
       const int j = nbr[e];
       if (scale_fac * dti[g] < dti[j]) {
         dti[j] = scale_fac * dti[g];
-        touched(j, g);
+        touched(j, rg);
       }
       if (scale_fac * dti[j] < dti[g]) {
         dti[g] = scale_fac * dti[j];
-        touched(g, g);
+        touched(g, rg);
       }
     }
   }
@@ -451,10 +478,11 @@ void fvm_substep_release(fvm_ctx *c) {
   SubstepState &s = c->sub;
   for (DBuf *b : {&s.gs1, &s.gs2, &s.gc, &s.gn, &s.goff, &s.gface, &s.gvol, &s.rung, &s.act, &s.apos, &s.fmark,
                   &s.fpos, &s.fl, &s.cmark, &s.iflag, &s.ipos, &s.g2l, &s.perm, &s.len, &s.hist, &s.err, &s.sdti,
-                  &s.hot, &s.hpos, &s.hlist, &s.chg, &s.dtig, &s.actg})
+                  &s.hot, &s.hpos, &s.hlist, &s.chg, &s.dtig, &s.rankg, &s.seq, &s.okey,
+                  &s.okey2, &s.oval2, &s.otmp})
     fvm_release(*b);
   for (auto &b : s.g) fvm_release(b);
-  for (void *h : {(void *)s.dtih, (void *)s.acth, (void *)s.sdh, (void *)s.permh, (void *)s.hoth, (void *)s.chgi,
+  for (void *h : {(void *)s.dtih, (void *)s.rankh, (void *)s.sdh, (void *)s.permh, (void *)s.hoth, (void *)s.chgi,
                   (void *)s.chgv, (void *)s.errh})
     if (h) (void)hipHostFree(h);
   s = SubstepState{};
@@ -482,8 +510,9 @@ int read_err(fvm_ctx *c, int *e) {
 }
 
 // Build the local layout of one substep in the context's mesh and fields:
-// active cells (rung in `mask`) in mesh order, then the inactive cells that
-// share a face with them; faces of the active cells, s1 active.
+// active cells (rung in `mask`) in the Scheduler's order, then the inactive
+// cells that share a face with them, in mesh order; faces of the active
+// cells, s1 active.
 int build_layout(fvm_ctx *c, unsigned mask, double t_global, int check, int *counts) {
   SubstepState &s = c->sub;
   const int n = s.n, nface = s.nface;
@@ -498,6 +527,17 @@ int build_layout(fvm_ctx *c, unsigned mask, double t_global, int check, int *cou
     return fvm_fail(c, FVM_EHIP, "substep: scan of active cells failed");
   if (*s.errh & 1) return fvm_fail(c, FVM_ESTATE, "substep: an active cell does not end its step at the current time");
   SLAUNCH(c, n, k_sub_compact, P<int>(s.act), P<int>(s.apos), n, 0, P<int>(s.perm));
+  // the active list in the Scheduler's order: finest rung first, each rung in
+  // push order (Scheduler.h pull_active_list)
+  if (nA > 1) {
+    SLAUNCH(c, nA, k_sub_order_keys, P<int>(s.perm), nA, P<int>(s.rung), P<unsigned long long>(s.seq),
+            P<unsigned long long>(s.okey));
+    size_t tb = s.otmp_bytes;
+    SCHK(c, rocprim::radix_sort_pairs(s.otmp.p, tb, P<unsigned long long>(s.okey), P<unsigned long long>(s.okey2),
+                                      P<int>(s.perm), P<int>(s.oval2), (unsigned)nA, 0, 64, st));
+    SCHK(c, hipMemcpyAsync(s.perm.p, s.oval2.p, 4 * (size_t)nA, hipMemcpyDeviceToDevice, st));
+    SLAUNCH(c, nA, k_sub_apos, P<int>(s.perm), nA, P<int>(s.apos));
+  }
   SLAUNCH(c, nface, k_sub_fill_i, P<int>(s.fmark), nface, 0);
   SLAUNCH(c, nA, k_sub_mark_faces, P<int>(s.perm), nA, P<int>(s.goff), P<int>(s.gface), P<int>(s.fmark));
   if (fvm::exclusive_scan_flags(c->scan, P<int>(s.fmark), P<int>(s.fpos), nface, &nfl, st))
@@ -640,35 +680,32 @@ int substep_sweep(fvm_ctx *c) {
   if (fvm::exclusive_scan_flags(c->scan, P<int>(s.hot), P<int>(s.hpos), nA, &nhot, st))
     return fvm_fail(c, FVM_EHIP, "substep: scan of the limiter's cells failed");
   if (nhot == 0) return FVM_OK;
+  // the hot cells' places in the active list, ascending
   SLAUNCH(c, nA, k_sub_compact, P<int>(s.hot), P<int>(s.hpos), nA, 0, P<int>(s.hlist));
-  SLAUNCH(c, nhot, k_sub_map, P<int>(s.hlist), nhot, P<int>(s.perm));
-  SCHK(c, hipMemcpyAsync(s.hoth, s.hlist.p, 4 * (size_t)nhot, hipMemcpyDeviceToHost, st));
   const bool global = (long long)npl * 8 > n;
+  SCHK(c, hipMemcpyAsync(s.hoth, s.hlist.p, 4 * (size_t)nhot, hipMemcpyDeviceToHost, st));
+  SCHK(c, hipMemcpyAsync(s.permh, s.perm.p, 4 * (size_t)(global ? nA : npl), hipMemcpyDeviceToHost, st));
   if (global) {
-    SLAUNCH(c, n, k_sub_fill_u8, P<unsigned char>(s.actg), n);
+    SLAUNCH(c, n, k_sub_fill_i, P<int>(s.rankg), n, -1);
     SLAUNCH(c, npl, k_sub_sweep_global, P<double>(s.sdti), P<int>(s.perm), npl, nA, P<double>(s.dtig),
-            P<unsigned char>(s.actg));
+            P<int>(s.rankg));
     SCHK(c, hipMemcpyAsync(s.dtih, s.dtig.p, 8 * (size_t)n, hipMemcpyDeviceToHost, st));
-    SCHK(c, hipMemcpyAsync(s.acth, s.actg.p, (size_t)n, hipMemcpyDeviceToHost, st));
+    SCHK(c, hipMemcpyAsync(s.rankh, s.rankg.p, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
     SCHK(c, hipStreamSynchronize(st));
   } else {
     SCHK(c, hipMemcpyAsync(s.sdh, s.sdti.p, 8 * (size_t)npl, hipMemcpyDeviceToHost, st));
-    SCHK(c, hipMemcpyAsync(s.permh, s.perm.p, 4 * (size_t)npl, hipMemcpyDeviceToHost, st));
     SCHK(c, hipStreamSynchronize(st));
     for (int l = 0; l < npl; l++) {
       const int g = s.permh[l];
       s.dtih[g] = s.sdh[l];
-      s.acth[g] = l < nA;
+      s.rankh[g] = l < nA ? l : -1;
     }
   }
   std::vector<int> chg;
-  sparse_sweep(s.hoff.data(), s.hnbr.data(), s.acth, s.hoth, nhot, s.dtih, s.pending.data(), s.pending.size(),
-               s.changed.data(), chg);
-  // acth is all zero between substeps
-  if (global)
-    std::memset(s.acth, 0, (size_t)n);
-  else
-    for (int l = 0; l < nA; l++) s.acth[s.permh[l]] = 0;
+  sparse_sweep(s.hoff.data(), s.hnbr.data(), s.rankh, s.permh, s.hoth, nhot, s.dtih, s.pending.data(),
+               s.pending.size(), s.changed.data(), chg);
+  // rankh is -1 everywhere between substeps
+  for (int l = 0; l < nA; l++) s.rankh[s.permh[l]] = -1;
   const int nchg = (int)chg.size();
   if (nchg > 0) {
     for (int k = 0; k < nchg; k++) {
@@ -730,11 +767,20 @@ int fvm_substep_setup(fvm_ctx *c, int ncell, int nface, const fvm_face *faces, c
                            {&s.apos, 4 * N}, {&s.fmark, 4 * F}, {&s.fpos, 4 * F}, {&s.fl, 4 * F}, {&s.cmark, 4 * N},
                            {&s.iflag, 4 * N}, {&s.ipos, 4 * N}, {&s.g2l, 4 * N}, {&s.perm, 4 * N}, {&s.len, 4 * N},
                            {&s.hist, 4 * kRungMax}, {&s.err, 4}, {&s.sdti, 8 * N}, {&s.hot, 4 * N},
-                           {&s.hpos, 4 * N}, {&s.hlist, 4 * N}, {&s.dtig, 8 * N}, {&s.actg, N},
-                           {&s.chg, 12 * N + 8}})))
+                           {&s.hpos, 4 * N}, {&s.hlist, 4 * N}, {&s.dtig, 8 * N}, {&s.rankg, 4 * N},
+                           {&s.chg, 12 * N + 8}, {&s.seq, 8 * N}, {&s.okey, 8 * N}, {&s.okey2, 8 * N},
+                           {&s.oval2, 4 * N}})))
     return rc;
+  {
+    // scratch of the active-list sort at its largest (every cell active)
+    size_t tb = 0;
+    SCHK(c, rocprim::radix_sort_pairs(nullptr, tb, P<unsigned long long>(s.okey), P<unsigned long long>(s.okey2),
+                                      P<int>(s.perm), P<int>(s.oval2), (unsigned)ncell, 0, 64, c->stream));
+    if ((rc = fvm_ensure(c, s.otmp, std::max<size_t>(tb, 256)))) return rc;
+    s.otmp_bytes = tb;
+  }
   SCHK(c, hipHostMalloc((void **)&s.dtih, 8 * N, hipHostMallocDefault));
-  SCHK(c, hipHostMalloc((void **)&s.acth, N, hipHostMallocDefault));
+  SCHK(c, hipHostMalloc((void **)&s.rankh, 4 * N, hipHostMallocDefault));
   SCHK(c, hipHostMalloc((void **)&s.sdh, 8 * N, hipHostMallocDefault));
   SCHK(c, hipHostMalloc((void **)&s.permh, 4 * N, hipHostMallocDefault));
   SCHK(c, hipHostMalloc((void **)&s.hoth, 4 * N, hipHostMallocDefault));
@@ -742,7 +788,7 @@ int fvm_substep_setup(fvm_ctx *c, int ncell, int nface, const fvm_face *faces, c
   SCHK(c, hipHostMalloc((void **)&s.chgv, 8 * N, hipHostMallocDefault));
   SCHK(c, hipHostMalloc((void **)&s.errh, sizeof(int), hipHostMallocDefault));
   *s.errh = 0;
-  std::memset(s.acth, 0, N);
+  for (size_t i = 0; i < N; i++) s.rankh[i] = -1;
   try {
     s.hoff.assign(cell_face_off, cell_face_off + N + 1);
     s.hnbr.resize((size_t)nref);
@@ -873,6 +919,9 @@ int fvm_substep_init_rungs(fvm_ctx *c, double dtmax, int nrung, int *rung_hist) 
   if (int rc = check_ready(c)) return rc;
   if (!(dtmax > 0.0) || nrung < 1 || nrung > kRungMax - 1) return fvm_fail(c, FVM_EARG, "fvm_substep_init_rungs: bad dtmax or nrung");
   int rc;
+  // every cell pushed in index order (System.cpp:300-315)
+  SLAUNCH(c, c->sub.n, k_sub_iota64, P<unsigned long long>(c->sub.seq), c->sub.n);
+  c->sub.next_seq = (unsigned long long)c->sub.n;
   if ((rc = build_layout(c, 0xFFFFFFFFu, 0.0, 0, nullptr))) return rc;
   double dtmin = 0.0;
   if ((rc = fvm_compute_timestep(c, &dtmin))) return rc;
@@ -900,6 +949,9 @@ int fvm_substep_run(fvm_ctx *c, unsigned rung_mask, double t_global, double dtma
   const int pos = fvm_compute_fluid_update(c);
   if (pos != FVM_OK && pos != FVM_EPOS) return pos;
   if ((rc = write_back(c, true))) return rc;
+  // the active cells pushed again in the order of the active list
+  SLAUNCH(c, cnt[0], k_sub_stamp, P<int>(c->sub.perm), cnt[0], c->sub.next_seq, P<unsigned long long>(c->sub.seq));
+  c->sub.next_seq += (unsigned long long)cnt[0];
   if (counts) {
     counts[0] = cnt[0];
     counts[1] = nfa;
@@ -930,12 +982,17 @@ int fvm_timestep_limiter_sweep(int ncell, const int *cell_off, const int *nbr, c
       }
       if (h) hot.push_back(i);
     }
-    std::vector<unsigned char> act(active, active + ncell), changed(ncell, 0);
-    for (auto &a : act) a = a ? 1 : 0;
+    // active cells visited in ascending index: rank = index
+    std::vector<int> rank(ncell), order(ncell);
+    for (int i = 0; i < ncell; i++) {
+      rank[i] = active[i] ? i : -1;
+      order[i] = i;
+    }
+    std::vector<unsigned char> changed(ncell, 0);
     std::vector<unsigned long long> pending(((size_t)ncell + 63) / 64, 0ull);
     std::vector<int> chg;
-    sparse_sweep(cell_off, nbr, act.data(), hot.data(), (int)hot.size(), dti, pending.data(), pending.size(),
-                 changed.data(), chg);
+    sparse_sweep(cell_off, nbr, rank.data(), order.data(), hot.data(), (int)hot.size(), dti, pending.data(),
+                 pending.size(), changed.data(), chg);
   } catch (...) {
     return FVM_EARG;
   }

@@ -6,8 +6,8 @@
 //                   Dedner B_n/psi exactly as computeFlux, FP64), divided by two
 //                   exact symmetries of the HLLD flux: the Galilean boost
 //                   V = w_n n + u_t and the MHD scaling (rho_s, c_s); the
-//                   network answers the residual over the central flux of the
-//                   two sides per unit normalised jump
+//                   network answers the residual over the HLL flux of the two
+//                   sides per unit normalised jump (fvm_mlp.h)
 //   training        Adam over bf16-operand MFMA GEMMs (v_mfma_f32_16x16x32_bf16,
 //                   FP32 accumulation), one kernel per GEMM / elementwise step
 //   inference       one fused kernel: each wavefront featurises 64 faces, runs
@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(256) k_samples(int n, double gamma, const doub
       const double GE = F[ENER] - (V[0] * Gp[0] + V[1] * Gp[1] + V[2] * Gp[2]) -
                         0.5 * (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) * Gm;
       const double Gb1 = F[WBY] + s.Bn * V[1], Gb2 = F[WBZ] + s.Bn * V[2];
-      // residual over the central flux, per unit jump
+      // residual over the HLL base flux, per unit jump
       const double g[7] = {Gm / (rs * cs), Gp[0] / Ps, Gp[1] / Ps, Gp[2] / Ps, GE / (Ps * cs), Gb1 / (Bs * cs),
                            Gb2 / (Bs * cs)};
       for (int k = 0; k < 7; k++) t[k] = (float)((g[k] - Fc[k]) / J);

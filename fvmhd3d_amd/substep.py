@@ -86,15 +86,16 @@ class RungClock:
         return ((mask >> np.asarray(rung, dtype=np.int64)) & 1).astype(bool), dt
 
 
-def substep_layout(mesh, active: np.ndarray, bnd: np.ndarray):
+def substep_layout(mesh, active: np.ndarray, bnd: np.ndarray, order: np.ndarray | None = None):
     """Reference layout of one substep.
 
     Returns (perm, nactive, nimport_loc, faces, cell_off, cell_face, cvol, is_ngb)
-    where perm maps local index -> mesh cell: active cells (mesh order) first,
-    then the inactive cells that share a face with an active cell.  Faces are the
-    faces of active cells, oriented so that s1 is active (the creating cell,
+    where perm maps local index -> mesh cell: the active cells first, in
+    ``order`` (the active list; mesh order when None), then the inactive cells
+    that share a face with an active cell, in mesh order.  Faces are the faces
+    of active cells, oriented so that s1 is active (the creating cell,
     localMesh.cpp:497-591) with the centroid in s1's frame."""
-    A = np.nonzero(active)[0]
+    A = np.nonzero(active)[0] if order is None else np.asarray(order, dtype=np.int64)
     n = mesh.ncell
     starts, ends = mesh.cell_off[A], mesh.cell_off[A + 1]
     lens = ends - starts
@@ -150,6 +151,12 @@ class Substepper:
     def __init__(self, mesh, W, gamma=5.0 / 3.0, courant=0.4, dtmax=1.0 / 64, limiter=0, nrung=6):
         n = mesh.ncell
         self.mesh = mesh
+        # the Scheduler's bucket order (Scheduler.h push_particle / pull_active_list):
+        # each cell's push stamp; a pull lists the finest rung first, each rung's
+        # cells in push order, and the active cells are pushed again, in that
+        # order, at the end of their step (Iterate.cpp:146-168)
+        self.seq = np.arange(n, dtype=np.int64)
+        self.next_seq = n
         self.gamma, self.courant, self.limiter, self.nrung = gamma, courant, limiter, nrung
         self.clock = RungClock(dtmax)
         self.W = np.array(W, dtype=np.float64)
@@ -179,9 +186,18 @@ class Substepper:
         dtn[perm] = st.dt_new
         self.rung[:] = self.clock.rungs_of(dtn, self.nrung)
         self.tend = self.tbeg + self.clock.dts_of(self.rung)
+        # every cell pushed in index order (System.cpp:300-315)
+        self.seq[:] = np.arange(self.mesh.ncell)
+        self.next_seq = self.mesh.ncell
 
-    def _state(self, active):
-        perm, nact, nimp, faces, coff, cface, cvol, is_ngb = substep_layout(self.mesh, active, self.bnd)
+    def active_list(self, active):
+        """The Scheduler's pull order of the active cells: finest rung first,
+        each rung in push order."""
+        A = np.nonzero(active)[0]
+        return A[np.lexsort((self.seq[A], -self.rung[A]))]
+
+    def _state(self, active, order=None):
+        perm, nact, nimp, faces, coff, cface, cvol, is_ngb = substep_layout(self.mesh, active, self.bnd, order)
         st = CellState(np=len(perm), nactive=nact, nimport_loc=nimp, faces=faces, cell_off=coff,
                        cell_face=cface, cvol=cvol, t_global=self.t_global, gamma=self.gamma,
                        courant=self.courant, box=tuple(float(b) for b in self.mesh.box))
@@ -211,7 +227,9 @@ class Substepper:
         active, _ = self.clock.pull(self.rung)
         self.t_global = self.clock.tsys()
         assert np.all(self.tend[active] == self.t_global)
-        st, perm = self._state(active)
+        order = self.active_list(active)
+        self.last_active = order
+        st, perm = self._state(active, order)
         nact = st.nactive
         # predict inactive local cells to the current time (Iterate.cpp:99-106), on the copy
         dtp = self.t_global - st.tbeg[nact:]
@@ -243,6 +261,9 @@ class Substepper:
         self.dU[I] += st.dU[nact:]
         self.dgp[I] += st.dgp[nact:]
         self.ddivB[I] += st.ddivB[nact:]
+        # pushed again in the order of the active list
+        self.seq[A] = self.next_seq + np.arange(nact)
+        self.next_seq += nact
         self.history.append((self.t_global, nact, nfa))
         return nact, nfa
 

@@ -9,14 +9,15 @@
  * FP32 accumulation on the matrix cores.  Inputs are the HLLD inputs of a face
  * in the face frame after a Galilean boost (face speed along the normal, mean
  * tangential velocity) and the MHD scaling (density, fast-speed scale) are
- * divided out, plus the size J of the normalised jump between the sides.
- * Outputs 0..6 are (G' - Fc)/J for the normalised moving-frame fluxes G' (mass,
- * three momenta, energy, two tangential induction components), Fc the central
- * flux of the two sides: the flux error scales with the jump.  Output 7
+ * divided out: the mean state of the two sides, the unit direction and the
+ * size J of the normalised jump between them, the log density and pressure
+ * ratios and the larger fast speed.  Outputs 0..6 are (G' - Fb)/J for the
+ * normalised moving-frame fluxes G' (mass, three momenta, energy, two
+ * tangential induction components), Fb the HLL flux of the two sides with the
+ * HLLD's own outer wave speeds: the flux error scales with the jump.  Output 7
  * predicts log10 of the weighted squared error of outputs 0..6 (the
- * confidence gate).  The
- * normal induction flux -w_n B_n, the Dedner B_n/psi, and the psi and entropy
- * fluxes are formed exactly as computeFlux forms them.
+ * confidence gate).  The normal induction flux -w_n B_n, the Dedner B_n/psi,
+ * and the psi and entropy fluxes are formed exactly as computeFlux forms them.
  *
  * Parameters are one FP32 vector, layer by layer: W0[16][128] b0[128]
  * W1[128][128] b1[128] W2[128][128] b2[128] W3[128][8] b3[8] (W[in][out]).

@@ -21,10 +21,14 @@ flux have been divided out:
 Both are pinned in tests/test_surrogate.py against the oracle's HLLD.
 
 Targets are residuals: with the normalised moving-frame states of both sides,
-their central flux Fc = (F(L) + F(R))/2 and the size J of the normalised jump,
-the network learns (G' - Fc)/J (G' the normalised moving-frame HLLD flux), so
-its error in the flux scales with the jump and vanishes for smooth flow; J is
-also feature 15.
+their HLL flux Fb (``hll_flux``) and the size J of the normalised jump, the
+network learns (G' - Fb)/J (G' the normalised moving-frame HLLD flux): what
+the HLLD's intermediate states add to the two-wave flux, per unit jump, so its
+error in the flux scales with the jump.  The features are the mean state of
+the two sides, the unit jump direction d/J, J, the log density and pressure
+ratios and the larger fast speed: a smooth-flow jump is a small difference of
+O(1) values that bf16 cannot hold, while its direction and size survive the
+rounding.
 
 Network: 16 -> 128 -> 128 -> 128 -> 8, ReLU, weights W[in][out]; every GEMM
 operand is rounded to bfloat16 (round-to-nearest-even) and products are
@@ -93,36 +97,56 @@ def featurize(gamma, wij, Wi, Wj, normal):
     Ps = rs * cs * cs
     Bs = np.sqrt(Ps)
     vnL, vnR = (vL[:, 0] - wn) / cs, (vR[:, 0] - wn) / cs
-    x = np.zeros((len(Wi), N_IN))
-    x[:, 0] = rL / rs - 1.0
-    x[:, 1] = pL / Ps
-    x[:, 2] = pR / Ps
-    x[:, 3] = vnL
-    x[:, 4] = vnR
-    x[:, 5] = (vL[:, 1] - ut[:, 0]) / cs
-    x[:, 6] = (vL[:, 2] - ut[:, 1]) / cs
-    x[:, 7] = Bn / Bs
-    x[:, 8] = BL[:, 1] / Bs
-    x[:, 9] = BR[:, 1] / Bs
-    x[:, 10] = BL[:, 2] / Bs
-    x[:, 11] = BR[:, 2] / Bs
-    x[:, 12] = np.log(rR / rL)
-    x[:, 13] = np.log(pR / pL)
-    x[:, 14] = vnR - vnL
-    # normalised moving-frame states and their central flux
+    # normalised moving-frame states: rho, v_n, v_t1, v_t2, p, e, B_n, B_t1, B_t2
     eL = Wi[:, 4] / Ps if gamma > 1.0 else pL / Ps
     eR = Wj[:, 4] / Ps if gamma > 1.0 else pR / Ps
     sideL = (rL / rs, vnL, (vL[:, 1] - ut[:, 0]) / cs, (vL[:, 2] - ut[:, 1]) / cs, pL / Ps, eL,
              Bn / Bs, BL[:, 1] / Bs, BL[:, 2] / Bs)
     sideR = (rR / rs, vnR, (vR[:, 1] - ut[:, 0]) / cs, (vR[:, 2] - ut[:, 1]) / cs, pR / Ps, eR,
              Bn / Bs, BR[:, 1] / Bs, BR[:, 2] / Bs)
-    Fc = 0.5 * (physical_flux(*sideL) + physical_flux(*sideR))
+    Fc = hll_flux(gamma, sideL, sideR)
     dl = np.stack([sideR[k] - sideL[k] for k in (0, 1, 2, 3, 4, 7, 8)], 1)
     J = np.sqrt((dl * dl).sum(1)) + J_EPS
-    x[:, 15] = J
+    x = np.zeros((len(Wi), N_IN))
+    x[:, 0] = 0.5 * (vnL + vnR)
+    x[:, 1] = 0.5 * (pL + pR) / Ps
+    x[:, 2] = Bn / Bs
+    x[:, 3] = 0.5 * (BL[:, 1] + BR[:, 1]) / Bs
+    x[:, 4] = 0.5 * (BL[:, 2] + BR[:, 2]) / Bs
+    x[:, 5:12] = dl / J[:, None]
+    x[:, 12] = J
+    x[:, 13] = np.log(rR / rL)
+    x[:, 14] = np.log(pR / pL)
+    x[:, 15] = fast_speed_max(gamma, sideL, sideR)
     ctx = dict(A=A, wn=wn, ut=ut, rs=rs, cs=cs, Ps=Ps, Bs=Bs, Bn=Bn, psi=psi, psiL=psiL, psiR=psiR,
                entrL=Wi[:, 9], entrR=Wj[:, 9], Fc=Fc, J=J)
     return x, ctx
+
+
+def fast_speed_max(gamma, sL, sR):
+    """max of the two sides' fast speeds sqrt((gamma p + |B|^2)/rho), normalised units."""
+    cf2 = lambda q: (gamma * q[4] + q[6] * q[6] + q[7] * q[7] + q[8] * q[8]) / q[0]
+    return np.sqrt(np.maximum(cf2(sL), cf2(sR)))
+
+
+def hll_flux(gamma, sL, sR):
+    """HLL flux (Harten, Lax & van Leer 1983) of two normalised moving-frame
+    states at the face, which is at rest in that frame, with the wave speeds
+    riemannSolver_HLLD starts from (computeFlux.cpp:218-229):
+    S_L = min(v_nL, v_nR) - c_f,max, S_R = max(v_nL, v_nR) + c_f,max.
+    Components: mass, mom n, mom t1, mom t2, energy, B t1, B t2."""
+    FL, FR = physical_flux(*sL), physical_flux(*sR)
+    cfm = fast_speed_max(gamma, sL, sR)
+    SL = np.minimum(sL[1], sR[1]) - cfm
+    SR = np.maximum(sL[1], sR[1]) + cfm
+
+    def cons(q):
+        E = q[5] + 0.5 * q[0] * (q[1] ** 2 + q[2] ** 2 + q[3] ** 2) + 0.5 * (q[6] ** 2 + q[7] ** 2 + q[8] ** 2)
+        return np.stack([q[0], q[0] * q[1], q[0] * q[2], q[0] * q[3], E, q[7], q[8]], 1)
+
+    UL, UR = cons(sL), cons(sR)
+    mid = (SR[:, None] * FL - SL[:, None] * FR + (SL * SR)[:, None] * (UR - UL)) / (SR - SL)[:, None]
+    return np.where((SL >= 0.0)[:, None], FL, np.where((SR <= 0.0)[:, None], FR, mid))
 
 
 def physical_flux(r, u, v1, v2, p, e, bn, b1, b2):

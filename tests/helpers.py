@@ -17,6 +17,16 @@ def lattice_mesh(n=12, jitter=0.1, seed=123):
     return _CACHE[key]
 
 
+def workload_mesh(generators, n, seed=123):
+    """The mesh of a workload (mesh.workload_mesh), built once per test process:
+    the 256^3 mesh takes a minute or two to build and is shared by the tests
+    that run on it."""
+    key = ("workload", generators, n, seed)
+    if key not in _CACHE:
+        _CACHE[key] = M.workload_mesh(generators, n, seed=seed)
+    return _CACHE[key]
+
+
 def random_mesh(n=2000, seed=7):
     key = ("rand", n, seed)
     if key not in _CACHE:

@@ -45,6 +45,61 @@ def test_substep_layout_is_reference_layout():
     assert np.all(np.sum(dri * faces["n"], 1) > 0)
 
 
+class ReferenceScheduler:
+    """Scheduler.h's rung lists, written out: push_particle appends a cell to
+    its rung's list; pull_active_list takes every list whose rung ends its step
+    now, finest rung first (bit i of tsys ^ tnext is rung 31 - i), each list in
+    push order, and clears it."""
+
+    def __init__(self, dtmax):
+        self.lists = [[] for _ in range(32)]
+        self.dt_tick = dtmax / (1 << 31)
+        self.tsysU = 0
+
+    def push(self, cell, rung):
+        self.lists[rung].append(int(cell))
+
+    def pull(self):
+        max_rung = max(r for r in range(32) if self.lists[r])
+        dtU = 1 << (31 - max_rung)
+        tnext = (self.tsysU // dtU + 1) * dtU
+        flags = self.tsysU ^ tnext
+        out = []
+        for i in range(32):
+            if flags & (1 << i):
+                out += self.lists[31 - i]
+                self.lists[31 - i] = []
+        self.tsysU = tnext
+        return out
+
+
+def test_active_list_follows_the_scheduler():
+    """The host driver's active list, substep by substep, equals the pull of
+    Scheduler.h's lists fed with the same pushes: every cell in index order at
+    the start (System.cpp:300-315), then each substep's active cells in the
+    order of its active list with their new rungs (Iterate.cpp:146-168)."""
+    pos = M.uniform_random(10 ** 3, seed=9)
+    pos = pos[M.morton_order(pos, (1, 1, 1))]
+    m = M.build_periodic(pos)
+    S = Substepper(m, ic_disk(m.pos), dtmax=0.25, limiter=2)
+    B = OracleBackend(limiter=2)
+    S.init_rungs(B)
+    ref = ReferenceScheduler(S.clock.dtmax)
+    for i in range(m.ncell):
+        ref.push(i, S.rung[i])
+    orders_differ = False
+    for _ in range(12):
+        want = ref.pull()
+        S.substep(B)
+        got = [int(c) for c in S.last_active]
+        assert got == want
+        orders_differ = orders_differ or got != sorted(got)
+        for c in got:
+            ref.push(c, S.rung[c])
+    assert len(set(int(r) for r in S.rung)) >= 3
+    assert orders_differ  # the pull order is not the mesh order on this run
+
+
 def test_substepper_oracle_runs_conservatively():
     pos = M.uniform_random(12 ** 3, seed=5)
     pos = pos[M.morton_order(pos, (1, 1, 1))]
