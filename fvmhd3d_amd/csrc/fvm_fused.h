@@ -76,6 +76,10 @@ __device__ __forceinline__ int xcd_cell(int nb) {
 // fewer bytes than with 256-byte records, and the gathers are no slower.
 constexpr int REC = 52;
 enum { R_W = 0, R_T = 10, R_G = 20 };
+// A second, compact copy of each cell's W, 12 values apart (48 bytes in FP32):
+// the predictor kernel gathers only the neighbours' W, and from this array a
+// cache line holds the W of more than two cells instead of part of one record.
+constexpr int RWC = 12;
 
 // Per face-end record (CSR order of the cell's face list): the neighbour
 // (j, or ~j when this cell is s2) and the face geometry in canonical
@@ -114,6 +118,8 @@ template <typename T> struct FusedArgs {
   const T *rec;             // [n][REC] records of this step
   T *rec_w;                 // records written by this launch (K1: rec itself; K2: next step's)
   T *rec_next;
+  const T *wcur;            // [n][RWC] compact W of this step (predictor gathers)
+  T *wnext;                 // [n][RWC] compact W written for the next step
   double *W64;              // [10][n] FP64 primitive state (own cell only)
   double *U;                // [10][n] FP64 conserved state (own cell only)
   T *mm;                    // [20][n] min/max (opt-in limiters)
@@ -307,9 +313,8 @@ k_cfl_predict_recon(FusedArgs<T> A) {
     const double g64 = A.gamma;
     const T g = (T)A.gamma;
     const double V = ntl(A.cvol + c);
-    const T *rc = A.rec + (size_t)c * REC;
     T Wc[NF];
-    ldv<NF>(rc + R_W, Wc);
+    ldv<NF>(A.wcur + (size_t)c * RWC, Wc);
 
     // CFL (computeTimestep.cpp:13-31), FP64 from U
     double pv[3] = {0.0, 0.0, 0.0};
@@ -366,7 +371,7 @@ k_cfl_predict_recon(FusedArgs<T> A) {
       if (q + 1 < deg) rn = A.fe[e + 64];
       const int j = fe_nbr(rc0.nbr);
       T Wj[NF];
-      ldv<NF>(A.rec + (size_t)j * REC + R_W, Wj);
+      ldv<NF>(A.wcur + (size_t)j * RWC, Wj);
       FE<T> fe;
       decode_fe(rc0, fe);
 
@@ -806,6 +811,7 @@ __device__ __forceinline__ void cell_update(const FusedArgs<T> &A, int c, double
     Wn[k] = (T)W1[k];
   }
   stv<NF>(A.rec_next + (size_t)c * REC + R_W, Wn);
+  stv<NF>(A.wnext + (size_t)c * RWC, Wn);
 }
 
 // the step's dt and the extrapolation time of the face states: every cell has
@@ -1124,12 +1130,14 @@ __global__ void k_scatter_soa(const double *src, double *dst, const int *perm, i
 }
 
 // W field (SoA FP64) -> W slice of the records; records -> fields
-template <typename T> __global__ void k_rec_from_W(const double *W, T *rec, int n) {
+template <typename T> __global__ void k_rec_from_W(const double *W, T *rec, T *wc, int n) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   T *r = rec + (size_t)c * REC;
+  T *w = wc + (size_t)c * RWC;
   for (int k = 0; k < REC; k++) r[k] = T(0.0);
-  for (int k = 0; k < NF; k++) r[R_W + k] = (T)SF(W, k, c, n);
+  for (int k = 0; k < RWC; k++) w[k] = T(0.0);
+  for (int k = 0; k < NF; k++) r[R_W + k] = w[k] = (T)SF(W, k, c, n);
 }
 template <typename T, typename S>
 __global__ void k_rec_to_field(const T *rec, int off, S *dst, const int *perm, int n) {
@@ -1163,6 +1171,8 @@ template <typename T> FusedArgs<T> fused_args(fvm_ctx *c, double dtmax) {
   A.rec = (const T *)z.rec[z.cur].p;
   A.rec_w = (T *)z.rec[z.cur].p;
   A.rec_next = (T *)z.rec[z.cur ^ 1].p;
+  A.wcur = (const T *)z.wc[z.cur].p;
+  A.wnext = (T *)z.wc[z.cur ^ 1].p;
   A.W64 = (double *)z.W64.p;
   A.U = (double *)z.U.p;
   A.mm = (T *)z.mm.p;
@@ -1221,6 +1231,7 @@ template <typename T> int fused_setup_t(fvm_ctx *c) {
       (rc = fvm_ensure(c, z.W64, NF * nd)) || (rc = fvm_ensure(c, z.cvol, nd)) ||
       (rc = fvm_ensure(c, z.mvel, 3 * nd)) || (rc = fvm_ensure(c, z.rec[0], es * REC * (size_t)n)) ||
       (rc = fvm_ensure(c, z.rec[1], es * REC * (size_t)n)) || (rc = fvm_ensure(c, z.mm, es * 20 * (size_t)n)) ||
+      (rc = fvm_ensure(c, z.wc[0], es * RWC * (size_t)n)) || (rc = fvm_ensure(c, z.wc[1], es * RWC * (size_t)n)) ||
       (rc = fvm_ensure(c, z.dt_new, nd)) || (rc = fvm_ensure(c, z.dtmin, sizeof(unsigned long long))) ||
       (rc = fvm_ensure(c, z.tdev, sizeof(double) * 2)))
     return rc;
@@ -1259,7 +1270,8 @@ template <typename T> int fused_setup_t(fvm_ctx *c) {
   hipLaunchKernelGGL(k_cell_lmin, dim3(nb), dim3(256), 0, c->stream, (const double *)z.cvol.p, (double *)z.lmin.p, n);
   FVM_HIPCHK(c, hipGetLastError());
   for (int q = 0; q < 2; q++) {
-    hipLaunchKernelGGL(k_rec_from_W<T>, dim3(nb), dim3(256), 0, c->stream, (const double *)z.W64.p, (T *)z.rec[q].p, n);
+    hipLaunchKernelGGL(k_rec_from_W<T>, dim3(nb), dim3(256), 0, c->stream, (const double *)z.W64.p, (T *)z.rec[q].p,
+                       (T *)z.wc[q].p, n);
     FVM_HIPCHK(c, hipGetLastError());
   }
   const unsigned long long inf = 0x7FF0000000000000ULL;

@@ -369,14 +369,63 @@ __global__ void k_sub_sweep_global(const double *__restrict__ sdti, const int *_
   rankg[g] = l < nA ? l : -1;
 }
 
-// sort keys of the active list: finest rung first, then push order
-// (Scheduler.h pull_active_list); the rungs are below 32 and the stamps below 2^58
+// Sort key of an active cell: finest rung first, then push order
+// (Scheduler.h pull_active_list).  Every stamp lies within 2^32 below the next
+// one to be given, so seq - base, base = next - 2^32, takes 32 bits; the rung
+// (below 32) takes the 5 above them, and the sort looks at 37 bits.
+constexpr int kKeyBits = 37;
+__device__ __forceinline__ unsigned long long order_key(int rung, unsigned long long seq, unsigned long long base) {
+  return ((unsigned long long)(kRungMax - 1 - rung) << 32) | ((seq - base) & 0xFFFFFFFFull);
+}
 __global__ void k_sub_order_keys(const int *__restrict__ perm, int nA, const int *__restrict__ rung,
-                                 const unsigned long long *__restrict__ seq, unsigned long long *__restrict__ key) {
+                                 const unsigned long long *__restrict__ seq, unsigned long long base,
+                                 unsigned long long *__restrict__ key) {
   const int k = gidx();
   if (k >= nA) return;
   const int g = perm[k];
-  key[k] = ((unsigned long long)(kRungMax - 1 - rung[g]) << 58) | seq[g];
+  key[k] = order_key(rung[g], seq[g], base);
+}
+
+// the same order for a short list, in one workgroup: bitonic sort in LDS of
+// the keys padded to a power of two (keys are distinct: the stamps are)
+constexpr int kSmallSort = 4096;
+__global__ void __launch_bounds__(1024) k_sub_small_sort(int *__restrict__ perm, int nA, const int *__restrict__ rung,
+                                                        const unsigned long long *__restrict__ seq,
+                                                        unsigned long long base) {
+  __shared__ unsigned long long key[kSmallSort];
+  __shared__ int val[kSmallSort];
+  int P = 1;
+  while (P < nA) P <<= 1;
+  for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    if (i < nA) {
+      const int g = perm[i];
+      key[i] = order_key(rung[g], seq[g], base);
+      val[i] = g;
+    } else {
+      key[i] = ~0ull;
+      val[i] = -1;
+    }
+  }
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const bool up = (i & k) == 0;
+          if ((key[i] > key[ixj]) == up) {
+            const unsigned long long tk = key[i];
+            key[i] = key[ixj];
+            key[ixj] = tk;
+            const int tv = val[i];
+            val[i] = val[ixj];
+            val[ixj] = tv;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int i = threadIdx.x; i < nA; i += blockDim.x) perm[i] = val[i];
 }
 
 // position of each active cell in the active list
@@ -530,12 +579,19 @@ int build_layout(fvm_ctx *c, unsigned mask, double t_global, int check, int *cou
   // the active list in the Scheduler's order: finest rung first, each rung in
   // push order (Scheduler.h pull_active_list)
   if (nA > 1) {
-    SLAUNCH(c, nA, k_sub_order_keys, P<int>(s.perm), nA, P<int>(s.rung), P<unsigned long long>(s.seq),
-            P<unsigned long long>(s.okey));
-    size_t tb = s.otmp_bytes;
-    SCHK(c, rocprim::radix_sort_pairs(s.otmp.p, tb, P<unsigned long long>(s.okey), P<unsigned long long>(s.okey2),
-                                      P<int>(s.perm), P<int>(s.oval2), (unsigned)nA, 0, 64, st));
-    SCHK(c, hipMemcpyAsync(s.perm.p, s.oval2.p, 4 * (size_t)nA, hipMemcpyDeviceToDevice, st));
+    const unsigned long long base = s.next_seq - (1ull << 32);
+    if (nA <= kSmallSort) {
+      hipLaunchKernelGGL(k_sub_small_sort, dim3(1), dim3(1024), 0, st, P<int>(s.perm), nA, P<int>(s.rung),
+                         P<unsigned long long>(s.seq), base);
+      SCHK(c, hipGetLastError());
+    } else {
+      SLAUNCH(c, nA, k_sub_order_keys, P<int>(s.perm), nA, P<int>(s.rung), P<unsigned long long>(s.seq), base,
+              P<unsigned long long>(s.okey));
+      size_t tb = s.otmp_bytes;
+      SCHK(c, rocprim::radix_sort_pairs(s.otmp.p, tb, P<unsigned long long>(s.okey), P<unsigned long long>(s.okey2),
+                                        P<int>(s.perm), P<int>(s.oval2), (unsigned)nA, 0, kKeyBits, st));
+      SCHK(c, hipMemcpyAsync(s.perm.p, s.oval2.p, 4 * (size_t)nA, hipMemcpyDeviceToDevice, st));
+    }
     SLAUNCH(c, nA, k_sub_apos, P<int>(s.perm), nA, P<int>(s.apos));
   }
   SLAUNCH(c, nface, k_sub_fill_i, P<int>(s.fmark), nface, 0);
@@ -775,7 +831,7 @@ int fvm_substep_setup(fvm_ctx *c, int ncell, int nface, const fvm_face *faces, c
     // scratch of the active-list sort at its largest (every cell active)
     size_t tb = 0;
     SCHK(c, rocprim::radix_sort_pairs(nullptr, tb, P<unsigned long long>(s.okey), P<unsigned long long>(s.okey2),
-                                      P<int>(s.perm), P<int>(s.oval2), (unsigned)ncell, 0, 64, c->stream));
+                                      P<int>(s.perm), P<int>(s.oval2), (unsigned)ncell, 0, kKeyBits, c->stream));
     if ((rc = fvm_ensure(c, s.otmp, std::max<size_t>(tb, 256)))) return rc;
     s.otmp_bytes = tb;
   }

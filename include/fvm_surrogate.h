@@ -93,10 +93,6 @@ int fvm_sg_compute_flux_device(fvm_sg *s, int n, const double *wij, const double
                                const int *jbnd, const double *normal, double *psi, double *Bn, double *flux,
                                int *n_fallback);
 
-/* Kernel timing on device-resident copies of one batch: average milliseconds
- * per launch of the surrogate flux (featurise + network + gate, then the
- * fallback launch), of the plain FP64 HLLD (computeFlux in double) and of the
- * FP32 HLLD over the same faces.  ms[3] = {surrogate, hlld_f64, hlld_f32}. */
 /* The surrogate in the device-resident FP32 face path (fvm_fused_*, fvm_gpu.h):
  * after fvm_fused_setup(ctx, FVM_FUSED_FP32, 0), every later fvm_fused_run
  * evaluates each face's flux once, by this surrogate where its gate accepts the
@@ -109,6 +105,11 @@ int fvm_sg_compute_flux_device(fvm_sg *s, int n, const double *wij, const double
  * (FVM_DIAG_SG_FALLBACK). */
 typedef struct fvm_ctx fvm_ctx;
 int fvm_fused_set_surrogate(fvm_ctx *ctx, fvm_sg *sg);
+
+/* Kernel timing on device-resident copies of one batch: average milliseconds
+ * per launch of the surrogate flux (featurise + network + gate, then the
+ * fallback launch), of the plain FP64 HLLD (computeFlux in double) and of the
+ * FP32 HLLD over the same faces.  ms[3] = {surrogate, hlld_f64, hlld_f32}. */
 int fvm_sg_time_flux(fvm_sg *s, int n, const double *wij, const double *Wi, const double *Wj, const int *jbnd,
                      const double *normal, int iters, double *ms, int *n_fallback);
 
