@@ -174,25 +174,66 @@ def substep_cpu_baseline(m, W0, dtmax, nsub, budget_s=15.0):
                       f"oracle/fvm_oracle.c FP64, {wall:.1f} s"}
 
 
-def substep_bench(nsub, n=2_097_152, dtmax=1.0 / 128, host_driver=True, cpu=True):
-    """Individual timesteps on the disk (rho ~ exp(-z^2/2H^2), Keplerian v_phi, B_z at
-    beta = 400, 1 % velocity noise; 6 rungs).  The measured driver keeps the per-cell
-    state on the device (fvm_substep_run: layout, gathers, rungs and write-back on the
-    GPU; the host keeps the rung clock and the sequential timestep-limiter sweep).
-    Reports active faces per second of wall time.  With host_driver, the host-side
-    driver (per-substep layout and full-state transfers) runs the same substeps after
-    it, for its time and as a check: its per-substep counts and final state must equal
-    the device driver's bit for bit.  dtmax = 1/128: the 6 rungs (2.4e-4 .. 7.8e-3)
-    then cover the initial CFL steps of the 2M-generator disk (4.1e-4 .. 6.8e-3); the
-    driver clamps to the finest rung, so a coarser dtmax would overstep the CFL limit."""
+# Algorithmic bytes of one k_slope_limiter<double> launch (slopeLimiter.cpp:27-237
+# + computeFluidUpdateI, computeFluidUpdate.cpp:25-75, smooth limiter, FP64
+# storage), each array it must touch counted once.  Per local cell: W, the two
+# neighbour extremes and the gradients x, y, z and Wt read (6 x 80 B), the
+# gradients and Wt written back (320 B) and the limiter written (80 B), tend,
+# position, boundary flag, neighbour flag, volume and face-end offset (52 B):
+# 1012 B.  Per active face: both cell indices, the face normal and centroid
+# (56 B), its entry of the active-face list (4 B) and its two face-end entries
+# (8 B): 68 B.
+SLOPE_LIMITER_B_PER_CELL = 1012
+SLOPE_LIMITER_B_PER_FACE = 68
+
+
+def substep_roofline(m, W0, dtmax, nsub, history):
+    """The same substeps again, from the same state, with HIP-event timing of the
+    reference-order kernels (fvm_kernel_timing waits for each timed launch, so this
+    run is not the timed one): the roofline of the substeps' dominant kernel,
+    k_slope_limiter, as algorithmic bytes over its summed launch time."""
+    from fvmhd3d_amd import gpu as G
+    from fvmhd3d_amd.substep import DeviceSubstepper
+    D = DeviceSubstepper(m, W0, dtmax=dtmax, limiter=G.LIMITER_SMOOTH)
+    D.init_rungs()
+    D.ctx.kernel_timing(True)
+    for _ in range(nsub):
+        D.substep()
+    kms = D.ctx.kernel_ms()
+    D.ctx.kernel_timing(False)
+    assert D.history == history, "the timed repeat took other substeps"
+    ms, launches = kms["k_slope_limiter"]
+    nbytes = sum(SLOPE_LIMITER_B_PER_CELL * (c[0] + c[2]) + SLOPE_LIMITER_B_PER_FACE * c[1] for c in D.local_counts)
+    D.ctx.close()
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    total = sum(v[0] for v in kms.values())
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(achieved / 8000.0, 4), "traffic": None,
+            "kernel": "k_slope_limiter<double> (slopeLimiter + computeFluidUpdateI on the active subset)",
+            "bytes": int(nbytes), "ms": round(ms, 3), "launches": launches,
+            "kernel_ms": {k: round(v[0], 3) for k, v in kms.items()}, "share_of_timed_kernels": round(ms / total, 3)}
+
+
+def substep_bench(nsub, n=128, dtmax=1.0 / 64, host_driver=True, cpu=True):
+    """Individual timesteps on the thin disk (substep.ic_thin_disk: rho ~ exp(-z^2/2H^2),
+    Keplerian v_phi, c_s = H Omega, B_z at beta = 400, 1 % velocity noise) on n^3 =
+    2,097,152 jittered lattice sites, 6 rungs.  dtmax = 1/64: the initial CFL steps
+    then span the six rungs with none below the finest, 81 % of the cells sit on the
+    coarsest rung, and each of the first 16 substeps updates 1.8-19 % of the faces.
+    The measured driver keeps the per-cell state on the device (fvm_substep_run:
+    layout, gathers, rungs and write-back on the GPU; the host keeps the rung clock
+    and the sequential timestep-limiter sweep).  Reports active faces per second of
+    wall time.  With host_driver, the host-side driver (per-substep layout and
+    full-state transfers) runs the same substeps after it, for its time and as a
+    check: its per-substep counts and final state must equal the device driver's
+    bit for bit."""
     from fvmhd3d_amd import gpu as G
     from fvmhd3d_amd import mesh as M
-    from fvmhd3d_amd.substep import DeviceSubstepper, GpuBackend, Substepper, ic_disk
+    from fvmhd3d_amd.substep import DeviceSubstepper, GpuBackend, Substepper, ic_thin_disk
     t = time.perf_counter()
-    pos = M.uniform_random(n, seed=123)
-    m = M.build_periodic(pos[M.morton_order(pos, (1.0, 1.0, 1.0))])
+    m = M.workload_mesh("lattice", n, seed=123)
     log(f"[substeps] mesh {m.ncell} cells, {m.nface} faces in {time.perf_counter() - t:.0f} s")
-    W0 = ic_disk(m.pos)
+    W0 = ic_thin_disk(m.pos)
     D = DeviceSubstepper(m, W0, dtmax=dtmax, limiter=G.LIMITER_SMOOTH)
     D.init_rungs()
     D.ctx.reset_diagnostics()
@@ -206,7 +247,8 @@ def substep_bench(nsub, n=2_097_152, dtmax=1.0 / 128, host_driver=True, cpu=True
            "value": sum(faces) / wall, "unit": "face-flux updates/s", "substeps": nsub, "wall_s": round(wall, 3),
            "active_face_fraction": [round(f / m.nface, 4) for f in faces],
            "rungs": int(D.nrung), "dtmax": dtmax, "diagnostics": D.ctx.diagnostics(),
-           "workload": f"disk, {m.ncell} random generators"}
+           "workload": f"thin disk, {m.ncell} jittered lattice generators"}
+    out["roofline"] = substep_roofline(m, W0, dtmax, nsub, D.history)
     if host_driver:
         S = Substepper(m, W0, dtmax=dtmax, limiter=G.LIMITER_SMOOTH)
         be = GpuBackend(S.gamma, S.courant, tuple(float(b) for b in m.box), limiter=G.LIMITER_SMOOTH)
@@ -339,7 +381,7 @@ def main():
                     help="the learned HLLD surrogate in the step against the plain FP64 path, the "
                          "configuration's step count (--verify-steps overrides it)")
     ap.add_argument("--substeps", type=int, default=0,
-                    help="individual timesteps: this many substeps of the 2M-generator disk (6 rungs)")
+                    help="individual timesteps: this many substeps of the 2M-generator thin disk (6 rungs)")
     ap.add_argument("--no-host-driver", action="store_true",
                     help="with --substeps: skip the host-layout driver run (comparison and bit-identity check)")
     args = ap.parse_args()

@@ -919,8 +919,10 @@ template <bool OFF32>
 __device__ __forceinline__ void gather_async(const float *__restrict__ rec, int jsel, float4 *buf, int lane) {
   const int rr = lane >> 4, sl = lane & 15;
   int jr[16];
+  // source lane 4k + rr: a byte address rr * 4 plus the constant 16 k, which
+  // the instruction takes as its offset
 #pragma unroll
-  for (int k = 0; k < 16; k++) jr[k] = __builtin_amdgcn_ds_bpermute((4 * k + rr) << 2, jsel);
+  for (int k = 0; k < 16; k++) jr[k] = __builtin_amdgcn_ds_bpermute((rr << 2) + 16 * k, jsel);
 #pragma unroll
   for (int k = 0; k < 16; k++) {
     // slots 13..15 of a 16-slot LDS record are unused: their lanes fetch a
@@ -929,9 +931,9 @@ __device__ __forceinline__ void gather_async(const float *__restrict__ rec, int 
     const unsigned slot0 = (unsigned)(sl ^ (4 * (k & 3) + rr));
     const unsigned slot = slot0 < (unsigned)GQ ? slot0 : slot0 - (unsigned)GQ;
     const float *src;
-    if constexpr (OFF32)
+    if constexpr (OFF32)  // record index < 2^24: one 24-bit multiply-add
       src = reinterpret_cast<const float *>(reinterpret_cast<const char *>(rec) +
-                                            ((unsigned)jr[k] * (unsigned)(REC * sizeof(float)) + 16u * slot));
+                                            (__umul24((unsigned)jr[k], (unsigned)(REC * sizeof(float))) + 16u * slot));
     else
       src = rec + 4 * slot + (size_t)(unsigned)jr[k] * REC;
     __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)(buf + 64 * k), 16, 0, 0);

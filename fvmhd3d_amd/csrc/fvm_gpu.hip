@@ -144,9 +144,33 @@ __global__ void k_faces_to_soa(const fvm_face *__restrict__ f, int n, int *s1, i
 }
 
 // ---------------------------------------------------------------------------
-// System::computeReconstruction (computeReconstruction.cpp:42-264), one thread
-// per active cell
+// System::computeReconstruction (computeReconstruction.cpp:42-264).  Two forms
+// of the cell loops here (reconstruction, slope limiter, flux scatter,
+// predictor cell update), with the same values bit for bit:
+//   k_<name>    one thread per cell, all variables in registers;
+//   k_<name>_v  one lane per (cell, variable): NF lanes per cell, CPW cells per
+//               wavefront (the last 64 - NF * CPW lanes idle).  Every lane of a
+//               cell repeats the face geometry and forms its own variable's
+//               terms in the reference's face order and operation order.
+// The per-variable form fills the chip on the small active sets of individual
+// timesteps, where one thread per cell leaves most SIMDs idle; on large sets
+// it repeats more geometry and touches more cache lines, and one thread per
+// cell is faster (ONE_THREAD_MIN_CELLS below).
 // ---------------------------------------------------------------------------
+constexpr int CPW = 64 / NF;  // cells per wavefront of the per-variable kernels
+// the cell and variable of this lane; false for the idle lanes
+__device__ __forceinline__ bool cell_var(int &cell, int &k) {
+  const int t = xgid();
+  const int sub = (t & 63) / NF;
+  k = (t & 63) - NF * sub;
+  cell = (t >> 6) * CPW + sub;
+  return sub < CPW;
+}
+// threads of a per-variable launch over n cells
+__host__ __device__ constexpr int cell_var_threads(int n) { return (n + CPW - 1) / CPW * 64; }
+// cell count from which the thread-per-cell forms are used
+constexpr int ONE_THREAD_MIN_CELLS = 131072;
+
 template <typename S>
 __global__ void __launch_bounds__(256) k_reconstruction(Cells<S> C, MeshView M, Phys P) {
   const int i = xgid();
@@ -270,6 +294,114 @@ __global__ void __launch_bounds__(256) k_reconstruction(Cells<S> C, MeshView M, 
   }
 }
 
+template <typename S>
+__global__ void __launch_bounds__(256) k_reconstruction_v(Cells<S> C, MeshView M, Phys P) {
+  int i, k;
+  if (!cell_var(i, k) || i >= C.nactive) return;
+  const int np = C.np;
+  const int base = (int)__lane_id() - k;  // the lane of this cell's variable 0
+  const double Wi = FLD(C.w, k, i, np);
+  const double ipos[3] = {FLD(C.pos, 0, i, np), FLD(C.pos, 1, i, np), FLD(C.pos, 2, i, np)};
+
+  if (C.bnd[i] != NO_BOUNDARY) {
+    FLD(C.x, k, i, np) = (S)0.0;
+    FLD(C.y, k, i, np) = (S)0.0;
+    FLD(C.z, k, i, np) = (S)0.0;
+    FLD(C.t, k, i, np) = (S)0.0;
+    return;
+  }
+
+  double Wx = 0.0, Wy = 0.0, Wz = 0.0;
+  S dmin = (S)0.0, dmax = (S)0.0;
+  const int fb = M.coff[i], fe = M.coff[i + 1];
+  bool zero_flag = false;
+  const double dtI = C.tend[i] - P.t_global;
+
+  auto face_dri = [&](int f, double *dri) {
+    for (int d = 0; d < 3; d++) {
+      double v = GEO_C(M, d, f) - ipos[d];
+      if (v > 0.5 * P.L[d]) v -= P.L[d];
+      else if (v < -0.5 * P.L[d]) v += P.L[d];
+      dri[d] = v;
+    }
+  };
+
+  for (int e = fb; e < fe; e++) {
+    const int f = M.cface[e];
+    double cen[3];
+    face_dri(f, cen);
+    const double fn[3] = {GEO_N(M, 0, f), GEO_N(M, 1, f), GEO_N(M, 2, f)};
+    const double area = dsqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+    const double cn = cen[0] * fn[0] + cen[1] * fn[1] + cen[2] * fn[2];
+    const double sc = (cn < 0.0) ? (-1.0 / area) : (1.0 / area);
+    const double normal[3] = {fn[0] * sc, fn[1] * sc, fn[2] * sc};
+    const double dsh = cen[0] * normal[0] + cen[1] * normal[1] + cen[2] * normal[2];
+    const double ids = (dsh != 0.0) ? 0.5 / dsh : 0.0;
+    const double idsA = area * ids;
+    const double drh[3] = {normal[0] * dsh, normal[1] * dsh, normal[2] * dsh};
+    const double fij[3] = {cen[0] - drh[0], cen[1] - drh[1], cen[2] - drh[2]};
+    const int s1 = M.s1[f];
+    const int j = (i == s1) ? M.s2[f] : s1;
+    const int bj = C.bnd[j];
+    if (bj != NO_BOUNDARY && bj != INFLOW) {
+      zero_flag = true;
+      break;
+    }
+    const double wj = FLD(C.w, k, j, np);
+    const double sum = wj + Wi;
+    const double diff = wj - Wi;
+    dmin = (S)smin((double)dmin, diff);
+    dmax = (S)smax((double)dmax, diff);
+    Wx += (sum * drh[0] + diff * fij[0]) * idsA;
+    Wy += (sum * drh[1] + diff * fij[1]) * idsA;
+    Wz += (sum * drh[2] + diff * fij[2]) * idsA;
+  }
+
+  if (zero_flag) {
+    FLD(C.x, k, i, np) = (S)0.0;
+    FLD(C.y, k, i, np) = (S)0.0;
+    FLD(C.z, k, i, np) = (S)0.0;
+    FLD(C.t, k, i, np) = (S)0.0;
+    return;
+  }
+
+  const double invV = 1.0 / M.cvol[i];
+  Wx *= invV;
+  Wy *= invV;
+  Wz *= invV;
+  // J = curl B from the B lanes of this cell
+  const double yBZ = __shfl(Wy, base + BZ), zBY = __shfl(Wz, base + BY);
+  const double zBX = __shfl(Wz, base + BX), xBZ = __shfl(Wx, base + BZ);
+  const double xBY = __shfl(Wx, base + BY), yBX = __shfl(Wy, base + BX);
+  if (k < 3) FLD(C.J, k, i, np) = (S)(k == 0 ? yBZ - zBY : k == 1 ? zBX - xBZ : xBY - yBX);
+
+  double limiter_min = 1.0;
+  if (P.limiter != LIM_SHIPPED) {
+    // the shipped Phi() returns 1 for every argument, so the face loop is
+    // only evaluated for the opt-in limiters
+    for (int e = fb; e < fe; e++) {
+      const int f = M.cface[e];
+      double cen[3];
+      face_dri(f, cen);
+      const int s1 = M.s1[f];
+      const int j = (i == s1) ? M.s2[f] : s1;
+      const double dtJ = C.tend[j] - P.t_global;
+      const double dtij = dtI - 0.5 * smin(dtI, dtJ);
+      const double dWx = Wx * cen[0] + Wy * cen[1] + Wz * cen[2];
+      const double dWt = 0.0 * (double)FLD(C.t, k, i, np) * dtij;
+      const double dWi = dWx + dWt;
+      limiter_min = smin(limiter_min, limiter_rec(P.limiter, dWi, (double)dmin, (double)dmax));
+    }
+  }
+  FLD(C.mm_lo, k, i, np) = (S)((double)dmin + Wi);
+  FLD(C.mm_hi, k, i, np) = (S)((double)dmax + Wi);
+  const double tau = 1.0 * limiter_min;
+  FLD(C.t, k, i, np) = (S)((double)FLD(C.t, k, i, np) * tau);
+  FLD(C.x, k, i, np) = (S)(Wx * tau);
+  FLD(C.y, k, i, np) = (S)(Wy * tau);
+  FLD(C.z, k, i, np) = (S)(Wz * tau);
+}
+
 // ---------------------------------------------------------------------------
 // slopeLimiter compute part (slopeLimiter.cpp:27-237) per cell over its active
 // face-ends, then computeFluidUpdateI (computeFluidUpdate.cpp:25-75)
@@ -350,6 +482,60 @@ __global__ void __launch_bounds__(256) k_slope_limiter(Cells<S> C, MeshView M, P
       FLD(C.z, k, c, np) = (S)((double)FLD(C.z, k, c, np) * f);
       FLD(C.t, k, c, np) = (S)((double)FLD(C.t, k, c, np) * f);
     }
+  }
+}
+
+// one lane per (cell, variable): every lane repeats the face-end geometry and
+// applies the limiter to its own variable
+template <typename S>
+__global__ void __launch_bounds__(256) k_slope_limiter_v(Cells<S> C, MeshView M, Phys P, unsigned long long *diag) {
+  int c, k;
+  if (!cell_var(c, k) || c >= C.np) return;
+  const int np = C.np;
+  S lim = (S)1.0;
+  double at[3] = {0.0, 0.0, 0.0};
+  const double wc = FLD(C.w, k, c, np);
+  const double mlo = (double)FLD(C.mm_lo, k, c, np);
+  const double mhi = (double)FLD(C.mm_hi, k, c, np);
+  for (int e = M.soff[c]; e < M.soff[c + 1]; e++) {
+    const int ent = M.sent[e];
+    const int a = ent >> 1, side = ent & 1;
+    const int f = M.fact[a];
+    const int i = M.s1[f], j = M.s2[f];
+    const double dtI = C.tend[i] - P.t_global;
+    const double dtJ = C.tend[j] - P.t_global;
+    const double dth = 0.5 * smin(dtI, dtJ);
+    const double pos_i[3] = {FLD(C.pos, 0, i, np), FLD(C.pos, 1, i, np), FLD(C.pos, 2, i, np)};
+    FaceGeo g;
+    face_geo(M, pos_i, f, np, g);
+    for (int d = 0; d < 3; d++) {
+      const double an = g.normal[d] * g.area;
+      at[d] = side == 0 ? at[d] + an : at[d] - an;
+    }
+    if (C.bnd[i] != NO_BOUNDARY || C.bnd[j] != NO_BOUNDARY) {
+      lim = (S)0.0;
+      continue;
+    }
+    if (P.limiter == LIM_SHIPPED) continue;  // Phi() == 1: min(lim, 1) leaves lim unchanged
+    const double *dr = side == 0 ? g.dri : g.drj;
+    const double dtt = side == 0 ? dtI - dth : dtJ - dth;
+    const double dWt = (double)FLD(C.t, k, c, np) * dtt;
+    const double dWx = (double)FLD(C.x, k, c, np) * dr[0] + (double)FLD(C.y, k, c, np) * dr[1] +
+                       (double)FLD(C.z, k, c, np) * dr[2];
+    const double dW = dWx + dWt;
+    lim = (S)smin((double)lim, limiter_sl(P.limiter, dW, smin(mlo - wc, 0.0), smax(mhi - wc, 0.0)));
+  }
+  if (k == 0 && c < C.nactive && C.bnd[c] == NO_BOUNDARY) {
+    const double aabs = dsqrt(at[0] * at[0] + at[1] * at[1] + at[2] * at[2]);
+    if (!(aabs < 1.0e-10 * pow(M.cvol[c], 2.0 / 3.0))) atomicAdd(&diag[FVM_DIAG_CLOSURE], 1ULL);
+  }
+  FLD(C.lim, k, c, np) = lim;
+  if (c < C.nactive || C.is_ngb[c]) {
+    const double f = 1.0 * (double)lim;
+    FLD(C.x, k, c, np) = (S)((double)FLD(C.x, k, c, np) * f);
+    FLD(C.y, k, c, np) = (S)((double)FLD(C.y, k, c, np) * f);
+    FLD(C.z, k, c, np) = (S)((double)FLD(C.z, k, c, np) * f);
+    FLD(C.t, k, c, np) = (S)((double)FLD(C.t, k, c, np) * f);
   }
 }
 
@@ -503,6 +689,35 @@ __global__ void __launch_bounds__(256) k_scatter(Cells<S> C, MeshView M, const d
   FLD(C.ddivB, 0, c, np) = FLD(C.ddivB, 0, c, np) + dv;
   for (int d = 0; d < 3; d++) FLD(C.dgp, d, c, np) = FLD(C.dgp, d, c, np) + gp[d];
   for (int k = 0; k < NF; k++) FLD(C.dU, k, c, np) = FLD(C.dU, k, c, np) + U[k];
+}
+
+// one lane per (cell, variable); lanes 0-2 also sum the gradPsi terms and
+// lane 3 the div B term
+template <typename S>
+__global__ void __launch_bounds__(256) k_scatter_v(Cells<S> C, MeshView M, const double *rec, const int *swp) {
+  int c, k;
+  if (!cell_var(c, k) || c >= C.np) return;
+  const int np = C.np;
+  S U = (S)0.0, gx = (S)0.0;  // gx: gradPsi[k] (k < 3) or div B (k == 3)
+  for (int e = M.soff[c]; e < M.soff[c + 1]; e++) {
+    const int ent = M.sent[e];
+    const int a = ent >> 1, side = ent & 1;
+    const bool is_i = (side == 0) != (swp[a] != 0);
+    const double r = REC(rec, k, a, NREC_F);
+    const double x = k < 4 ? REC(rec, NF + k, a, NREC_F) : 0.0;
+    if (is_i) {
+      U = (S)((double)U - r);
+      if (k < 3) gx = gx + (S)x;
+      else if (k == 3) gx = (S)((double)gx + x);
+    } else {
+      U = (S)((double)U + r);
+      if (k < 3) gx = gx - (S)x;
+      else if (k == 3) gx = (S)((double)gx - x);
+    }
+  }
+  if (k == 3) FLD(C.ddivB, 0, c, np) = FLD(C.ddivB, 0, c, np) + gx;
+  if (k < 3) FLD(C.dgp, k, c, np) = FLD(C.dgp, k, c, np) + gx;
+  FLD(C.dU, k, c, np) = FLD(C.dU, k, c, np) + U;
 }
 
 // ---------------------------------------------------------------------------
@@ -820,6 +1035,88 @@ __global__ void __launch_bounds__(256) k_predictor_cell(Cells<S> C, MeshView M, 
   for (int k = 0; k < NF; k++) FLD(C.t, k, c, np) = (S)((W1[k] - W0[k]) * idt);
 }
 
+// The face sums take one lane per (cell, variable) (lanes 0-2 also gradPsi,
+// lane 3 div B, lane 4 the volume change); the cell update then runs in every
+// lane of the cell on the sums gathered from its lanes, and lane k stores Wt[k].
+template <typename S>
+__global__ void __launch_bounds__(256) k_predictor_cell_v(Cells<S> C, MeshView M, Phys P, const double *rec,
+                                        const int *swp, double dtmax, int min_rung,
+                                        unsigned long long *diag) {
+  int c, k;
+  if (!cell_var(c, k) || c >= C.nactive) return;
+  const int np = C.np;
+  const int base = (int)__lane_id() - k;
+  S dUk = (S)0.0, gx = (S)0.0;  // gx: gradPsi[k] (k < 3) or div B (k == 3)
+  double dVk = 0.0;             // lane 4: the volume change
+  for (int e = M.soff[c]; e < M.soff[c + 1]; e++) {
+    const int ent = M.sent[e];
+    const int a = ent >> 1, side = ent & 1;
+    const bool sw = swp[a] != 0;
+    const bool is_i = (side == 0) != sw;
+    const double r = REC(rec, k, a, NREC_P);
+    const double x = k < 5 ? REC(rec, NF + k, a, NREC_P) : 0.0;
+    if (is_i) {
+      dUk = (S)((double)dUk - r);
+      if (k < 3) gx = gx + (S)x;
+      else if (k == 3) gx = (S)((double)gx + x);
+      else if (k == 4) dVk += x;
+    } else {
+      // j receives only when not a boundary (computePredictor.cpp:127-128);
+      // every cell of this kernel is active
+      if (C.bnd[c] != NO_BOUNDARY) continue;
+      if (k == 4) dVk -= x;
+      dUk = (S)((double)dUk + r);
+      if (k < 3) gx = gx - (S)x;
+      else if (k == 3) gx = (S)((double)gx - x);
+    }
+  }
+  if (C.bnd[c] != NO_BOUNDARY) return;
+  // every lane of the cell gathers the sums
+  double dU[NF];
+  for (int m = 0; m < NF; m++) dU[m] = (double)__shfl((double)dUk, base + m);
+  const double dPsi[3] = {(double)__shfl((double)gx, base), (double)__shfl((double)gx, base + 1),
+                          (double)__shfl((double)gx, base + 2)};
+  const double divB = __shfl((double)gx, base + 3);
+  const double dV = __shfl(dVk, base + 4);
+  const double g = P.gamma;
+  double U0[NF], W0[NF];
+  for (int m = 0; m < NF; m++) U0[m] = FLD(C.U, m, c, np);
+  to_primitive(U0, M.cvol[c], W0);
+  double dtn = C.dt_new[c];
+  while (U0[MASS] + dU[MASS] * dtn <= 0.0) dtn *= 0.5;
+  if (k == 0) C.dt_new[c] = dtn;
+  const double dt = rung_dt(dtmax, min_rung, dtn) / 128;
+  double Uc[NF];
+  for (int m = 0; m < NF; m++) Uc[m] = U0[m] + dU[m] * dt;
+  Uc[WBX] -= W0[VELX] * divB * dt;
+  Uc[WBY] -= W0[VELY] * divB * dt;
+  Uc[WBZ] -= W0[VELZ] * divB * dt;
+  const double pres = pressure(g, W0[ETHM]);
+  const double B2 = W0[BX] * W0[BX] + W0[BY] * W0[BY] + W0[BZ] * W0[BZ];
+  const double dcs2 = g * pres + B2;
+  const double ch_sig = dsqrt(dcs2 / W0[DENS]);
+  Uc[WBX] -= dPsi[0] * dt;
+  Uc[WBY] -= dPsi[1] * dt;
+  Uc[WBZ] -= dPsi[2] * dt;
+  Uc[MPSI] -= dcs2 * divB * dt;
+  const double cr = 0.5;
+  const double l_min = pow(M.cvol[c] / (4.0 * REF_PI / 3.0), 1.0 / 3.0);
+  Uc[MPSI] *= exp(-cr * ch_sig / l_min * dt);
+  double W1[NF];
+  to_primitive(Uc, M.cvol[c] * exp(dV * dt / M.cvol[c]), W1);
+  if (W1[ETHM] <= 0.0) {
+    W1[ETHM] = ethm_from_entropy(g, W1);
+    if (k == 0) atomicAdd(&diag[FVM_DIAG_ETHM_FAILED], 1ULL);
+  } else {
+    W1[ENTR] = entropy_from_ethm(g, W1);
+  }
+  const double idt = 1.0 / dt;
+  double out = 0.0;
+  for (int m = 0; m < NF; m++)
+    if (m == k) out = (W1[m] - W0[m]) * idt;
+  FLD(C.t, k, c, np) = (S)out;
+}
+
 // ---------------------------------------------------------------------------
 // batched System::computeFlux (FP64)
 // ---------------------------------------------------------------------------
@@ -849,6 +1146,9 @@ __global__ void __launch_bounds__(256) k_flux_batch(int n, double gamma, const d
 __global__ void k_fill_int(int *p, int v, int n) {
   const int i = gid();
   if (i < n) p[i] = v;
+}
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) {
+  if (gid() == 0) *p = v;
 }
 
 // Cell lists from the face list as localMesh.cpp:586-589 builds them: every
@@ -1053,6 +1353,12 @@ int fvm_fail(fvm_ctx *c, int code, const std::string &msg) {
 static int fail(fvm_ctx *c, int code, const std::string &msg) { return fvm_fail(c, code, msg); }
 #define HIPCHK(c, call) FVM_HIPCHK(c, call)
 
+// the end of an entry point: wait for its work unless a substep defers it
+static int finish(fvm_ctx *c) {
+  if (!c->defer_sync) HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FVM_OK;
+}
+
 int fvm_ensure(fvm_ctx *c, DBuf &b, size_t bytes) {
   if (b.bytes >= bytes && b.p) return FVM_OK;
   if (b.p) HIPCHK(c, hipFree(b.p));
@@ -1160,6 +1466,22 @@ static inline unsigned nblk(long n, int bs = 256) { return (unsigned)((n + bs - 
     }                                                                                  \
   } while (0)
 
+// LAUNCH_X timed as kernel `id` when fvm_kernel_timing is on: events around the
+// launch on the context's stream, waited for, the elapsed time added up
+#define LAUNCH_T(c, id, kern, n, ...)                                                   \
+  do {                                                                                  \
+    if ((n) > 0 && (c)->kt.on) HIPCHK(c, hipEventRecord((c)->kt.ev[0], (c)->stream));   \
+    LAUNCH_X(c, kern, n, __VA_ARGS__);                                                  \
+    if ((n) > 0 && (c)->kt.on) {                                                        \
+      HIPCHK(c, hipEventRecord((c)->kt.ev[1], (c)->stream));                            \
+      HIPCHK(c, hipEventSynchronize((c)->kt.ev[1]));                                    \
+      float ms_ = 0.0f;                                                                 \
+      HIPCHK(c, hipEventElapsedTime(&ms_, (c)->kt.ev[0], (c)->kt.ev[1]));               \
+      (c)->kt.ms[id] += ms_;                                                            \
+      (c)->kt.launches[id] += 1;                                                        \
+    }                                                                                   \
+  } while (0)
+
 static int check_params(const fvm_params *p, std::string &why) {
   if (!p) { why = "params is NULL"; return FVM_EARG; }
   if (!(p->gamma_gas > 0.0)) { why = "gamma_gas must be > 0"; return FVM_EARG; }
@@ -1223,8 +1545,33 @@ int fvm_destroy(fvm_ctx *c) {
   fvm::scan_release(c->scan);
   fvm_fast_release(c);
   fvm_substep_release(c);
+  for (hipEvent_t &e : c->kt.ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+  return FVM_OK;
+}
+
+int fvm_kernel_timing(fvm_ctx *c, int enable) {
+  if (!c) return FVM_EARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  for (hipEvent_t &e : c->kt.ev)
+    if (!e) HIPCHK(c, hipEventCreate(&e));
+  c->kt.on = enable != 0;
+  for (int k = 0; k < FVM_NKT; k++) {
+    c->kt.ms[k] = 0.0;
+    c->kt.launches[k] = 0;
+  }
+  return FVM_OK;
+}
+
+int fvm_kernel_ms(fvm_ctx *c, double *ms, long long *launches, int n) {
+  if (!c) return FVM_EARG;
+  if (n < 0 || n > FVM_NKT || (n > 0 && (!ms || !launches))) return fvm_fail(c, FVM_EARG, "fvm_kernel_ms: bad n or null output");
+  for (int k = 0; k < n; k++) {
+    ms[k] = c->kt.ms[k];
+    launches[k] = c->kt.launches[k];
+  }
   return FVM_OK;
 }
 
@@ -1500,7 +1847,7 @@ int fvm_get_active_faces(fvm_ctx *c, int *nface_active) {
   c->nfa = total;
   rc = build_face_ends(c);
   if (rc) return rc;
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if ((rc = finish(c))) return rc;
   *nface_active = total;
   return FVM_OK;
 }
@@ -1536,9 +1883,12 @@ int fvm_download_active_faces(fvm_ctx *c, int *face_active, int n) {
   ((c)->prm.storage == FVM_STORAGE_SHIPPED ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
 
 template <typename S> static int do_reconstruction(fvm_ctx *c) {
-  LAUNCH_X(c, fvm::k_reconstruction<S>, c->nactive, cells<S>(c), meshview(c), phys(c));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return FVM_OK;
+  if (c->nactive >= fvm::ONE_THREAD_MIN_CELLS)
+    LAUNCH_T(c, FVM_KT_RECONSTRUCTION, fvm::k_reconstruction<S>, c->nactive, cells<S>(c), meshview(c), phys(c));
+  else
+    LAUNCH_T(c, FVM_KT_RECONSTRUCTION, fvm::k_reconstruction_v<S>, fvm::cell_var_threads(c->nactive), cells<S>(c),
+             meshview(c), phys(c));
+  return finish(c);
 }
 int fvm_compute_reconstruction(fvm_ctx *c) {
   if (!c) return FVM_EARG;
@@ -1548,9 +1898,13 @@ int fvm_compute_reconstruction(fvm_ctx *c) {
 }
 
 template <typename S> static int do_limiter(fvm_ctx *c) {
-  LAUNCH_X(c, fvm::k_slope_limiter<S>, c->np, cells<S>(c), meshview(c), phys(c), (unsigned long long *)c->diag.p);
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return FVM_OK;
+  if (c->np >= fvm::ONE_THREAD_MIN_CELLS)
+    LAUNCH_T(c, FVM_KT_SLOPE_LIMITER, fvm::k_slope_limiter<S>, c->np, cells<S>(c), meshview(c), phys(c),
+             (unsigned long long *)c->diag.p);
+  else
+    LAUNCH_T(c, FVM_KT_SLOPE_LIMITER, fvm::k_slope_limiter_v<S>, fvm::cell_var_threads(c->np), cells<S>(c), meshview(c),
+             phys(c), (unsigned long long *)c->diag.p);
+  return finish(c);
 }
 int fvm_slope_limiter(fvm_ctx *c) {
   if (!c) return FVM_EARG;
@@ -1571,10 +1925,16 @@ template <typename S> static int do_fluid_update(fvm_ctx *c) {
   if ((rc = ensure(c, c->rec, sizeof(double) * fvm::NREC_P * (size_t)nfa)) || (rc = ensure(c, c->swp, sizeof(int) * (size_t)nfa)))
     return rc;
   unsigned long long before[FVM_NDIAG], after[FVM_NDIAG];
-  if ((rc = read_diag(c, before))) return rc;
-  LAUNCH_X(c, fvm::k_face_flux<S>, nfa, cells<S>(c), meshview(c), phys(c), (double *)c->rec.p, (int *)c->swp.p,
+  if (!c->defer_sync && (rc = read_diag(c, before))) return rc;
+  LAUNCH_T(c, FVM_KT_FACE_FLUX, fvm::k_face_flux<S>, nfa, cells<S>(c), meshview(c), phys(c), (double *)c->rec.p, (int *)c->swp.p,
          (unsigned long long *)c->diag.p);
-  LAUNCH_X(c, fvm::k_scatter<S>, c->np, cells<S>(c), meshview(c), (const double *)c->rec.p, (const int *)c->swp.p);
+  if (c->np >= fvm::ONE_THREAD_MIN_CELLS)
+    LAUNCH_T(c, FVM_KT_SCATTER, fvm::k_scatter<S>, c->np, cells<S>(c), meshview(c), (const double *)c->rec.p,
+             (const int *)c->swp.p);
+  else
+    LAUNCH_T(c, FVM_KT_SCATTER, fvm::k_scatter_v<S>, fvm::cell_var_threads(c->np), cells<S>(c), meshview(c),
+             (const double *)c->rec.p, (const int *)c->swp.p);
+  if (c->defer_sync) return FVM_OK;  // the substep compares the positivity count itself
   if ((rc = read_diag(c, after))) return rc;
   if (after[FVM_DIAG_POSITIVITY] != before[FVM_DIAG_POSITIVITY])
     return fail(c, FVM_EPOS, "computeFluidUpdate: non-positive density or thermal energy at a face");
@@ -1618,9 +1978,8 @@ int fvm_get_face_states(fvm_ctx *c, int first, int count, double *wij, double *W
 }
 
 template <typename S> static int do_primitives(fvm_ctx *c, int flag) {
-  LAUNCH_X(c, fvm::k_primitives<S>, c->nactive, cells<S>(c), meshview(c), phys(c), flag, (unsigned long long *)c->diag.p);
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return FVM_OK;
+  LAUNCH_T(c, FVM_KT_PRIMITIVES, fvm::k_primitives<S>, c->nactive, cells<S>(c), meshview(c), phys(c), flag, (unsigned long long *)c->diag.p);
+  return finish(c);
 }
 int fvm_compute_primitives(fvm_ctx *c, int predictor_flag) {
   if (!c) return FVM_EARG;
@@ -1631,9 +1990,9 @@ int fvm_compute_primitives(fvm_ctx *c, int predictor_flag) {
 
 template <typename S> static int do_timestep(fvm_ctx *c, double *dt_min) {
   unsigned long long *bits = (unsigned long long *)c->scalars.p;
-  const unsigned long long init = 0x7FF0000000000000ULL;  // +inf
-  HIPCHK(c, hipMemcpyAsync(bits, &init, sizeof init, hipMemcpyHostToDevice, c->stream));
-  LAUNCH_X(c, fvm::k_timestep<S>, c->nactive, cells<S>(c), meshview(c), phys(c), bits);
+  LAUNCH(c, fvm::k_set_u64, 1, bits, 0x7FF0000000000000ULL);  // +inf
+  LAUNCH_T(c, FVM_KT_TIMESTEP, fvm::k_timestep<S>, c->nactive, cells<S>(c), meshview(c), phys(c), bits);
+  if (c->defer_sync && !dt_min) return FVM_OK;
   unsigned long long h = 0;
   HIPCHK(c, hipMemcpyAsync(&h, bits, sizeof h, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1692,12 +2051,16 @@ template <typename S> static int do_predictor(fvm_ctx *c, double dtmax, int min_
   int rc;
   if ((rc = ensure(c, c->rec, sizeof(double) * fvm::NREC_P * (size_t)nfa)) || (rc = ensure(c, c->swp, sizeof(int) * (size_t)nfa)))
     return rc;
-  LAUNCH_X(c, fvm::k_predictor_face<S>, nfa, cells<S>(c), meshview(c), phys(c), (double *)c->rec.p, (int *)c->swp.p,
+  LAUNCH_T(c, FVM_KT_PREDICTOR_FACE, fvm::k_predictor_face<S>, nfa, cells<S>(c), meshview(c), phys(c), (double *)c->rec.p, (int *)c->swp.p,
          (unsigned long long *)c->diag.p);
-  LAUNCH_X(c, fvm::k_predictor_cell<S>, c->nactive, cells<S>(c), meshview(c), phys(c), (const double *)c->rec.p,
-         (const int *)c->swp.p, dtmax, min_rung, (unsigned long long *)c->diag.p);
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return FVM_OK;
+  if (c->nactive >= fvm::ONE_THREAD_MIN_CELLS)
+    LAUNCH_T(c, FVM_KT_PREDICTOR_CELL, fvm::k_predictor_cell<S>, c->nactive, cells<S>(c), meshview(c), phys(c),
+             (const double *)c->rec.p, (const int *)c->swp.p, dtmax, min_rung, (unsigned long long *)c->diag.p);
+  else
+    LAUNCH_T(c, FVM_KT_PREDICTOR_CELL, fvm::k_predictor_cell_v<S>, fvm::cell_var_threads(c->nactive), cells<S>(c),
+             meshview(c), phys(c), (const double *)c->rec.p, (const int *)c->swp.p, dtmax, min_rung,
+             (unsigned long long *)c->diag.p);
+  return finish(c);
 }
 int fvm_compute_predictor(fvm_ctx *c, double dtmax, int min_rung) {
   if (!c) return FVM_EARG;

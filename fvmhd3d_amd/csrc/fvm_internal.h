@@ -93,7 +93,8 @@ struct SubstepState {
   int *hoth = nullptr;          // pinned, n: the hot cells
   int *chgi = nullptr;          // pinned, n: changed cells
   double *chgv = nullptr;       // pinned, n: their values
-  int *errh = nullptr;          // pinned: the layout's error flag
+  int *errh = nullptr;          // pinned, 2: the layout's error flag, the rung update's
+  unsigned long long *posh = nullptr;  // pinned, 2: positivity count before and after the fluid update
 };
 
 struct fvm_ctx {
@@ -105,6 +106,10 @@ struct fvm_ctx {
   int np = 0, nactive = 0, nimport_loc = 0, nface = 0, nref = 0, nfa = -1;
   bool have_mesh = false;
   bool lists_complete = false;  // every face listed under each of its active cells
+  // set while fvm_substep_run drives the entry points: they leave their final
+  // synchronisation (and the fluid update its positivity read-back) to the
+  // substep, which synchronises once at its end
+  bool defer_sync = false;
   DBuf field[FVM_NFIELDS];
   DBuf m_s1, m_s2, m_c, m_n, m_coff, m_cface, m_cvol, m_fact, m_soff, m_sent;
   DBuf m_cn;  // per-face geometry record (centroid, n) of the same faces as m_c / m_n
@@ -114,6 +119,14 @@ struct fvm_ctx {
   fvm::ScanScratch scan;
   FusedState fz;
   SubstepState sub;
+  // HIP-event timing of the reference-order kernels (fvm_kernel_timing): per
+  // kernel FVM_KT_*, total milliseconds and launches since timing was enabled
+  struct {
+    bool on = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double ms[FVM_NKT] = {};
+    long long launches[FVM_NKT] = {};
+  } kt;
 };
 
 int fvm_fail(fvm_ctx *c, int code, const std::string &msg);

@@ -532,7 +532,7 @@ void fvm_substep_release(fvm_ctx *c) {
     fvm_release(*b);
   for (auto &b : s.g) fvm_release(b);
   for (void *h : {(void *)s.dtih, (void *)s.rankh, (void *)s.sdh, (void *)s.permh, (void *)s.hoth, (void *)s.chgi,
-                  (void *)s.chgv, (void *)s.errh})
+                  (void *)s.chgv, (void *)s.errh, (void *)s.posh})
     if (h) (void)hipHostFree(h);
   s = SubstepState{};
 }
@@ -549,12 +549,6 @@ int ensure_all(fvm_ctx *c, std::initializer_list<std::pair<DBuf *, size_t>> l) {
     int rc = fvm_ensure(c, *e.first, e.second ? e.second : 4);
     if (rc) return rc;
   }
-  return FVM_OK;
-}
-
-int read_err(fvm_ctx *c, int *e) {
-  SCHK(c, hipMemcpyAsync(e, c->sub.err.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  SCHK(c, hipStreamSynchronize(c->stream));
   return FVM_OK;
 }
 
@@ -682,9 +676,8 @@ int update_rungs(fvm_ctx *c, double dtmax, int min_rung, int nrung, double t_glo
   SLAUNCH(c, c->nactive, k_sub_rungs, P<double>(c->field[FVM_FIELD_DT_NEW]), P<double>(c->field[FVM_FIELD_TBEG]),
           P<double>(c->field[FVM_FIELD_TEND]), P<int>(s.perm), c->nactive, exponent_of(dtmax), min_rung, nrung, dt_tick,
           t_global, P<int>(s.rung), P<int>(s.err));
-  int e = 0;
-  if (int rc = read_err(c, &e)) return rc;
-  if (e & 2) return fvm_fail(c, FVM_ESTATE, "substep: a timestep is below the finest rung of the clock");
+  // read back with the substep's final synchronisation
+  SCHK(c, hipMemcpyAsync(s.errh + 1, s.err.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   return FVM_OK;
 }
 
@@ -774,7 +767,8 @@ int substep_sweep(fvm_ctx *c) {
     SCHK(c, hipMemcpyAsync(dv, s.chgv, 8 * (size_t)nchg, hipMemcpyHostToDevice, st));
     SLAUNCH(c, nchg, k_sub_apply_changes, di, dv, nchg, P<int>(s.g2l), P<double>(c->field[FVM_FIELD_DT_NEW]));
   }
-  SCHK(c, hipStreamSynchronize(st));
+  // the copies from chgi / chgv complete before the next substep's host sweep
+  // writes them again: every substep synchronises at its end
   return FVM_OK;
 }
 
@@ -842,8 +836,10 @@ int fvm_substep_setup(fvm_ctx *c, int ncell, int nface, const fvm_face *faces, c
   SCHK(c, hipHostMalloc((void **)&s.hoth, 4 * N, hipHostMallocDefault));
   SCHK(c, hipHostMalloc((void **)&s.chgi, 4 * N, hipHostMallocDefault));
   SCHK(c, hipHostMalloc((void **)&s.chgv, 8 * N, hipHostMallocDefault));
-  SCHK(c, hipHostMalloc((void **)&s.errh, sizeof(int), hipHostMallocDefault));
-  *s.errh = 0;
+  SCHK(c, hipHostMalloc((void **)&s.errh, 2 * sizeof(int), hipHostMallocDefault));
+  SCHK(c, hipHostMalloc((void **)&s.posh, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+  s.errh[0] = s.errh[1] = 0;
+  s.posh[0] = s.posh[1] = 0;
   for (size_t i = 0; i < N; i++) s.rankh[i] = -1;
   try {
     s.hoff.assign(cell_face_off, cell_face_off + N + 1);
@@ -907,13 +903,18 @@ int fvm_substep_setup(fvm_ctx *c, int ncell, int nface, const fvm_face *faces, c
   std::memset(s.chgi, 0, 4 * N);
   std::memset(s.chgv, 0, 8 * N);
   std::memset(s.dtih, 0, 8 * N);
+  // the same asynchronous copies on the context's stream as the substeps make
   const void *zero = s.g[FVM_FIELD_DT_NEW].p;  // 8n zero bytes
-  SCHK(c, hipMemcpy(s.dtih, zero, 8 * N, hipMemcpyDeviceToHost));
-  SCHK(c, hipMemcpy(s.sdh, zero, 8 * N, hipMemcpyDeviceToHost));
-  SCHK(c, hipMemcpy(s.permh, zero, 4 * N, hipMemcpyDeviceToHost));
-  SCHK(c, hipMemcpy(s.hoth, zero, 4 * N, hipMemcpyDeviceToHost));
-  SCHK(c, hipMemcpy(s.chg.p, s.chgi, 4 * N, hipMemcpyHostToDevice));
-  SCHK(c, hipMemcpy(s.chg.p, s.chgv, 8 * N, hipMemcpyHostToDevice));
+  std::vector<int> minus1(N, -1);
+  SCHK(c, hipMemcpyAsync(s.dtih, zero, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  SCHK(c, hipMemcpyAsync(s.sdh, zero, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  SCHK(c, hipMemcpyAsync(s.permh, zero, 4 * N, hipMemcpyDeviceToHost, c->stream));
+  SCHK(c, hipMemcpyAsync(s.hoth, zero, 4 * N, hipMemcpyDeviceToHost, c->stream));
+  SCHK(c, hipMemcpyAsync(s.rankh, zero, 4 * N, hipMemcpyDeviceToHost, c->stream));
+  SCHK(c, hipMemcpyAsync(s.chg.p, s.chgi, 4 * N, hipMemcpyHostToDevice, c->stream));
+  SCHK(c, hipMemcpyAsync(s.chg.p, s.chgv, 8 * N, hipMemcpyHostToDevice, c->stream));
+  SCHK(c, hipStreamSynchronize(c->stream));
+  std::memcpy(s.rankh, minus1.data(), 4 * N);  // -1 everywhere between substeps
   s.n = ncell;
   s.nface = nface;
   s.nref = nref;
@@ -992,30 +993,41 @@ int fvm_substep_run(fvm_ctx *c, unsigned rung_mask, double t_global, double dtma
   if (!(dtmax > 0.0) || nrung < 1 || nrung > kRungMax - 1 || min_rung < 0 || min_rung >= kRungMax)
     return fvm_fail(c, FVM_EARG, "fvm_substep_run: bad dtmax, nrung or min_rung");
   int cnt[3] = {0, 0, 0}, nfa = 0, rc;
+  SubstepState &s = c->sub;
+  // the entry points below leave their synchronisations to this substep: it
+  // waits for the counts that size its launches and once at its end
+  struct Defer {
+    fvm_ctx *c;
+    ~Defer() { c->defer_sync = false; }
+  } defer{c};
+  c->defer_sync = true;
   if ((rc = build_layout(c, rung_mask, t_global, 1, cnt))) return rc;
   if ((rc = fvm_get_active_faces(c, &nfa))) return rc;
   if ((rc = fvm_compute_primitives(c, 0))) return rc;
-  double dtmin = 0.0;
-  if ((rc = fvm_compute_timestep(c, &dtmin))) return rc;
+  if ((rc = fvm_compute_timestep(c, nullptr))) return rc;
   if ((rc = substep_sweep(c))) return rc;
   if ((rc = fvm_compute_predictor(c, dtmax, min_rung))) return rc;
   if ((rc = update_rungs(c, dtmax, min_rung, nrung, t_global, nullptr))) return rc;
   if ((rc = fvm_compute_reconstruction(c))) return rc;
   if ((rc = fvm_slope_limiter(c))) return rc;
-  const int pos = fvm_compute_fluid_update(c);
-  if (pos != FVM_OK && pos != FVM_EPOS) return pos;
+  unsigned long long *diag = P<unsigned long long>(c->diag) + FVM_DIAG_POSITIVITY;
+  SCHK(c, hipMemcpyAsync(s.posh, diag, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+  if ((rc = fvm_compute_fluid_update(c))) return rc;
+  SCHK(c, hipMemcpyAsync(s.posh + 1, diag, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
   if ((rc = write_back(c, true))) return rc;
   // the active cells pushed again in the order of the active list
-  SLAUNCH(c, cnt[0], k_sub_stamp, P<int>(c->sub.perm), cnt[0], c->sub.next_seq, P<unsigned long long>(c->sub.seq));
-  c->sub.next_seq += (unsigned long long)cnt[0];
+  SLAUNCH(c, cnt[0], k_sub_stamp, P<int>(s.perm), cnt[0], s.next_seq, P<unsigned long long>(s.seq));
+  s.next_seq += (unsigned long long)cnt[0];
+  if ((rc = rung_histogram(c, rung_hist))) return rc;  // synchronises
+  if (s.errh[1] & 2) return fvm_fail(c, FVM_ESTATE, "substep: a timestep is below the finest rung of the clock");
   if (counts) {
     counts[0] = cnt[0];
     counts[1] = nfa;
     counts[2] = cnt[1];
     counts[3] = cnt[2];
-    counts[4] = pos == FVM_EPOS;
+    counts[4] = s.posh[1] != s.posh[0];
   }
-  return rung_histogram(c, rung_hist);
+  return FVM_OK;
 }
 
 int fvm_timestep_limiter_sweep(int ncell, const int *cell_off, const int *nbr, const unsigned char *active,

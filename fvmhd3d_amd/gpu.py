@@ -30,6 +30,9 @@ FIELDS = {
 SUBSTEP_RUNG = len(FIELDS)  # rung per cell in fvm_substep_upload / _download
 NDIAG = 8
 FLUX_NOALLOC_MAX = 65536  # FVM_FLUX_NOALLOC_MAX of include/fvm_gpu.h
+# reference-order kernels timed by fvm_kernel_timing, in FVM_KT_* order
+KT_NAMES = ["k_reconstruction", "k_slope_limiter", "k_face_flux", "k_scatter", "k_primitives", "k_timestep",
+            "k_predictor_face", "k_predictor_cell"]
 DIAG_NAMES = ["positivity", "closure", "ethm_failed", "mass_failed", "dsl", "sg_fallback"]
 
 
@@ -78,6 +81,8 @@ SIGNATURES = {
     "fvm_fused_sync": [_P],
     "fvm_fused_timing": [_P, _I],
     "fvm_fused_kernel_ms": [_P, _P, _P, _I],
+    "fvm_kernel_timing": [_P, _I],
+    "fvm_kernel_ms": [_P, _P, _P, _I],
     "fvm_measure_copy_bandwidth": [_P, ctypes.c_longlong, _I, ctypes.POINTER(ctypes.c_double)],
     "fvm_get_face_states": [_P, _I, _I, _P, _P, _P, _P, _P],
     "fvm_energy": [_P, _P],
@@ -343,6 +348,19 @@ class Context:
         nl = np.zeros(4, dtype=np.int64)
         self._chk(self.L.fvm_fused_kernel_ms(self.h, _ptr(ms), _ptr(nl), 4))
         return ms, nl
+
+    def kernel_timing(self, enable=True):
+        """HIP-event timing of the reference-order kernels (fvm_kernel_timing);
+        enabling it resets the totals."""
+        self._chk(self.L.fvm_kernel_timing(self.h, int(bool(enable))))
+
+    def kernel_ms(self):
+        """{kernel name: (total ms, launches)} since timing was enabled."""
+        n = len(KT_NAMES)
+        ms = np.zeros(n)
+        nl = np.zeros(n, dtype=np.int64)
+        self._chk(self.L.fvm_kernel_ms(self.h, _ptr(ms), _ptr(nl), n))
+        return {k: (float(ms[i]), int(nl[i])) for i, k in enumerate(KT_NAMES)}
 
     def copy_bandwidth(self, nbytes=1 << 30, iters=20):
         g = ctypes.c_double()

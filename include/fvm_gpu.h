@@ -231,6 +231,23 @@ int fvm_fused_sync(fvm_ctx *ctx);
  * ms[0] CFL+predictor+reconstruction, ms[1] limiter, ms[2] flux+apply, ms[3] advance */
 int fvm_fused_timing(fvm_ctx *ctx, int enable);
 int fvm_fused_kernel_ms(fvm_ctx *ctx, double *ms, long long *launches, int n);
+/* per-kernel HIP-event timing of the reference-order entry points (and so of
+ * fvm_substep_run): with timing enabled, each launch of the kernels below is
+ * bracketed by events on the context's stream and waited for, so enable it only
+ * to measure.  fvm_kernel_ms returns the total milliseconds and launch counts
+ * since timing was (re-)enabled, indexed by FVM_KT_*.  No counterpart in the
+ * reference. */
+#define FVM_KT_RECONSTRUCTION 0  /* k_reconstruction (computeReconstruction) */
+#define FVM_KT_SLOPE_LIMITER 1   /* k_slope_limiter (slopeLimiter + computeFluidUpdateI) */
+#define FVM_KT_FACE_FLUX 2       /* k_face_flux (computeFluidUpdate, face part) */
+#define FVM_KT_SCATTER 3         /* k_scatter (computeFluidUpdate, per-cell sums) */
+#define FVM_KT_PRIMITIVES 4      /* k_primitives (computePrimitives) */
+#define FVM_KT_TIMESTEP 5        /* k_timestep (computeTimestep) */
+#define FVM_KT_PREDICTOR_FACE 6  /* k_predictor_face (computePredictor, face part) */
+#define FVM_KT_PREDICTOR_CELL 7  /* k_predictor_cell (computePredictor, cell part) */
+#define FVM_NKT 8
+int fvm_kernel_timing(fvm_ctx *ctx, int enable);
+int fvm_kernel_ms(fvm_ctx *ctx, double *ms, long long *launches, int n);
 
 /* Individual timesteps with the per-cell state resident on the device, in
  * mesh order.  One fvm_substep_run is one substep of the reference's driver
