@@ -129,6 +129,7 @@ template <typename T> struct FusedArgs {
   unsigned long long *diag;
   double gamma, courant, dtmax;
   int write_w64;            // FP32 path: store FP64 W (last step of a run)
+  int k1_pass;              // FP32 path, opt-in limiters: 0 first pass of K1, 1 its check-and-redo pass
 };
 
 #define SF(p, k, i, n) (p)[(size_t)(k) * (size_t)(n) + (size_t)(i)]
@@ -297,6 +298,86 @@ template <int NQ> __device__ __forceinline__ void acc_st(float4 *p, const float 
 #pragma unroll
   for (int i = 0; i < NQ; i++) p[i] = make_float4(x[4 * i], x[4 * i + 1], x[4 * i + 2], x[4 * i + 3]);
 }
+// FP32 limiters from one pass.  phi (minmod min(1, x); smooth
+// (x^2 + 2x + e)/(x^2 + x + 2 + e)) rises on [0, X1] to phi(X1) = 1 and stays
+// >= 1 beyond (X1 = 1 minmod, 2 smooth), so a limiter min(1, min_f phi(x_f))
+// with x_f = bound / d_f >= 0 equals min(1, phi(min_f x_f)), and min_f x_f
+// = min(mx / max_f d_f, mn / min_f d_f).  Faces with |d_f| <= tau = 1e-16
+// max(|mn|, |mx|) count as 1 in the reference; they never set the minimum
+// while a face beyond tau exists, and a side whose extreme is within tau is
+// ignored, as the reference ignores it.  The reconstruction limiter takes d_f
+// = grad . cen_f (computeReconstruction.cpp:214-239); the slope limiter takes
+// the reconstruction-limited lr (grad . dr_f + t dtt) (slopeLimiter.cpp:20-77);
+// in the FP32 records cen_f = dr_f.
+template <int LIM> __device__ __forceinline__ float lim_from_ratio(float x) {
+  constexpr float X1 = LIM == LIM_MINMOD ? 1.0f : 2.0f;
+  return x >= X1 ? 1.0f : phi_c<LIM, float>(x);
+}
+// min_f bound / (s d_f) over the faces beyond the flat threshold
+__device__ __forceinline__ float min_ratio(float mn, float mx, float dmin, float dmax, float s) {
+  const float inf = __builtin_huge_valf();
+  const float tau = 1.0e-16f * smax(dabs(mx), dabs(mn));
+  const float hi = dmax * s, lo = dmin * s;
+  const float a = hi > tau ? (mx > 0.0f ? fdiv(mx, hi) : 0.0f) : inf;
+  const float b = lo < -tau ? (mn < 0.0f ? fdiv(mn, lo) : 0.0f) : inf;
+  return smin(a, b);
+}
+
+struct Dr3 {
+  float x, y, z;
+};
+
+// the slope limiter's time offset for a step of length dt from t0: every cell
+// has tend = t0 + dt, so dtI = dt, dth = dt / 2 and dtt = dtI - dth
+__device__ __forceinline__ float limiter_dtt(double t0, double dt_glob) {
+  const double tend = t0 + dt_glob;
+  const double dtI = tend - t0;
+  const double dth = 0.5 * smin(dtI, dtI);
+  return (float)(dtI - dth);
+}
+
+// both FP32 limiters of one cell from one pass over its own-side offsets: mn /
+// mx the raw neighbour differences, tv and gg (x, y, z blocks) Wt and the
+// gradients, limited in place
+template <int LIM>
+__device__ __forceinline__ void limit_cell32(const FusedArgs<float> &A, int e0, int deg, const float *mn,
+                                             const float *mx, float dtt, float *tv, float *gg) {
+  // extremes of d_f = grad . dr_f over the faces; the slope limiter's
+  // y_f = d_f + t dtt differs by a face-independent term, and rounding is
+  // monotone, so max_f fl(d_f + t dtt) = fl(max_f d_f + t dtt)
+  float dmin[NF], dmax[NF];
+  for (int k = 0; k < NF; k++) {
+    dmin[k] = __builtin_huge_valf();
+    dmax[k] = -__builtin_huge_valf();
+  }
+  // the own-side offsets only (12 bytes per face-end, not the 32-byte record)
+  const Dr3 *fdr = reinterpret_cast<const Dr3 *>(A.fdr);
+  Dr3 rn;
+  if (deg > 0) rn = fdr[e0];
+  for (int q = 0, e = e0; q < deg; q++, e += 64) {
+    const Dr3 r = rn;
+    if (q + 1 < deg) rn = fdr[e + 64];
+    const float dr[3] = {r.x, r.y, r.z};
+    for (int k = 0; k < NF; k++) {
+      const float d = gg[k] * dr[0] + gg[NF + k] * dr[1] + gg[2 * NF + k] * dr[2];
+      dmin[k] = smin(dmin[k], d);
+      dmax[k] = smax(dmax[k], d);
+    }
+  }
+  for (int k = 0; k < NF; k++) {
+    // the extremes start at 0 (a cell without faces gives 0, 0)
+    const float dlo = smin(0.0f, dmin[k]), dhi = smax(0.0f, dmax[k]);
+    const float ylo = smin(0.0f, __builtin_fmaf(tv[k], dtt, dmin[k]));
+    const float yhi = smax(0.0f, __builtin_fmaf(tv[k], dtt, dmax[k]));
+    const float lr = lim_from_ratio<LIM>(min_ratio(mn[k], mx[k], dlo, dhi, 1.0f));
+    const float lm = lr * lim_from_ratio<LIM>(min_ratio(mn[k], mx[k], ylo, yhi, lr));
+    tv[k] *= lm;
+    gg[k] *= lm;
+    gg[NF + k] *= lm;
+    gg[2 * NF + k] *= lm;
+  }
+}
+
 template <typename T> struct K1Waves {
   static constexpr int value = std::is_same<T, float>::value ? 3 : 1;
 };
@@ -305,6 +386,22 @@ template <typename T, bool MOVING, int LIM>
 __global__ void __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(K1Waves<T>::value)))
 k_cfl_predict_recon(FusedArgs<T> A) {
   constexpr bool LIMIT = LIM != LIM_SHIPPED;
+  constexpr bool LIMIT32 = LIMIT && std::is_same<T, float>::value;
+  // FP32 opt-in limiters: applied here, with the step's dt taken to be the
+  // last step's (dt is the global minimum this kernel is computing).  A second
+  // launch (k1_pass 1) compares it with the true dt: it returns at once when
+  // they agree (every step of a run but the first, while the rung holds) and
+  // otherwise redoes every cell with the true dt, so the result never depends
+  // on the guess
+  double dt_lim = 0.0;
+  if constexpr (LIMIT32) {
+    dt_lim = A.tdev[1];
+    if (A.k1_pass) {
+      const double dt_true = rung_dt0(A.dtmax, __longlong_as_double((long long)*A.dtmin_bits));
+      if (dt_true == dt_lim) return;  // uniform over the grid
+      dt_lim = dt_true;
+    }
+  }
   const int c = xcd_cell(A.n);
   const bool live = c < A.n;
   double dtn = REF_HUGE;
@@ -502,7 +599,7 @@ k_cfl_predict_recon(FusedArgs<T> A) {
       const bool ef = W1[ETHM] <= 0.0;
       if (ef) W1[ETHM] = ethm_from_entropy_d(g64, W1);
       else W1[ENTR] = entropy_from_ethm_d(g64, W1);
-      wave_count(&A.diag[FVM_DIAG_ETHM_FAILED], ef);
+      if (!A.k1_pass) wave_count(&A.diag[FVM_DIAG_ETHM_FAILED], ef);  // a redo counts nothing twice
       const double idt = 1.0 / dt;
       for (int k = 0; k < NF; k++) tv[k] = (T)((W1[k] - W0[k]) * idt);
     }
@@ -521,12 +618,20 @@ k_cfl_predict_recon(FusedArgs<T> A) {
         gy[k] = (T)((double)gy[k] * invV);
         gz[k] = (T)((double)gz[k] * invV);
       }
-      if constexpr (LIMIT && std::is_same<T, float>::value) {
-        // FP32: k_limit evaluates this limiter together with the slope limiter
-        // from the raw differences and the unlimited gradients
+      if constexpr (LIMIT32) {
+        // FP32: the reconstruction and slope limiters in one pass over the
+        // own-side offsets, from the raw differences and the unlimited gradients
+        T gg[3 * NF];
         for (int k = 0; k < NF; k++) {
-          SF(A.mm, k, c, n) = dmn[k];
-          SF(A.mm, NF + k, c, n) = dmx[k];
+          gg[k] = gx[k];
+          gg[NF + k] = gy[k];
+          gg[2 * NF + k] = gz[k];
+        }
+        limit_cell32<LIM>(A, e0, deg, dmn, dmx, limiter_dtt(A.tdev[0], dt_lim), tv, gg);
+        for (int k = 0; k < NF; k++) {
+          gx[k] = gg[k];
+          gy[k] = gg[NF + k];
+          gz[k] = gg[2 * NF + k];
         }
       } else if constexpr (LIMIT) {
         T lm[NF];
@@ -564,100 +669,15 @@ k_cfl_predict_recon(FusedArgs<T> A) {
 }
 
 // ---------------------------------------------------------------------------
-// K1b (opt-in limiters): slopeLimiter compute part + computeFluidUpdateI for
-// the global step, per cell over its own face-ends (own data only)
+// K1b (opt-in limiters, FP64 path): slopeLimiter compute part +
+// computeFluidUpdateI for the global step, per cell over its own face-ends
+// (own data only)
 // ---------------------------------------------------------------------------
-// FP32 limiters from one pass.  phi (minmod min(1, x); smooth
-// (x^2 + 2x + e)/(x^2 + x + 2 + e)) rises on [0, X1] to phi(X1) = 1 and stays
-// >= 1 beyond (X1 = 1 minmod, 2 smooth), so a limiter min(1, min_f phi(x_f))
-// with x_f = bound / d_f >= 0 equals min(1, phi(min_f x_f)), and min_f x_f
-// = min(mx / max_f d_f, mn / min_f d_f).  Faces with |d_f| <= tau = 1e-16
-// max(|mn|, |mx|) count as 1 in the reference; they never set the minimum
-// while a face beyond tau exists, and a side whose extreme is within tau is
-// ignored, as the reference ignores it.  The reconstruction limiter takes d_f
-// = grad . cen_f (computeReconstruction.cpp:214-239); the slope limiter takes
-// the reconstruction-limited lr (grad . dr_f + t dtt) (slopeLimiter.cpp:20-77);
-// in the FP32 records cen_f = dr_f.
-template <int LIM> __device__ __forceinline__ float lim_from_ratio(float x) {
-  constexpr float X1 = LIM == LIM_MINMOD ? 1.0f : 2.0f;
-  return x >= X1 ? 1.0f : phi_c<LIM, float>(x);
-}
-// min_f bound / (s d_f) over the faces beyond the flat threshold
-__device__ __forceinline__ float min_ratio(float mn, float mx, float dmin, float dmax, float s) {
-  const float inf = __builtin_huge_valf();
-  const float tau = 1.0e-16f * smax(dabs(mx), dabs(mn));
-  const float hi = dmax * s, lo = dmin * s;
-  const float a = hi > tau ? (mx > 0.0f ? fdiv(mx, hi) : 0.0f) : inf;
-  const float b = lo < -tau ? (mn < 0.0f ? fdiv(mn, lo) : 0.0f) : inf;
-  return smin(a, b);
-}
-
-struct Dr3 {
-  float x, y, z;
-};
-
-template <int LIM>
-__device__ __forceinline__ void limit_fp32(const FusedArgs<float> &A, int c) {
-  const int n = A.n;
-  const double t0 = A.tdev[0];
-  const double dt_glob = rung_dt0(A.dtmax, __longlong_as_double((long long)*A.dtmin_bits));
-  const double tend = t0 + dt_glob;
-  const double dtI = tend - t0;
-  const double dth = 0.5 * smin(dtI, dtI);
-  const float dtt = (float)(dtI - dth);
-  float *rc = A.rec_w + (size_t)c * REC;
-  float tv[NF], gg[3 * NF];
-  ldv<NF>(rc + R_T, tv);
-  ldv<3 * NF>(rc + R_G, gg);
-  // extremes of d_f = grad . dr_f over the faces; the slope limiter's
-  // y_f = d_f + t dtt differs by a face-independent term, and rounding is
-  // monotone, so max_f fl(d_f + t dtt) = fl(max_f d_f + t dtt)
-  float dmin[NF], dmax[NF];
-  for (int k = 0; k < NF; k++) {
-    dmin[k] = __builtin_huge_valf();
-    dmax[k] = -__builtin_huge_valf();
-  }
-  const int deg = A.deg[c];
-  const int e0 = A.gbase[c >> 6] + (c & 63);
-  // the own-side offsets only (12 bytes per face-end, not the 32-byte record)
-  const Dr3 *fdr = reinterpret_cast<const Dr3 *>(A.fdr);
-  Dr3 rn;
-  if (deg > 0) rn = fdr[e0];
-  for (int q = 0, e = e0; q < deg; q++, e += 64) {
-    const Dr3 r = rn;
-    if (q + 1 < deg) rn = fdr[e + 64];
-    const float dr[3] = {r.x, r.y, r.z};
-    for (int k = 0; k < NF; k++) {
-      const float d = gg[k] * dr[0] + gg[NF + k] * dr[1] + gg[2 * NF + k] * dr[2];
-      dmin[k] = smin(dmin[k], d);
-      dmax[k] = smax(dmax[k], d);
-    }
-  }
-  for (int k = 0; k < NF; k++) {
-    const float mn = SF(A.mm, k, c, n), mx = SF(A.mm, NF + k, c, n);
-    // the extremes start at 0 (a cell without faces gives 0, 0)
-    const float dlo = smin(0.0f, dmin[k]), dhi = smax(0.0f, dmax[k]);
-    const float ylo = smin(0.0f, __builtin_fmaf(tv[k], dtt, dmin[k]));
-    const float yhi = smax(0.0f, __builtin_fmaf(tv[k], dtt, dmax[k]));
-    const float lr = lim_from_ratio<LIM>(min_ratio(mn, mx, dlo, dhi, 1.0f));
-    const float lm = lr * lim_from_ratio<LIM>(min_ratio(mn, mx, ylo, yhi, lr));
-    tv[k] *= lm;
-    gg[k] *= lm;
-    gg[NF + k] *= lm;
-    gg[2 * NF + k] *= lm;
-  }
-  stv<NF>(rc + R_T, tv);
-  stv<3 * NF>(rc + R_G, gg);
-}
-
 template <typename T, int LIM>
 __global__ void __launch_bounds__(FB) k_limit(FusedArgs<T> A) {
+  static_assert(std::is_same<T, double>::value, "the FP32 path limits inside k_cfl_predict_recon");
   const int c = xcd_cell(A.n);
   if (c >= A.n) return;
-  if constexpr (std::is_same<T, float>::value) {
-    limit_fp32<LIM>(A, c);
-    return;
-  }
   const int n = A.n;
   const double t0 = A.tdev[0];
   const double dt_glob = rung_dt0(A.dtmax, __longlong_as_double((long long)*A.dtmin_bits));
@@ -1186,6 +1206,7 @@ template <typename T> FusedArgs<T> fused_args(fvm_ctx *c, double dtmax) {
   A.courant = c->prm.courant_no;
   A.dtmax = dtmax;
   A.write_w64 = 0;
+  A.k1_pass = 0;
   return A;
 }
 
@@ -1232,7 +1253,7 @@ template <typename T> int fused_setup_t(fvm_ctx *c) {
       (rc = fvm_ensure(c, z.fe, sizeof(FaceEndRec<T>) * z.slots)) || (rc = fvm_ensure(c, z.U, NF * nd)) ||
       (rc = fvm_ensure(c, z.W64, NF * nd)) || (rc = fvm_ensure(c, z.cvol, nd)) ||
       (rc = fvm_ensure(c, z.mvel, 3 * nd)) || (rc = fvm_ensure(c, z.rec[0], es * REC * (size_t)n)) ||
-      (rc = fvm_ensure(c, z.rec[1], es * REC * (size_t)n)) || (rc = fvm_ensure(c, z.mm, es * 20 * (size_t)n)) ||
+      (rc = fvm_ensure(c, z.rec[1], es * REC * (size_t)n)) || (rc = fvm_ensure(c, z.mm, es * 20 * (size_t)(es == 8 ? n : 1)))  /* FP64 path only */ ||
       (rc = fvm_ensure(c, z.wc[0], es * RWC * (size_t)n)) || (rc = fvm_ensure(c, z.wc[1], es * RWC * (size_t)n)) ||
       (rc = fvm_ensure(c, z.dt_new, nd)) || (rc = fvm_ensure(c, z.dtmin, sizeof(unsigned long long))) ||
       (rc = fvm_ensure(c, z.tdev, sizeof(double) * 2)))
@@ -1293,7 +1314,15 @@ template <typename T, bool MOVING, int LIM> int launch_step(fvm_ctx *c, const Fu
   if (ev) (void)hipEventRecord(ev[0], c->stream);
   hipLaunchKernelGGL((k_cfl_predict_recon<T, MOVING, LIM>), dim3(grid), dim3(FB), 0, c->stream, A);
   if (ev) (void)hipEventRecord(ev[1], c->stream);
-  if constexpr (LIM != LIM_SHIPPED) hipLaunchKernelGGL((k_limit<T, LIM>), dim3(grid), dim3(FB), 0, c->stream, A);
+  if constexpr (LIM != LIM_SHIPPED) {
+    if constexpr (std::is_same<T, float>::value) {
+      FusedArgs<T> A2 = A;  // the check-and-redo pass of the limiters' dt
+      A2.k1_pass = 1;
+      hipLaunchKernelGGL((k_cfl_predict_recon<T, MOVING, LIM>), dim3(grid), dim3(FB), 0, c->stream, A2);
+    } else {
+      hipLaunchKernelGGL((k_limit<T, LIM>), dim3(grid), dim3(FB), 0, c->stream, A);
+    }
+  }
   if (ev) (void)hipEventRecord(ev[2], c->stream);
   if constexpr (std::is_same<T, float>::value) {
     // FVM_FUSED_ADDR64 set: always the 64-bit addressing (it serves meshes past

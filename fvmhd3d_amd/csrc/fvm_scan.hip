@@ -50,16 +50,29 @@ __global__ void __launch_bounds__(SB) k_scan_reduce(const int *__restrict__ in, 
   if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
 }
 
-// one block scans all block totals (chunks of SB)
+// one block scans all block totals, in chunks of SB * BSI: each thread sums
+// BSI consecutive totals, the block scans the thread sums, and each thread
+// writes its BSI offsets
+constexpr int BSI = 16;
 __global__ void __launch_bounds__(SB) k_scan_blocksums(const int *__restrict__ sums, int nb, int *__restrict__ offs,
                                            int *__restrict__ total) {
   int carry = 0;
-  for (int b0 = 0; b0 < nb; b0 += SB) {
-    const int b = b0 + threadIdx.x;
-    const int v = b < nb ? sums[b] : 0;
+  for (int b0 = 0; b0 < nb; b0 += SB * BSI) {
+    const int b = b0 + (int)threadIdx.x * BSI;
+    int v[BSI];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < BSI; k++) {
+      v[k] = b + k < nb ? sums[b + k] : 0;
+      s += v[k];
+    }
     int tot;
-    const int ex = block_excl_scan(v, &tot);
-    if (b < nb) offs[b] = carry + ex;
+    int run = carry + block_excl_scan(s, &tot);
+#pragma unroll
+    for (int k = 0; k < BSI; k++) {
+      if (b + k < nb) offs[b + k] = run;
+      run += v[k];
+    }
     carry += tot;
   }
   if (threadIdx.x == 0) *total = carry;

@@ -228,7 +228,10 @@ int fvm_fused_dt_history(fvm_ctx *ctx, double *dts, int n);
 /* copy the last step's gradients, Wt and dt_new into the per-cell fields */
 int fvm_fused_sync(fvm_ctx *ctx);
 /* per-kernel HIP-event timing of fvm_fused_run on the context's stream:
- * ms[0] CFL+predictor+reconstruction, ms[1] limiter, ms[2] flux+apply, ms[3] advance */
+ * ms[0] CFL+predictor+reconstruction (FP32 path: with the opt-in limiters),
+ * ms[1] FP32 path: the second launch that checks the limiters' dt and redoes
+ *       the step when it changed; FP64 path: the limiter launch,
+ * ms[2] flux+apply, ms[3] advance */
 int fvm_fused_timing(fvm_ctx *ctx, int enable);
 int fvm_fused_kernel_ms(fvm_ctx *ctx, double *ms, long long *launches, int n);
 /* per-kernel HIP-event timing of the reference-order entry points (and so of

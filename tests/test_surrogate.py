@@ -318,9 +318,11 @@ def test_gpu_training_lowers_loss_and_gates_accuracy():
     falls by 10x and the faces the gate accepts are within 2e-2 of the FP64 HLLD
     (L1 over the accepted faces' mass, momentum, energy and induction fluxes,
     relative to the FP64 L1); the rest are answered by the FP64 HLLD.  The
-    timing against the FP64 and FP32 HLLD on the same faces is printed: both
-    HLLD batches are HBM-bound on this chip, so the surrogate, which reads and
-    writes the same bytes, is not faster there."""
+    timing against the FP64 and FP32 HLLD on the same faces is printed only: a
+    batch of faces staged through HBM is bound by its bytes for all three.  What
+    the network gains over the plain FP64 method is asserted where it is used,
+    in the step (tests/test_sg_step.py::test_surrogate_step_gain_and_accuracy:
+    faster per step, density and energy within 1e-4 after 100 steps)."""
     from fvmhd3d_amd import gpu as G
     from fvmhd3d_amd.surrogate import Surrogate
     train = _harvest(n=20, steps=4)
@@ -343,5 +345,4 @@ def test_gpu_training_lowers_loss_and_gates_accuracy():
     big = tuple(np.concatenate([a] * max(1, 400000 // len(a))) for a in test)
     ms, _ = sg.time_flux(*big, iters=5)
     print(f"{len(big[1])} faces: surrogate+fallback {ms[0]:.3f} ms, FP64 HLLD {ms[1]:.3f} ms, FP32 HLLD {ms[2]:.3f} ms")
-    assert np.all(ms > 0)
     sg.close()
