@@ -78,6 +78,27 @@ def k1_bytes(ncell, nfe, prec_bytes, limiter):
     return ncell * per_cell + nfe * per_fe
 
 
+def runtime_copy_gbps(nbytes, iters):
+    """The HIP runtime's own device-to-device copy (torch's copy_ of a contiguous
+    buffer), timed like the library's stream-copy probe: a second opinion on what
+    a copy reaches on this box (read + written bytes per second)."""
+    import torch
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(iters):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / iters
+    del a, b
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -401,7 +422,8 @@ def main():
     probe = G.Context()
     copy_gbps = probe.copy_bandwidth(1 << 30, 10)
     probe.close()
-    log(f"[bench] stream copy {copy_gbps:.0f} GB/s")
+    memcpy_gbps = runtime_copy_gbps(1 << 30, 10)
+    log(f"[bench] stream copy {copy_gbps:.0f} GB/s; the runtime's device-to-device copy {memcpy_gbps:.0f} GB/s")
     m, st, lim = build_workload(args.workload)
     nfe = int(m.cell_off[-1])
 
@@ -453,7 +475,8 @@ def main():
         "k_cfl_predict_recon": {"ms": round(k1_ms, 4), "GB/s": round(k1_b / (k1_ms * 1e-3) / 1e9, 1)},
         "k_cfl_predict_recon_check": {"ms": round(kc_ms, 4), "redone": "first step of a run, and when the rung changes"},
         "k_flux_apply": {"ms": round(k2_ms, 4), "GB/s": round(achieved, 1)}},
-        "step_GB_alg": round((k1_b + k2_b) / 1e9, 3), "stream_copy_GBps": round(copy_gbps, 1)}
+        "step_GB_alg": round((k1_b + k2_b) / 1e9, 3), "stream_copy_GBps": round(copy_gbps, 1),
+        "runtime_d2d_copy_GBps": round(memcpy_gbps, 1)}
     # the whole step against the same peak: every kernel's algorithmic bytes over the step time
     step_gbs = (k1_b + k2_b) / (elapsed / args.steps) / 1e9
     extra["pipeline_roofline"] = {"achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

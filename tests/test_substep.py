@@ -136,14 +136,23 @@ def wt_bound(S):
 
 
 @pytest.mark.gpu
-def test_substeps_gpu_match_oracle():
-    from fvmhd3d_amd.substep import GpuBackend
-    pos = M.uniform_random(14 ** 3, seed=5)
-    pos = pos[M.morton_order(pos, (1, 1, 1))]
-    m = M.build_periodic(pos)
-    W = ic_disk(m.pos)
-    So = Substepper(m, W, dtmax=0.25, limiter=2)
-    Sg = Substepper(m, W, dtmax=0.25, limiter=2)
+@pytest.mark.parametrize("case", ["disk_random_14", "thin_disk_lattice_16"])
+def test_substeps_gpu_match_oracle(case):
+    """The host driver with the oracle and with the library behind it, 10
+    substeps.  The thin disk is the individual-timestep workload's initial
+    state on 16^3 lattice sites, with dtmax scaled with the cell size (1/64 at
+    128^3, 1/8 here)."""
+    from fvmhd3d_amd.substep import GpuBackend, ic_thin_disk
+    if case == "disk_random_14":
+        pos = M.uniform_random(14 ** 3, seed=5)
+        pos = pos[M.morton_order(pos, (1, 1, 1))]
+        m = M.build_periodic(pos)
+        W, dtmax = ic_disk(m.pos), 0.25
+    else:
+        m = workload_mesh("lattice", 16)
+        W, dtmax = ic_thin_disk(m.pos), 1.0 / 8
+    So = Substepper(m, W, dtmax=dtmax, limiter=2)
+    Sg = Substepper(m, W, dtmax=dtmax, limiter=2)
     Bo = OracleBackend(limiter=2)
     Bg = GpuBackend(Sg.gamma, Sg.courant, tuple(m.box), limiter=2)
     So.init_rungs(Bo)
