@@ -78,27 +78,6 @@ def k1_bytes(ncell, nfe, prec_bytes, limiter):
     return ncell * per_cell + nfe * per_fe
 
 
-def runtime_copy_gbps(nbytes, iters):
-    """The HIP runtime's own device-to-device copy (torch's copy_ of a contiguous
-    buffer), timed like the library's stream-copy probe: a second opinion on what
-    a copy reaches on this box (read + written bytes per second)."""
-    import torch
-    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record()
-    for _ in range(iters):
-        b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / iters
-    del a, b
-    torch.cuda.empty_cache()
-    return 2.0 * nbytes / (ms * 1e-3) / 1e9
-
-
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -421,8 +400,8 @@ def main():
         return
     probe = G.Context()
     copy_gbps = probe.copy_bandwidth(1 << 30, 10)
+    memcpy_gbps = probe.runtime_copy_bandwidth(1 << 30, 10)
     probe.close()
-    memcpy_gbps = runtime_copy_gbps(1 << 30, 10)
     log(f"[bench] stream copy {copy_gbps:.0f} GB/s; the runtime's device-to-device copy {memcpy_gbps:.0f} GB/s")
     m, st, lim = build_workload(args.workload)
     nfe = int(m.cell_off[-1])

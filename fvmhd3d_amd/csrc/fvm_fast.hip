@@ -153,7 +153,10 @@ __global__ void __launch_bounds__(256) k_stream_copy(const f4v_t *__restrict__ s
 }
 }  // namespace fvm
 
-int fvm_measure_copy_bandwidth(fvm_ctx *c, long long bytes, int iters, double *gbps) {
+// the library's stream copy (runtime == false) or the HIP runtime's own
+// device-to-device copy (runtime == true) of `bytes`, iters times after one
+// untimed pass; read plus written bytes per second
+static int copy_bandwidth(fvm_ctx *c, long long bytes, int iters, double *gbps, bool runtime) {
   if (!c || bytes < (1 << 20) || iters < 1 || !gbps) return FVM_EARG;
   (void)hipSetDevice(c->device);
   const size_t n = (size_t)bytes / 16;
@@ -168,10 +171,13 @@ int fvm_measure_copy_bandwidth(fvm_ctx *c, long long bytes, int iters, double *g
   (void)hipEventCreate(&e1);
   (void)hipMemsetAsync(a, 0, n * 16, c->stream);
   const unsigned grid = 256 * 8 * 4;
-  hipLaunchKernelGGL(fvm::k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const fvm::f4v_t *)a, (fvm::f4v_t *)b, n);
-  (void)hipEventRecord(e0, c->stream);
-  for (int k = 0; k < iters; k++)
-    hipLaunchKernelGGL(fvm::k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const fvm::f4v_t *)a, (fvm::f4v_t *)b, n);
+  for (int k = 0; k <= iters; k++) {
+    if (k == 1) (void)hipEventRecord(e0, c->stream);
+    if (runtime)
+      (void)hipMemcpyAsync(b, a, n * 16, hipMemcpyDeviceToDevice, c->stream);
+    else
+      hipLaunchKernelGGL(fvm::k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const fvm::f4v_t *)a, (fvm::f4v_t *)b, n);
+  }
   (void)hipEventRecord(e1, c->stream);
   hipError_t err = hipEventSynchronize(e1);
   float ms = 0.0f;
@@ -183,4 +189,12 @@ int fvm_measure_copy_bandwidth(fvm_ctx *c, long long bytes, int iters, double *g
   if (err != hipSuccess) return fvm_fail(c, FVM_EHIP, hipGetErrorString(err));
   *gbps = 2.0 * (double)(n * 16) * iters / (ms * 1e-3) / 1e9;
   return FVM_OK;
+}
+
+int fvm_measure_copy_bandwidth(fvm_ctx *c, long long bytes, int iters, double *gbps) {
+  return copy_bandwidth(c, bytes, iters, gbps, false);
+}
+
+int fvm_measure_runtime_copy_bandwidth(fvm_ctx *c, long long bytes, int iters, double *gbps) {
+  return copy_bandwidth(c, bytes, iters, gbps, true);
 }

@@ -84,6 +84,7 @@ SIGNATURES = {
     "fvm_kernel_timing": [_P, _I],
     "fvm_kernel_ms": [_P, _P, _P, _I],
     "fvm_measure_copy_bandwidth": [_P, ctypes.c_longlong, _I, ctypes.POINTER(ctypes.c_double)],
+    "fvm_measure_runtime_copy_bandwidth": [_P, ctypes.c_longlong, _I, ctypes.POINTER(ctypes.c_double)],
     "fvm_get_face_states": [_P, _I, _I, _P, _P, _P, _P, _P],
     "fvm_energy": [_P, _P],
     "fvm_dump_binary": [_P, ctypes.c_char_p, _P, _P, _P],
@@ -365,6 +366,12 @@ class Context:
     def copy_bandwidth(self, nbytes=1 << 30, iters=20):
         g = ctypes.c_double()
         self._chk(self.L.fvm_measure_copy_bandwidth(self.h, int(nbytes), int(iters), ctypes.byref(g)))
+        return g.value
+
+    def runtime_copy_bandwidth(self, nbytes=1 << 30, iters=20):
+        """The HIP runtime's own device-to-device copy, timed the same way."""
+        g = ctypes.c_double()
+        self._chk(self.L.fvm_measure_runtime_copy_bandwidth(self.h, int(nbytes), int(iters), ctypes.byref(g)))
         return g.value
 
     # individual timesteps with the state resident on the device

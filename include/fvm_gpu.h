@@ -293,6 +293,10 @@ int fvm_timestep_limiter_sweep(int ncell, const int *cell_off, const int *nbr, c
 
 /* read+write bandwidth of a plain 16-byte-vector copy of `bytes` bytes, GB/s */
 int fvm_measure_copy_bandwidth(fvm_ctx *ctx, long long bytes, int iters, double *gbps);
+/* the same measurement with the HIP runtime's own device-to-device copy
+ * (hipMemcpyAsync) in place of the library's copy kernel: a second opinion on
+ * what a copy reaches on the device.  No counterpart in the reference. */
+int fvm_measure_runtime_copy_bandwidth(fvm_ctx *ctx, long long bytes, int iters, double *gbps);
 
 /* diagnostics counters accumulated since the last reset */
 int fvm_get_diagnostics(fvm_ctx *ctx, long long *out, int n);
