@@ -56,9 +56,10 @@ def build_workload(name):
 
 def k2_bytes(ncell, nfe, prec_bytes):
     """Algorithmic HBM bytes of one flux+apply launch: every array it must touch, once.
-    per cell: the gather record (W 10, Wt 10, gradients 30 in face precision), FP64 U,
-    volume, CSR offset; writes FP64 U and next step's W slice (10).  The FP64 path also
-    reads and writes the stored FP64 primitives (the FP32 path re-derives them from U).
+    per cell: the gather record and Wt (FP64: W 10, Wt 10, gradients 30; FP32: the
+    half-step state 10 and gradients 30, Wt 10 beside them), FP64 U, volume, CSR offset;
+    writes FP64 U and the next step's W (10).  The FP64 path also reads and writes the
+    stored FP64 primitives (the FP32 path re-derives them from U).
     per face-end: one 32-byte face-end record (neighbour, area vector, centroid offset)."""
     p = prec_bytes
     per_cell = 50 * p + 80 + 8 + 4 + 80 + 10 * p + (160 if p == 8 else 0)
@@ -67,13 +68,14 @@ def k2_bytes(ncell, nfe, prec_bytes):
 
 
 def k1_bytes(ncell, nfe, prec_bytes, limiter):
-    """CFL + predictor + reconstruction launch: per cell read FP64 U, volume, offset and the
-    W of the record; write Wt + gradients (40) and dt_new; per face-end the same 32-byte
-    record stream.  With an opt-in limiter the FP32 path also limits here (a second pass
-    over the 12-byte own-side offsets of each face-end); the FP64 path writes min/max (20)
-    for its separate limiter launch."""
+    """CFL + predictor + reconstruction launch: per cell read FP64 U, volume, offset and W;
+    write Wt + gradients (40) and dt_new (the FP32 path also the half-step state W + Wt dti
+    at the head of its record, 10 more); per face-end the same 32-byte record stream.
+    With an opt-in limiter the FP32 path also limits here (a second pass over the 12-byte
+    own-side offsets of each face-end); the FP64 path writes min/max (20) for its separate
+    limiter launch."""
     p = prec_bytes
-    per_cell = 80 + 8 + 4 + 10 * p + 40 * p + 8 + (20 * p if limiter and p == 8 else 0)
+    per_cell = 80 + 8 + 4 + 10 * p + 40 * p + 8 + (10 * p if p == 4 else 0) + (20 * p if limiter and p == 8 else 0)
     per_fe = (32 + (12 if limiter else 0)) if p == 4 else 80
     return ncell * per_cell + nfe * per_fe
 

@@ -70,12 +70,17 @@ __device__ __forceinline__ int xcd_cell(int nb) {
 
 // Per-cell gather record: the state a neighbour needs, contiguous, so one
 // gather is a few 16-byte vector loads.
-//   [0,10) W   [10,20) Wt   [20,50) gradients x[10] y[10] z[10]   (stride 52)
-// The stride is 13 16-byte words in FP32 (208 bytes): the passes that stream
-// the records cell by cell (the limiter, the own-cell reads) then move 19 %
-// fewer bytes than with 256-byte records, and the gathers are no slower.
+//   FP64: [0,10) W   [10,20) Wt   [20,50) gradients x[10] y[10] z[10]   (stride 52)
+//   FP32: [0,10) W + Wt dti, the state at the half step that every face state of
+//         the step starts from   [10,40) gradients   (stride 40: ten 16-byte
+//         words, 160 bytes); Wt itself is kept per cell beside it (FusedArgs::tv)
+// The FP64 path keeps W and Wt apart, because the reference forms a face state
+// as W + (grad . dr + Wt dti) and the order is part of its arithmetic.
 constexpr int REC = 52;
 enum { R_W = 0, R_T = 10, R_G = 20 };
+constexpr int REC32 = 40;
+constexpr int R32_G = 10;
+template <typename T> constexpr int rec_len() { return sizeof(T) == 4 ? REC32 : REC; }
 // A second, compact copy of each cell's W, 12 values apart (48 bytes in FP32):
 // the predictor kernel gathers only the neighbours' W, and from this array a
 // cache line holds the W of more than two cells instead of part of one record.
@@ -115,11 +120,12 @@ template <typename T> struct FusedArgs {
   const double *cvol;       // [n]
   const double *lmin;       // [n] pow(V / (4 pi / 3), 1 / 3), formed once at setup
   const double *mvel;       // [3][n] generator velocity (moving mesh only)
-  const T *rec;             // [n][REC] records of this step
+  const T *rec;             // [n][rec_len<T>()] records of this step
   T *rec_w;                 // records written by this launch (K1: rec itself; K2: next step's)
   T *rec_next;
   const T *wcur;            // [n][RWC] compact W of this step (predictor gathers)
   T *wnext;                 // [n][RWC] compact W written for the next step
+  T *tv;                    // FP32: [n][NF] Wt of this step (the predictor kernel writes it)
   double *W64;              // [10][n] FP64 primitive state (own cell only)
   double *U;                // [10][n] FP64 conserved state (own cell only)
   T *mm;                    // [20][n] min/max (opt-in limiters)
@@ -129,7 +135,7 @@ template <typename T> struct FusedArgs {
   unsigned long long *diag;
   double gamma, courant, dtmax;
   int write_w64;            // FP32 path: store FP64 W (last step of a run)
-  int k1_pass;              // FP32 path, opt-in limiters: 0 first pass of K1, 1 its check-and-redo pass
+  int k1_pass;              // FP32 path: 0 first pass of K1, 1 its check-and-redo pass
 };
 
 #define SF(p, k, i, n) (p)[(size_t)(k) * (size_t)(n) + (size_t)(i)]
@@ -378,6 +384,27 @@ __device__ __forceinline__ void limit_cell32(const FusedArgs<float> &A, int e0, 
   }
 }
 
+// the step's dt and the extrapolation time of the face states: every cell has
+// tend = t0 + dt, so dtI = dtJ, dt = min(dtI, dtJ), dti = dtI - dt/2
+struct StepTimes {
+  double t0, dt_glob, dtf, dti;
+};
+__device__ __forceinline__ StepTimes step_times_dt(double t0, double dt_glob) {
+  StepTimes s;
+  s.t0 = t0;
+  s.dt_glob = dt_glob;
+  const double tend = s.t0 + s.dt_glob;
+  const double dtI = tend - s.t0;
+  s.dtf = smin(dtI, dtI);
+  s.dti = dtI - 0.5 * s.dtf;
+  return s;
+}
+template <typename T> __device__ __forceinline__ StepTimes step_times(const FusedArgs<T> &A) {
+  return step_times_dt(A.tdev[0], rung_dt0(A.dtmax, __longlong_as_double((long long)*A.dtmin_bits)));
+}
+// the half-step state W + Wt dti of one variable (FP32 path)
+__device__ __forceinline__ float half_step32(float W, float t, float dti) { return __builtin_fmaf(t, dti, W); }
+
 template <typename T> struct K1Waves {
   static constexpr int value = std::is_same<T, float>::value ? 3 : 1;
 };
@@ -386,20 +413,21 @@ template <typename T, bool MOVING, int LIM>
 __global__ void __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(K1Waves<T>::value)))
 k_cfl_predict_recon(FusedArgs<T> A) {
   constexpr bool LIMIT = LIM != LIM_SHIPPED;
-  constexpr bool LIMIT32 = LIMIT && std::is_same<T, float>::value;
-  // FP32 opt-in limiters: applied here, with the step's dt taken to be the
-  // last step's (dt is the global minimum this kernel is computing).  A second
-  // launch (k1_pass 1) compares it with the true dt: it returns at once when
-  // they agree (every step of a run but the first, while the rung holds) and
-  // otherwise redoes every cell with the true dt, so the result never depends
-  // on the guess
-  double dt_lim = 0.0;
-  if constexpr (LIMIT32) {
-    dt_lim = A.tdev[1];
+  constexpr bool FP32 = std::is_same<T, float>::value;
+  constexpr bool LIMIT32 = LIMIT && FP32;
+  // FP32 path: the opt-in limiters and the half-step states of the records are
+  // formed here with the step's dt taken to be the last step's (dt is the
+  // global minimum this kernel is computing).  A second launch (k1_pass 1)
+  // compares it with the true dt: it returns at once when they agree (every
+  // step of a run but the first, while the rung holds) and otherwise redoes
+  // every cell with the true dt, so the result never depends on the guess
+  double dt_step = 0.0;
+  if constexpr (FP32) {
+    dt_step = A.tdev[1];
     if (A.k1_pass) {
       const double dt_true = rung_dt0(A.dtmax, __longlong_as_double((long long)*A.dtmin_bits));
-      if (dt_true == dt_lim) return;  // uniform over the grid
-      dt_lim = dt_true;
+      if (dt_true == dt_step) return;  // uniform over the grid
+      dt_step = dt_true;
     }
   }
   const int c = xcd_cell(A.n);
@@ -627,7 +655,7 @@ k_cfl_predict_recon(FusedArgs<T> A) {
           gg[NF + k] = gy[k];
           gg[2 * NF + k] = gz[k];
         }
-        limit_cell32<LIM>(A, e0, deg, dmn, dmx, limiter_dtt(A.tdev[0], dt_lim), tv, gg);
+        limit_cell32<LIM>(A, e0, deg, dmn, dmx, limiter_dtt(A.tdev[0], dt_step), tv, gg);
         for (int k = 0; k < NF; k++) {
           gx[k] = gg[k];
           gy[k] = gg[NF + k];
@@ -653,11 +681,25 @@ k_cfl_predict_recon(FusedArgs<T> A) {
           gz[k] = gz[k] * lm[k];
         }
       }
-      T *wc = A.rec_w + (size_t)c * REC;
-      stv<NF>(wc + R_T, tv);
-      stv<NF>(wc + R_G, gx);
-      stv<NF>(wc + R_G + NF, gy);
-      stv<NF>(wc + R_G + 2 * NF, gz);
+      if constexpr (FP32) {
+        // the record starts from the half-step state W + Wt dti of the step,
+        // formed as the flux kernel formed it; Wt goes to its own array
+        const float dti = (float)step_times_dt(A.tdev[0], dt_step).dti;
+        T wh[NF];
+        for (int k = 0; k < NF; k++) wh[k] = half_step32(Wc[k], tv[k], dti);
+        T *r = A.rec_w + (size_t)c * REC32;
+        stv<NF>(r, wh);
+        stv<NF>(r + R32_G, gx);
+        stv<NF>(r + R32_G + NF, gy);
+        stv<NF>(r + R32_G + 2 * NF, gz);
+        stv<NF>(A.tv + (size_t)c * NF, tv);
+      } else {
+        T *wc = A.rec_w + (size_t)c * REC;
+        stv<NF>(wc + R_T, tv);
+        stv<NF>(wc + R_G, gx);
+        stv<NF>(wc + R_G + NF, gy);
+        stv<NF>(wc + R_G + 2 * NF, gz);
+      }
     }
     A.dt_new[c] = dtn;
   }
@@ -830,24 +872,8 @@ __device__ __forceinline__ void cell_update(const FusedArgs<T> &A, int c, double
     nts(Ui[k], &SF(A.U, k, c, n));
     Wn[k] = (T)W1[k];
   }
-  stv<NF>(A.rec_next + (size_t)c * REC + R_W, Wn);
+  if constexpr (sizeof(T) == 8) stv<NF>(A.rec_next + (size_t)c * REC + R_W, Wn);  // FP32: K1 writes the record
   stv<NF>(A.wnext + (size_t)c * RWC, Wn);
-}
-
-// the step's dt and the extrapolation time of the face states: every cell has
-// tend = t0 + dt, so dtI = dtJ, dt = min(dtI, dtJ), dti = dtI - dt/2
-struct StepTimes {
-  double t0, dt_glob, dtf, dti;
-};
-template <typename T> __device__ __forceinline__ StepTimes step_times(const FusedArgs<T> &A) {
-  StepTimes s;
-  s.t0 = A.tdev[0];
-  s.dt_glob = rung_dt0(A.dtmax, __longlong_as_double((long long)*A.dtmin_bits));
-  const double tend = s.t0 + s.dt_glob;
-  const double dtI = tend - s.t0;
-  s.dtf = smin(dtI, dtI);
-  s.dti = dtI - 0.5 * s.dtf;
-  return s;
 }
 
 // generic K2: each lane gathers its own neighbour records
@@ -915,7 +941,7 @@ __global__ void __launch_bounds__(FB) k_flux_apply(FusedArgs<T> A) {
 // lanes served by one LDS cycle then read distinct banks, and with the LDS
 // records 256-byte aligned a lane addresses slot s as (its base | 16 (r % 16))
 // ^ 16 s, one instruction per read.
-constexpr int GQ = 13;  // float4 of a record that the flux needs (floats 0..51)
+constexpr int GQ = REC32 / 4;  // float4 of an FP32 record (all of it)
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) void glb_void_t;
@@ -953,9 +979,9 @@ __device__ __forceinline__ void gather_async(const float *__restrict__ rec, int 
     const float *src;
     if constexpr (OFF32)  // record index < 2^24: one 24-bit multiply-add
       src = reinterpret_cast<const float *>(reinterpret_cast<const char *>(rec) +
-                                            (__umul24((unsigned)jr[k], (unsigned)(REC * sizeof(float))) + 16u * slot));
+                                            (__umul24((unsigned)jr[k], (unsigned)(REC32 * sizeof(float))) + 16u * slot));
     else
-      src = rec + 4 * slot + (size_t)(unsigned)jr[k] * REC;
+      src = rec + 4 * slot + (size_t)(unsigned)jr[k] * REC32;
     __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)(buf + 64 * k), 16, 0, 0);
   }
 }
@@ -966,7 +992,6 @@ typedef __attribute__((address_space(3))) const f4v_t lds_cf4_t;
 // face state of one side: (W + t dti) + grad . dr with explicit FMAs.  A cell
 // forms its own side and its neighbour's with the same operations, so the two
 // cells of a face build bit-identical states and fluxes.
-__device__ __forceinline__ float half_step32(float W, float t, float dti) { return __builtin_fmaf(t, dti, W); }
 __device__ __forceinline__ float face_state32(float Wh, float g0, float g1, float g2, const float *dr) {
   return __builtin_fmaf(g2, dr[2], __builtin_fmaf(g1, dr[1], __builtin_fmaf(g0, dr[0], Wh)));
 }
@@ -998,16 +1023,12 @@ __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
   if (deg > 1) rn = A.fe[e0 + 64];
 
   const StepTimes st = step_times(A);
-  const T dti = (T)st.dti;
   // own state at the half step, extrapolated per face-end along drL / drR
   T Wc[NF], gc[3 * NF];
   if (live) {
-    const T *rc = A.rec + (size_t)c * REC;
-    T tc[NF];
-    ldv<NF>(rc + R_W, Wc);
-    ldv<NF>(rc + R_T, tc);
-    ldv<3 * NF>(rc + R_G, gc);
-    for (int k = 0; k < NF; k++) Wc[k] = half_step32(Wc[k], tc[k], dti);
+    const T *rc = A.rec + (size_t)c * REC32;
+    ldv<NF>(rc, Wc);
+    ldv<3 * NF>(rc + R32_G, gc);
   }
   double dU[NF], gp[3], dv = 0.0;
   for (int k = 0; k < NF; k++) dU[k] = 0.0;
@@ -1030,7 +1051,7 @@ __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
       }
       const T *dr = fe.isL ? fe.drR : fe.drL;
       for (int k = 0; k < NF; k++)
-        Wn[k] = face_state32(half_step32(x[R_W + k], x[R_T + k], dti), x[R_G + k], x[R_G + NF + k], x[R_G + 2 * NF + k], dr);
+        Wn[k] = face_state32(x[k], x[R32_G + k], x[R32_G + NF + k], x[R32_G + 2 * NF + k], dr);
     }
     __builtin_amdgcn_s_waitcnt(WAIT_LGKM);  // every lane's reads of buf are done
     wave_lds_sync();
@@ -1049,8 +1070,8 @@ __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
   }
   wave_add(&A.diag[FVM_DIAG_POSITIVITY], nbad);
   if (live) {
-    T tc[NF];  // reloaded rather than held through the face loop
-    ldv<NF>(A.rec + (size_t)c * REC + R_T, tc);
+    T tc[NF];  // Wt of the step, for the Powell and Dedner terms
+    ldv<NF>(A.tv + (size_t)c * NF, tc);
     cell_update<T>(A, c, st.t0, st.dt_glob, dU, gp, dv, tc);
   }
 }
@@ -1155,18 +1176,19 @@ __global__ void k_scatter_soa(const double *src, double *dst, const int *perm, i
 template <typename T> __global__ void k_rec_from_W(const double *W, T *rec, T *wc, int n) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  T *r = rec + (size_t)c * REC;
+  T *r = rec + (size_t)c * rec_len<T>();
   T *w = wc + (size_t)c * RWC;
-  for (int k = 0; k < REC; k++) r[k] = T(0.0);
+  for (int k = 0; k < rec_len<T>(); k++) r[k] = T(0.0);
   for (int k = 0; k < RWC; k++) w[k] = T(0.0);
-  for (int k = 0; k < NF; k++) r[R_W + k] = w[k] = (T)SF(W, k, c, n);
+  for (int k = 0; k < NF; k++) r[k] = w[k] = (T)SF(W, k, c, n);
 }
+// ten values per cell, `stride` apart, at offset `off` -> a mesh-order field
 template <typename T, typename S>
-__global__ void k_rec_to_field(const T *rec, int off, S *dst, const int *perm, int n) {
+__global__ void k_rec_to_field(const T *rec, int stride, int off, S *dst, const int *perm, int n) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const int o = perm[c];
-  for (int k = 0; k < NF; k++) SF(dst, k, o, n) = (S)rec[(size_t)c * REC + off + k];
+  for (int k = 0; k < NF; k++) SF(dst, k, o, n) = (S)rec[(size_t)c * stride + off + k];
 }
 
 // ===========================================================================
@@ -1195,6 +1217,7 @@ template <typename T> FusedArgs<T> fused_args(fvm_ctx *c, double dtmax) {
   A.rec_next = (T *)z.rec[z.cur ^ 1].p;
   A.wcur = (const T *)z.wc[z.cur].p;
   A.wnext = (T *)z.wc[z.cur ^ 1].p;
+  A.tv = (T *)z.tv.p;
   A.W64 = (double *)z.W64.p;
   A.U = (double *)z.U.p;
   A.mm = (T *)z.mm.p;
@@ -1252,8 +1275,9 @@ template <typename T> int fused_setup_t(fvm_ctx *c) {
       (rc = fvm_ensure(c, z.inv, ni)) || (rc = fvm_ensure(c, z.deg, ni)) ||
       (rc = fvm_ensure(c, z.fe, sizeof(FaceEndRec<T>) * z.slots)) || (rc = fvm_ensure(c, z.U, NF * nd)) ||
       (rc = fvm_ensure(c, z.W64, NF * nd)) || (rc = fvm_ensure(c, z.cvol, nd)) ||
-      (rc = fvm_ensure(c, z.mvel, 3 * nd)) || (rc = fvm_ensure(c, z.rec[0], es * REC * (size_t)n)) ||
-      (rc = fvm_ensure(c, z.rec[1], es * REC * (size_t)n)) || (rc = fvm_ensure(c, z.mm, es * 20 * (size_t)(es == 8 ? n : 1)))  /* FP64 path only */ ||
+      (rc = fvm_ensure(c, z.mvel, 3 * nd)) || (rc = fvm_ensure(c, z.rec[0], es * rec_len<T>() * (size_t)n)) ||
+      (rc = fvm_ensure(c, z.rec[1], es * rec_len<T>() * (size_t)n)) ||
+      (rc = fvm_ensure(c, z.tv, es * NF * (size_t)(es == 4 ? n : 1)))  /* FP32 path only */ || (rc = fvm_ensure(c, z.mm, es * 20 * (size_t)(es == 8 ? n : 1)))  /* FP64 path only */ ||
       (rc = fvm_ensure(c, z.wc[0], es * RWC * (size_t)n)) || (rc = fvm_ensure(c, z.wc[1], es * RWC * (size_t)n)) ||
       (rc = fvm_ensure(c, z.dt_new, nd)) || (rc = fvm_ensure(c, z.dtmin, sizeof(unsigned long long))) ||
       (rc = fvm_ensure(c, z.tdev, sizeof(double) * 2)))
@@ -1314,14 +1338,12 @@ template <typename T, bool MOVING, int LIM> int launch_step(fvm_ctx *c, const Fu
   if (ev) (void)hipEventRecord(ev[0], c->stream);
   hipLaunchKernelGGL((k_cfl_predict_recon<T, MOVING, LIM>), dim3(grid), dim3(FB), 0, c->stream, A);
   if (ev) (void)hipEventRecord(ev[1], c->stream);
-  if constexpr (LIM != LIM_SHIPPED) {
-    if constexpr (std::is_same<T, float>::value) {
-      FusedArgs<T> A2 = A;  // the check-and-redo pass of the limiters' dt
-      A2.k1_pass = 1;
-      hipLaunchKernelGGL((k_cfl_predict_recon<T, MOVING, LIM>), dim3(grid), dim3(FB), 0, c->stream, A2);
-    } else {
-      hipLaunchKernelGGL((k_limit<T, LIM>), dim3(grid), dim3(FB), 0, c->stream, A);
-    }
+  if constexpr (std::is_same<T, float>::value) {
+    FusedArgs<T> A2 = A;  // the check-and-redo pass of the step's dt
+    A2.k1_pass = 1;
+    hipLaunchKernelGGL((k_cfl_predict_recon<T, MOVING, LIM>), dim3(grid), dim3(FB), 0, c->stream, A2);
+  } else if constexpr (LIM != LIM_SHIPPED) {
+    hipLaunchKernelGGL((k_limit<T, LIM>), dim3(grid), dim3(FB), 0, c->stream, A);
   }
   if (ev) (void)hipEventRecord(ev[2], c->stream);
   if constexpr (std::is_same<T, float>::value) {
@@ -1333,7 +1355,7 @@ template <typename T, bool MOVING, int LIM> int launch_step(fvm_ctx *c, const Fu
         const int rc = sg_step_launch(c, A);
         if (rc) return rc;
       }
-    } else if (!addr64 && (size_t)c->np * REC * sizeof(float) <= (size_t(1) << 32))
+    } else if (!addr64 && (size_t)c->np * REC32 * sizeof(float) <= (size_t(1) << 32))
       hipLaunchKernelGGL((k_flux_apply_coop<MOVING, true>), dim3(grid), dim3(FB), 0, c->stream, A);
     else
       hipLaunchKernelGGL((k_flux_apply_coop<MOVING, false>), dim3(grid), dim3(FB), 0, c->stream, A);
@@ -1401,10 +1423,13 @@ template <typename T, typename S> int fused_sync_t(fvm_ctx *c) {
   // the last completed step's gradients sit in the records it read (z.cur ^ 1)
   const T *rec = (const T *)z.rec[z.cur ^ 1].p;
   const int dstf[4] = {FVM_FIELD_GRAD_X, FVM_FIELD_GRAD_Y, FVM_FIELD_GRAD_Z, FVM_FIELD_DWDT};
-  const int off[4] = {R_G, R_G + NF, R_G + 2 * NF, R_T};
+  constexpr int G = sizeof(T) == 4 ? R32_G : R_G;
+  const int off[4] = {G, G + NF, G + 2 * NF, R_T};
   for (int q = 0; q < 4; q++) {
-    hipLaunchKernelGGL((k_rec_to_field<T, S>), dim3(nb), dim3(256), 0, c->stream, rec, off[q],
-                       (S *)c->field[dstf[q]].p, pp, n);
+    // FP32: Wt sits in its own array
+    const bool own = sizeof(T) == 4 && q == 3;
+    hipLaunchKernelGGL((k_rec_to_field<T, S>), dim3(nb), dim3(256), 0, c->stream, own ? (const T *)z.tv.p : rec,
+                       own ? NF : rec_len<T>(), own ? 0 : off[q], (S *)c->field[dstf[q]].p, pp, n);
     FVM_HIPCHK(c, hipGetLastError());
   }
   hipLaunchKernelGGL(k_scatter_soa, dim3(nb), dim3(256), 0, c->stream, (const double *)z.dt_new.p,

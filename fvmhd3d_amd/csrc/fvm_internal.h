@@ -49,7 +49,7 @@ struct FusedState {
   DBuf fdr;  // FP32 path: own-side centroid offset per face-end slot (3 floats)
   DBuf U, W64, cvol, mvel;  // FP64 state and geometry in internal order
   DBuf lmin;                // (3V/4pi)^(1/3) per cell (the volume is fixed on this path)
-  DBuf rec[2], wc[2], mm, dt_new, dtmin, tdev, hist;
+  DBuf rec[2], wc[2], tv, mm, dt_new, dtmin, tdev, hist;
   size_t slots = 0;
   // the learned HLLD surrogate in the flux step (FP32 path, fvm_sg_step.h):
   // every face once, L face-end records, face of each slot, flux records

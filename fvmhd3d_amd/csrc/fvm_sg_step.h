@@ -98,7 +98,7 @@ __device__ __forceinline__ void frame32(float gamma, const float *WL, const floa
 }
 
 // one extrapolated face state from a gathered record (the flux kernel's operations)
-__device__ __forceinline__ void face_state_rec(const float *__restrict__ rec, float dti, const float *dr, float *Wf) {
+__device__ __forceinline__ void face_state_rec(const float *__restrict__ rec, const float *dr, float *Wf) {
   float x[4 * GQ];
 #pragma unroll
   for (int k = 0; k < GQ; k++) {
@@ -106,7 +106,7 @@ __device__ __forceinline__ void face_state_rec(const float *__restrict__ rec, fl
     x[4 * k] = v.x; x[4 * k + 1] = v.y; x[4 * k + 2] = v.z; x[4 * k + 3] = v.w;
   }
   for (int k = 0; k < NF; k++)
-    Wf[k] = face_state32(half_step32(x[R_W + k], x[R_T + k], dti), x[R_G + k], x[R_G + NF + k], x[R_G + 2 * NF + k], dr);
+    Wf[k] = face_state32(x[k], x[R32_G + k], x[R32_G + NF + k], x[R32_G + 2 * NF + k], dr);
 }
 
 // Faces in batches of 64 (one per wavefront).  Block b runs on XCD b % 8 and
@@ -127,8 +127,6 @@ __global__ void __launch_bounds__(SGB) k_sg_faces(FusedArgs<float> A, SgStepArgs
   __bf16 *feat = reinterpret_cast<__bf16 *>(smem + sg::P_BYTES + wv * SG_FEAT_B);
   float *out = reinterpret_cast<float *>(feat);
 
-  const StepTimes st = step_times(A);
-  const float dti = (float)st.dti;
   const float g = (float)A.gamma;
   const int nbatch = (S.nface + 63) >> 6;
   const int bpx = (int)gridDim.x >> 3;  // blocks per XCD
@@ -151,8 +149,8 @@ __global__ void __launch_bounds__(SGB) k_sg_faces(FusedArgs<float> A, SgStepArgs
       FE<float> fe;
       decode_fe(S.face[f], fe);
       float WL[NF], WR[NF];
-      face_state_rec(A.rec + (size_t)L * REC, dti, fe.drL, WL);
-      face_state_rec(A.rec + (size_t)fe.j * REC, dti, fe.drR, WR);
+      face_state_rec(A.rec + (size_t)L * REC32, fe.drL, WL);
+      face_state_rec(A.rec + (size_t)fe.j * REC32, fe.drR, WR);
       nbad += (!(WL[DENS] > 0.0f) || !(WL[ETHM] > 0.0f) || !(WR[DENS] > 0.0f) || !(WR[ETHM] > 0.0f)) ? 1 : 0;
       for (int d = 0; d < 3; d++) nrm[d] = fe.nrm[d];
       area = fe.area;
@@ -215,8 +213,8 @@ __global__ void __launch_bounds__(SGB) k_sg_faces(FusedArgs<float> A, SgStepArgs
       FE<float> fe;
       decode_fe(S.face[f], fe);
       float WL[NF], WR[NF];
-      face_state_rec(A.rec + (size_t)L * REC, dti, fe.drL, WL);
-      face_state_rec(A.rec + (size_t)fe.j * REC, dti, fe.drR, WR);
+      face_state_rec(A.rec + (size_t)L * REC32, fe.drL, WL);
+      face_state_rec(A.rec + (size_t)fe.j * REC32, fe.drR, WR);
       const float wij[3] = {0.0f, 0.0f, 0.0f};
       compute_flux<float>(g, wij, WL, WR, NO_BOUNDARY, nrm, psi, Bn, flux);
       nfb++;
@@ -258,7 +256,7 @@ __global__ void __launch_bounds__(FB) k_sg_cells(FusedArgs<float> A, SgStepArgs 
     dv = dv - (double)Bn * sa;
   }
   float tc[NF];
-  ldv<NF>(A.rec + (size_t)c * REC + R_T, tc);
+  ldv<NF>(A.tv + (size_t)c * NF, tc);
   cell_update<float>(A, c, st.t0, st.dt_glob, dU, gp, dv, tc);
 }
 
