@@ -9,7 +9,7 @@
 //   K1b k_limit             opt-in limiters only: FP64 slopeLimiter + apply;
 //        FP32 both limiters from one pass over 12-byte own-side offsets
 //   K2 k_flux_apply (FP64) / k_flux_apply_coop (FP32, neighbour records
-//        gathered cooperatively into LDS one face-end step ahead), one
+//        gathered cooperatively into LDS two face-end steps ahead), one
 //        thread per cell, one pass over its face-ends: space-time
 //        extrapolation + HLLD (computeFluidUpdate), FP64 sums of flux*dt*A in
 //        the cell's face order, then computePrimitives (FP64 pressure recovery,
@@ -928,20 +928,21 @@ __global__ void __launch_bounds__(FB) k_flux_apply(FusedArgs<T> A) {
 }
 
 // FP32 K2 with a cooperative, asynchronous neighbour gather.  At face-end
-// step q the wavefront needs 64 neighbour records.  Lane-per-record loads
-// touch 64 cache lines per instruction and saturate the vector-memory address
-// unit; here 16 lanes fetch one whole 208-byte record (lane l of load k: a
-// 16-byte slot of record 4k + l / 16; the three lanes past the record's 13
-// slots fetch one of its slots again), so each load instruction reads 4
-// records.  The loads go straight to LDS (global_load_lds_dwordx4: lane l of
-// load k lands at byte 1024k + 16l of the wavefront's buffer, i.e. record r at
-// 256r) and are issued for step q+1 as soon as every lane has read its step-q
-// record, so the gather latency overlaps the HLLD of step q.  Slot s of record
-// r is stored at position s ^ (r % 16) of the record's 256 LDS bytes: the 16
-// lanes served by one LDS cycle then read distinct banks, and with the LDS
-// records 256-byte aligned a lane addresses slot s as (its base | 16 (r % 16))
-// ^ 16 s, one instruction per read.
+// step q the wavefront needs 64 neighbour records of 160 bytes (10 16-byte
+// slots).  Lane-per-record loads touch 64 records per instruction and saturate
+// the vector-memory address unit; here the 64 records of a step are fetched as
+// one stream of 640 slots, 64 per load: lane l of load k fetches slot
+// (64k + l) % 10 of record (64k + l) / 10, so each load instruction reads 6.4
+// whole records contiguously, and the loads go straight to LDS
+// (global_load_lds_dwordx4: lane l of load k lands at byte 1024k + 16l of the
+// buffer, i.e. record r at 160r, packed).  Each wavefront has two buffers: the
+// gather of step q+2 is issued as soon as every lane has read its step-q
+// record, so a gather has two HLLD steps to arrive.  Reading its record, lane
+// l addresses slot s as its base 160l plus the instruction's constant 16s;
+// lanes l and l + 8 of a 16-lane LDS cycle then share banks (a two-way
+// conflict on 10 reads per step, small beside the step's HLLD).
 constexpr int GQ = REC32 / 4;  // float4 of an FP32 record (all of it)
+constexpr int GBUF = 64 * GQ;  // float4 of one step's buffer (10 KiB)
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) void glb_void_t;
@@ -954,35 +955,50 @@ __device__ __forceinline__ void wave_lds_sync() {
 // s_waitcnt immediates (gfx9 encoding: vmcnt [3:0]+[15:14], expcnt [6:4], lgkmcnt [11:8])
 constexpr int WAIT_ALL = 0;           // vmcnt(0) expcnt(0) lgkmcnt(0)
 constexpr int WAIT_LGKM = 0xC07F;     // lgkmcnt(0) only
+constexpr int WAIT_VM0 = 0x0F70;      // vmcnt(0) only
+constexpr int WAIT_VMGQ = 0x0F70 | GQ;  // vmcnt(GQ): all but the last GQ vector-memory operations
+static_assert(GQ < 16, "vmcnt immediate");
 
 // gather of one face-end step: lane l asks for the record of neighbour jsel
 // (>= 0; lanes without a face-end ask for record 0, so that every lane issues
-// the same loads).  buf is the wavefront's buffer (uniform).  Record r = 4k +
-// l / 16 of load k has r % 16 = 4 (k % 4) + l / 16.
-// OFF32: the records span at most 4 GiB (n <= 2^24 cells), so a load
-// addresses them as the uniform base plus a 32-bit byte offset.
+// the same loads).  buf is the wavefront's buffer of the step (uniform).
+// Position 64k + l = 60k + t (t = 4k + l < 100) is slot t % 10 of record
+// 6k + t / 10.
+// Two forms.  Up to 2^24 cells (records under 2.7 GB) they are read as a
+// buffer (rs: base and byte size) at a 32-bit byte offset, one 24-bit
+// multiply-add per load; a load past the size reads zeros.  Larger sets use
+// 64-bit global addresses.  The buffer form also keeps the compiler's wait
+// counting exact: it treats a pending global (FLAT-encoded) load into LDS as
+// possibly out of order and waits for every load before using any register
+// loaded after it, which would also wait for the gather of the step after
+// next.
+struct GatherSrc {
+  __amdgpu_buffer_rsrc_t rs;
+  const float *rec;
+};
 template <bool OFF32>
-__device__ __forceinline__ void gather_async(const float *__restrict__ rec, int jsel, float4 *buf, int lane) {
-  const int rr = lane >> 4, sl = lane & 15;
-  int jr[16];
-  // source lane 4k + rr: a byte address rr * 4 plus the constant 16 k, which
-  // the instruction takes as its offset
+__device__ __forceinline__ void gather_async(const GatherSrc &G, int jsel, float4 *buf, int lane) {
+  // every lane's neighbour index first (the permutes overlap one another),
+  // then the loads
+  int jr[GQ];
 #pragma unroll
-  for (int k = 0; k < 16; k++) jr[k] = __builtin_amdgcn_ds_bpermute((rr << 2) + 16 * k, jsel);
+  for (int k = 0; k < GQ; k++) {
+    // record 6k + t / 10 belongs to that lane: byte address 4 (t / 10) plus
+    // the constant 24 k, which the instruction takes as its offset
+    const unsigned t = (unsigned)(4 * k + lane);
+    jr[k] = __builtin_amdgcn_ds_bpermute((int)(4u * (t / 10u)) + 24 * k, jsel);
+  }
 #pragma unroll
-  for (int k = 0; k < 16; k++) {
-    // slots 13..15 of a 16-slot LDS record are unused: their lanes fetch a
-    // slot of the same record again, so every lane issues the same loads and
-    // stays inside the record
-    const unsigned slot0 = (unsigned)(sl ^ (4 * (k & 3) + rr));
-    const unsigned slot = slot0 < (unsigned)GQ ? slot0 : slot0 - (unsigned)GQ;
-    const float *src;
-    if constexpr (OFF32)  // record index < 2^24: one 24-bit multiply-add
-      src = reinterpret_cast<const float *>(reinterpret_cast<const char *>(rec) +
-                                            (__umul24((unsigned)jr[k], (unsigned)(REC32 * sizeof(float))) + 16u * slot));
-    else
-      src = rec + 4 * slot + (size_t)(unsigned)jr[k] * REC32;
-    __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)(buf + 64 * k), 16, 0, 0);
+  for (int k = 0; k < GQ; k++) {
+    const unsigned t = (unsigned)(4 * k + lane);
+    const unsigned slot = t - 10u * (t / 10u);
+    if constexpr (OFF32) {
+      const unsigned off = __umul24((unsigned)jr[k], (unsigned)(REC32 * sizeof(float))) + 16u * slot;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(G.rs, (lds_void_t *)(buf + 64 * k), 16, (int)off, 0, 0, 0);
+    } else {
+      const float *src = G.rec + 4 * slot + (size_t)(unsigned)jr[k] * REC32;
+      __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)(buf + 64 * k), 16, 0, 0);
+    }
   }
 }
 
@@ -996,14 +1012,34 @@ __device__ __forceinline__ float face_state32(float Wh, float g0, float g1, floa
   return __builtin_fmaf(g2, dr[2], __builtin_fmaf(g1, dr[1], __builtin_fmaf(g0, dr[0], Wh)));
 }
 
+// a face-end record as the two 16-byte vectors it is loaded as: a record
+// carried across loop iterations in this form stays in the registers its
+// load writes
+struct FeRaw {
+  f4v_t a, b;
+};
+__device__ __forceinline__ FeRaw fe_raw(const FaceEndRec<float> *fe, int slot) {
+  const f4v_t *p = reinterpret_cast<const f4v_t *>(fe + slot);
+  return FeRaw{p[0], p[1]};
+}
+__device__ __forceinline__ FaceEndRec<float> fe_rec(const FeRaw &r) {
+  FaceEndRec<float> x;
+  x.nbr = __float_as_int(r.a.x);
+  x.nrm[0] = r.a.y; x.nrm[1] = r.a.z; x.nrm[2] = r.a.w;
+  x.drL[0] = r.b.x; x.drL[1] = r.b.y; x.drL[2] = r.b.z;
+  x.area = r.b.w;
+  return x;
+}
+
 template <bool MOVING, bool OFF32>
 __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
   using T = float;
-  __shared__ __attribute__((aligned(256))) float4 sbuf[FB / 64][64 * 16];
-  float4 *buf = sbuf[__builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6)];
+  __shared__ __attribute__((aligned(256))) float4 sbuf[FB / 64][2][GBUF];
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  float4 *buf0 = sbuf[wv][0], *buf1 = sbuf[wv][1];
   const int lane = (int)__lane_id();
-  // LDS byte address of this lane's record with its swizzle folded in
-  const unsigned myb = (unsigned)(size_t)(lds_cf4_t *)(buf + 16 * lane) | ((unsigned)(lane & 15) << 4);
+  // LDS byte address of this lane's record in the first buffer
+  const unsigned myb = (unsigned)(size_t)(lds_cf4_t *)(buf0 + GQ * lane);
   const int c = xcd_cell(A.n);
   const bool live = c < A.n;
   // a wavefront is 64 consecutive cells of one 64-cell group; lanes past the
@@ -1014,15 +1050,23 @@ __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
   maxdeg = __builtin_amdgcn_readfirstlane(maxdeg);
   if (maxdeg == 0 && !live) return;  // whole wavefront past the last cell (uniform)
 
-  // step 0's gather; every later one is issued by the loop, which waits for
-  // each before it ends, so no load into LDS is in flight when the wave exits
+  // gathers of steps 0 and 1; every later one is issued by the loop, and the
+  // wave waits for all of them after it, so no load into LDS is in flight
+  // when the wave exits.  A load whose value a later step needs is issued before the
+  // gather that precedes that step and lands in the register that consumes
+  // it: no copy of a load in flight (which would wait for it, and for every
+  // gather issued after it) is ever made.  rcur holds the face-end record of
+  // the step, jn the neighbour of the step after next.  Lanes past their
+  // degree load slot 0 (always present), so that every load is unconditional.
+  GatherSrc G;
+  G.rec = A.rec;
+  if constexpr (OFF32)  // raw buffer: stride 0, size in bytes, the gfx9 format word
+    G.rs = __builtin_amdgcn_make_buffer_rsrc((void *)A.rec, (short)0, (int)((unsigned)A.n * (unsigned)(REC32 * sizeof(float))),
+                                             0x00020000);
   const int e0 = live ? A.gbase[c >> 6] + (c & 63) : 0;
-  FaceEndRec<T> rcur, rn;
-  if (deg > 0) rcur = A.fe[e0];
-  if (maxdeg > 0) gather_async<OFF32>(A.rec, deg > 0 ? fe_nbr(rcur.nbr) : 0, buf, lane);
-  if (deg > 1) rn = A.fe[e0 + 64];
-
-  const StepTimes st = step_times(A);
+  FeRaw rcur = fe_raw(A.fe, deg > 0 ? e0 : 0);
+  const int j1 = fe_nbr(A.fe[deg > 1 ? e0 + 64 : 0].nbr);
+  int jn = A.fe[deg > 2 ? e0 + 128 : 0].nbr;  // raw, decoded where it is used
   // own state at the half step, extrapolated per face-end along drL / drR
   T Wc[NF], gc[3 * NF];
   if (live) {
@@ -1030,37 +1074,45 @@ __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
     ldv<NF>(rc, Wc);
     ldv<3 * NF>(rc + R32_G, gc);
   }
+  gather_async<OFF32>(G, deg > 0 ? fe_nbr(fe_rec(rcur).nbr) : 0, buf0, lane);
+  gather_async<OFF32>(G, deg > 1 ? j1 : 0, buf1, lane);
+
+  const StepTimes st = step_times(A);
   double dU[NF], gp[3], dv = 0.0;
   for (int k = 0; k < NF; k++) dU[k] = 0.0;
   gp[0] = gp[1] = gp[2] = 0.0;
   int nbad = 0;
 
+  // Every step issues a gather, so that the loads after any load of the loop
+  // are the same on every path: the last two fetch record 0 into buffers that
+  // are read no more.
   for (int q = 0, e = e0; q < maxdeg; q++, e += 64) {
-    __builtin_amdgcn_s_waitcnt(WAIT_ALL);  // step q's records are in LDS
+    // step q's records are in LDS: all but the last GQ loads (step q+1's
+    // gather, issued after step q's) have completed
+    __builtin_amdgcn_s_waitcnt(WAIT_VMGQ);
     wave_lds_sync();
     const bool act = q < deg;
+    const unsigned rb = myb + (unsigned)(q & 1) * (unsigned)(GBUF * sizeof(float4));
     FE<T> fe;
+    decode_fe(fe_rec(rcur), fe);
     T Wn[NF];
     if (act) {
-      decode_fe(rcur, fe);
       T x[4 * GQ];
 #pragma unroll
       for (int k = 0; k < GQ; k++) {
-        const f4v_t w = *(lds_cf4_t *)(size_t)(myb ^ (16u * (unsigned)k));
+        const f4v_t w = *(lds_cf4_t *)(size_t)(rb + 16u * (unsigned)k);
         x[4 * k] = w.x; x[4 * k + 1] = w.y; x[4 * k + 2] = w.z; x[4 * k + 3] = w.w;
       }
       const T *dr = fe.isL ? fe.drR : fe.drL;
       for (int k = 0; k < NF; k++)
         Wn[k] = face_state32(x[k], x[R32_G + k], x[R32_G + NF + k], x[R32_G + 2 * NF + k], dr);
     }
-    __builtin_amdgcn_s_waitcnt(WAIT_LGKM);  // every lane's reads of buf are done
+    __builtin_amdgcn_s_waitcnt(WAIT_LGKM);  // every lane's reads of the buffer are done
     wave_lds_sync();
-    if (q + 1 < maxdeg) {
-      const FaceEndRec<T> rnext = rn;
-      if (q + 2 < deg) rn = A.fe[e + 128];
-      gather_async<OFF32>(A.rec, q + 1 < deg ? fe_nbr(rnext.nbr) : 0, buf, lane);
-      rcur = rnext;
-    }
+    rcur = fe_raw(A.fe, q + 1 < deg ? e + 64 : 0);
+    const int jnext = A.fe[q + 3 < deg ? e + 192 : 0].nbr;
+    gather_async<OFF32>(G, q + 2 < deg ? fe_nbr(jn) : 0, (q & 1) ? buf1 : buf0, lane);
+    jn = jnext;
     if (act) {
       T Wo[NF];
       const T *dr = fe.isL ? fe.drL : fe.drR;
@@ -1068,6 +1120,7 @@ __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
       face_update<T, MOVING>(A, c, fe, Wn, Wo, st.dtf, dU, gp, dv, nbad);
     }
   }
+  __builtin_amdgcn_s_waitcnt(WAIT_VM0);  // no load into LDS in flight past here
   wave_add(&A.diag[FVM_DIAG_POSITIVITY], nbad);
   if (live) {
     T tc[NF];  // Wt of the step, for the Powell and Dedner terms
@@ -1355,7 +1408,7 @@ template <typename T, bool MOVING, int LIM> int launch_step(fvm_ctx *c, const Fu
         const int rc = sg_step_launch(c, A);
         if (rc) return rc;
       }
-    } else if (!addr64 && (size_t)c->np * REC32 * sizeof(float) <= (size_t(1) << 32))
+    } else if (!addr64 && c->np <= (1 << 24))  // 24-bit record indices, < 2.7 GB of records
       hipLaunchKernelGGL((k_flux_apply_coop<MOVING, true>), dim3(grid), dim3(FB), 0, c->stream, A);
     else
       hipLaunchKernelGGL((k_flux_apply_coop<MOVING, false>), dim3(grid), dim3(FB), 0, c->stream, A);

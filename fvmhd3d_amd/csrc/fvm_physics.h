@@ -147,18 +147,10 @@ __device__ __forceinline__ void riemann_hlld(T gamma_gas, T *flux, const T Bx, c
   const T c0 = T(0.0), c1 = T(1.0);
   const T signBx = Bx == c0 ? c0 : (Bx > c0 ? +c1 : -c1);
   const T momx_L = dens_L * velx_L;
-  const T momy_L = dens_L * vely_L;
-  const T momz_L = dens_L * velz_L;
   const T momx_R = dens_R * velx_R;
-  const T momy_R = dens_R * vely_R;
-  const T momz_R = dens_R * velz_R;
   const T chalf = T(0.5);
   const T B2_L = sqr(Bx) + sqr(By_L) + sqr(Bz_L);
-  const T v2_L = sqr(velx_L) + sqr(vely_L) + sqr(velz_L);
-  const T etot_L = ethm_L + chalf * (dens_L * v2_L + B2_L);
   const T B2_R = sqr(Bx) + sqr(By_R) + sqr(Bz_R);
-  const T v2_R = sqr(velx_R) + sqr(vely_R) + sqr(velz_R);
-  const T etot_R = ethm_R + chalf * (dens_R * v2_R + B2_R);
   const T gpl = gamma_gas * pres_L;
   const T gpr = gamma_gas * pres_R;
   const T gpbl = gpl + B2_L;
@@ -175,7 +167,6 @@ __device__ __forceinline__ void riemann_hlld(T gamma_gas, T *flux, const T Bx, c
   const T ipTs = frcp((S_R - velx_R) * dens_R - (S_L - velx_L) * dens_L);
   const T pT_s = ipTs * ((S_R - velx_R) * dens_R * pT_L - (S_L - velx_L) * dens_L * pT_R +
                          dens_L * dens_R * (S_R - velx_R) * (S_L - velx_L) * (velx_R - velx_L));
-  const T velx_L_s = S_M, velx_L_ss = S_M, velx_R_s = S_M, velx_R_ss = S_M;
   const T B2x = Bx * Bx;
   const T iSLmSM = frcp(S_L - S_M);
   const T iSRmSM = frcp(S_R - S_M);
@@ -193,15 +184,6 @@ __device__ __forceinline__ void riemann_hlld(T gamma_gas, T *flux, const T Bx, c
   const T velz_R_s = velz_R - Bx * Bz_R * (S_M - velx_R) * idivR;
   const T By_R_s = By_R * (dens_R * sqr(S_R - velx_R) - B2x) * idivR;
   const T Bz_R_s = Bz_R * (dens_R * sqr(S_R - velx_R) - B2x) * idivR;
-  const T vB_L = velx_L * Bx + vely_L * By_L + velz_L * Bz_L;
-  const T vB_L_s = velx_L_s * Bx + vely_L_s * By_L_s + velz_L_s * Bz_L_s;
-  const T etot_L_s =
-      ((S_L - velx_L) * etot_L - pT_L * velx_L + pT_s * S_M + Bx * (vB_L - vB_L_s)) * iSLmSM;
-  const T vB_R = velx_R * Bx + vely_R * By_R + velz_R * Bz_R;
-  const T vB_R_s = velx_R_s * Bx + vely_R_s * By_R_s + velz_R_s * Bz_R_s;
-  const T etot_R_s =
-      ((S_R - velx_R) * etot_R - pT_R * velx_R + pT_s * S_M + Bx * (vB_R - vB_R_s)) * iSRmSM;
-  const T dens_L_ss = dens_L_s, dens_R_ss = dens_R_s;
   T sDens_L_s, sDens_R_s, S_L_s, S_R_s;
   if constexpr (std::is_same<T, float>::value) {
     // FP32: one reciprocal square root gives both sqrt(rho*) and Bx / sqrt(rho*)
@@ -225,24 +207,6 @@ __device__ __forceinline__ void riemann_hlld(T gamma_gas, T *flux, const T Bx, c
                               sDens_L_s * sDens_R_s * (vely_R_s - vely_L_s) * signBx);
   const T Bz_ss = idsqroot * (sDens_L_s * Bz_R_s + sDens_R_s * Bz_L_s +
                               sDens_L_s * sDens_R_s * (velz_R_s - velz_L_s) * signBx);
-  const T vB_L_ss = velx_L_ss * Bx + vely_ss * By_ss + velz_ss * Bz_ss;
-  const T etot_L_ss = etot_L_s - sDens_L_s * (vB_L_s - vB_L_ss) * signBx;
-  const T vB_R_ss = velx_R_ss * Bx + vely_ss * By_ss + velz_ss * Bz_ss;
-  const T etot_R_ss = etot_R_s + sDens_R_s * (vB_R_s - vB_R_ss) * signBx;
-  const T Fdens_L = dens_L * velx_L;
-  const T Fmomx_L = momx_L * velx_L + pT_L - B2x;
-  const T Fmomy_L = momy_L * velx_L - Bx * By_L;
-  const T Fmomz_L = momz_L * velx_L - Bx * Bz_L;
-  const T Fetot_L = etot_L * velx_L + pT_L * velx_L - Bx * vB_L;
-  const T Fdens_R = dens_R * velx_R;
-  const T Fmomx_R = momx_R * velx_R + pT_R - B2x;
-  const T Fmomy_R = momy_R * velx_R - Bx * By_R;
-  const T Fmomz_R = momz_R * velx_R - Bx * Bz_R;
-  const T Fetot_R = etot_R * velx_R + pT_R * velx_R - Bx * vB_R;
-  const T Fby_L = By_L * velx_L - Bx * vely_L;
-  const T Fbz_L = Bz_L * velx_L - Bx * velz_L;
-  const T Fby_R = By_R * velx_R - Bx * vely_R;
-  const T Fbz_R = Bz_R * velx_R - Bx * velz_R;
   const bool w_lt_SL = w <= S_L;
   const bool w_le_SLs = w <= S_L_s;
   const bool w_le_SM = w <= S_M;
@@ -257,34 +221,56 @@ __device__ __forceinline__ void riemann_hlld(T gamma_gas, T *flux, const T Bx, c
                                       : (w_le_SRs ? -S_R_s + S_R : (w_le_SR ? +S_R - w : cnull)))));
   const T c =
       (w_lt_SL ? -w : (w_le_SLs ? -S_L : (w_le_SM ? -S_L : (w_le_SRs ? -S_R : (w_le_SR ? -S_R : -w)))));
-  // left/right selection by the contact (w <= S_M)
-  const T fdens = w_le_SM ? Fdens_L : Fdens_R;
-  const T fetot = w_le_SM ? Fetot_L : Fetot_R;
-  const T fmomx = w_le_SM ? Fmomx_L : Fmomx_R;
-  const T fmomy = w_le_SM ? Fmomy_L : Fmomy_R;
-  const T fmomz = w_le_SM ? Fmomz_L : Fmomz_R;
-  const T fby = w_le_SM ? Fby_L : Fby_R;
-  const T fbz = w_le_SM ? Fbz_L : Fbz_R;
+  // The flux takes the states of one side of the contact (L for w <= S_M, R
+  // otherwise), so that side's terms are formed alone: its inputs are
+  // selected first.  Each term is the expression the reference evaluates for
+  // that side (momx_L = dens_L velx_L, etot_L_s, Fetot_L, ...), operation for
+  // operation, so the value is the one the reference selects.
   const T dens = w_le_SM ? dens_L : dens_R;
-  const T dens_s = w_le_SM ? dens_L_s : dens_R_s;
-  const T dens_ss = w_le_SM ? dens_L_ss : dens_R_ss;
-  const T etot = w_le_SM ? etot_L : etot_R;
-  const T etot_s = w_le_SM ? etot_L_s : etot_R_s;
-  const T etot_ss = w_le_SM ? etot_L_ss : etot_R_ss;
-  const T momx = w_le_SM ? momx_L : momx_R;
-  const T momx_s = w_le_SM ? dens_L_s * velx_L_s : dens_R_s * velx_R_s;
-  const T momx_ss = w_le_SM ? dens_L_ss * velx_L_ss : dens_R_ss * velx_R_ss;
-  const T momy = w_le_SM ? momy_L : momy_R;
-  const T momy_s = w_le_SM ? dens_L_s * vely_L_s : dens_R_s * vely_R_s;
-  const T momy_ss = w_le_SM ? dens_L_ss * vely_ss : dens_R_ss * vely_ss;
-  const T momz = w_le_SM ? momz_L : momz_R;
-  const T momz_s = w_le_SM ? dens_L_s * velz_L_s : dens_R_s * velz_R_s;
-  const T momz_ss = w_le_SM ? dens_L_ss * velz_ss : dens_R_ss * velz_ss;
+  const T velx = w_le_SM ? velx_L : velx_R;
+  const T vely = w_le_SM ? vely_L : vely_R;
+  const T velz = w_le_SM ? velz_L : velz_R;
   const T by = w_le_SM ? By_L : By_R;
-  const T by_s = w_le_SM ? By_L_s : By_R_s;
-  const T by_ss = By_ss;
   const T bz = w_le_SM ? Bz_L : Bz_R;
+  const T ethm = w_le_SM ? ethm_L : ethm_R;
+  const T B2 = w_le_SM ? B2_L : B2_R;
+  const T pT = w_le_SM ? pT_L : pT_R;
+  const T S_X = w_le_SM ? S_L : S_R;
+  const T iSXmSM = w_le_SM ? iSLmSM : iSRmSM;
+  const T dens_s = w_le_SM ? dens_L_s : dens_R_s;
+  const T vely_s = w_le_SM ? vely_L_s : vely_R_s;
+  const T velz_s = w_le_SM ? velz_L_s : velz_R_s;
+  const T by_s = w_le_SM ? By_L_s : By_R_s;
   const T bz_s = w_le_SM ? Bz_L_s : Bz_R_s;
+  const T sDens_s = w_le_SM ? sDens_L_s : sDens_R_s;
+  const T velx_s = S_M, velx_ss = S_M;
+  const T momx = w_le_SM ? momx_L : momx_R;
+  const T momy = dens * vely;
+  const T momz = dens * velz;
+  const T v2 = sqr(velx) + sqr(vely) + sqr(velz);
+  const T etot = ethm + chalf * (dens * v2 + B2);
+  const T vB = velx * Bx + vely * by + velz * bz;
+  const T vB_s = velx_s * Bx + vely_s * by_s + velz_s * bz_s;
+  const T etot_s = ((S_X - velx) * etot - pT * velx + pT_s * S_M + Bx * (vB - vB_s)) * iSXmSM;
+  const T vB_ss = velx_ss * Bx + vely_ss * By_ss + velz_ss * Bz_ss;
+  // etot_L_ss = etot_L_s - d, etot_R_ss = etot_R_s + d
+  const T dE = sDens_s * (vB_s - vB_ss) * signBx;
+  const T etot_ss = w_le_SM ? etot_s - dE : etot_s + dE;
+  const T fdens = dens * velx;
+  const T fmomx = momx * velx + pT - B2x;
+  const T fmomy = momy * velx - Bx * by;
+  const T fmomz = momz * velx - Bx * bz;
+  const T fetot = etot * velx + pT * velx - Bx * vB;
+  const T fby = by * velx - Bx * vely;
+  const T fbz = bz * velx - Bx * velz;
+  const T dens_ss = dens_s;
+  const T momx_s = dens_s * velx_s;
+  const T momx_ss = dens_ss * velx_ss;
+  const T momy_s = dens_s * vely_s;
+  const T momy_ss = dens_ss * vely_ss;
+  const T momz_s = dens_s * velz_s;
+  const T momz_ss = dens_ss * velz_ss;
+  const T by_ss = By_ss;
   const T bz_ss = Bz_ss;
 
   flux[MASS] = fdens + a * dens_ss + b * dens_s + c * dens;
@@ -337,20 +323,30 @@ __device__ __forceinline__ void compute_flux(const T gamma_gas, const T *wij, co
   T dens_L = Wi[DENS];
   T ethm_L = Wi[ETHM];
   T velx_L = Axx * Wi[VELX] + Axy * Wi[VELY] + Axz * Wi[VELZ];
-  T vely_L = Ayx * Wi[VELX] + Ayy * Wi[VELY] + Ayz * Wi[VELZ];
+  T vely_L, By_L, vely_R, By_R;
+  if constexpr (std::is_same<T, float>::value) {
+    // FP32: the second frame axis has no z component (Ayz = 0), so its term is
+    // left out rather than multiplied by zero
+    vely_L = Ayx * Wi[VELX] + Ayy * Wi[VELY];
+    By_L = Ayx * Wi[BX] + Ayy * Wi[BY];
+    vely_R = Ayx * Wj[VELX] + Ayy * Wj[VELY];
+    By_R = Ayx * Wj[BX] + Ayy * Wj[BY];
+  } else {
+    vely_L = Ayx * Wi[VELX] + Ayy * Wi[VELY] + Ayz * Wi[VELZ];
+    By_L = Ayx * Wi[BX] + Ayy * Wi[BY] + Ayz * Wi[BZ];
+    vely_R = Ayx * Wj[VELX] + Ayy * Wj[VELY] + Ayz * Wj[VELZ];
+    By_R = Ayx * Wj[BX] + Ayy * Wj[BY] + Ayz * Wj[BZ];
+  }
   T velz_L = Azx * Wi[VELX] + Azy * Wi[VELY] + Azz * Wi[VELZ];
   T Bx_L = Axx * Wi[BX] + Axy * Wi[BY] + Axz * Wi[BZ];
-  T By_L = Ayx * Wi[BX] + Ayy * Wi[BY] + Ayz * Wi[BZ];
   T Bz_L = Azx * Wi[BX] + Azy * Wi[BY] + Azz * Wi[BZ];
   T psiL = Wi[PSI];
   T entrL = Wi[ENTR];
   T dens_R = Wj[DENS];
   T ethm_R = Wj[ETHM];
   T velx_R = Axx * Wj[VELX] + Axy * Wj[VELY] + Axz * Wj[VELZ];
-  T vely_R = Ayx * Wj[VELX] + Ayy * Wj[VELY] + Ayz * Wj[VELZ];
   T velz_R = Azx * Wj[VELX] + Azy * Wj[VELY] + Azz * Wj[VELZ];
   T Bx_R = Axx * Wj[BX] + Axy * Wj[BY] + Axz * Wj[BZ];
-  T By_R = Ayx * Wj[BX] + Ayy * Wj[BY] + Ayz * Wj[BZ];
   T Bz_R = Azx * Wj[BX] + Azy * Wj[BY] + Azz * Wj[BZ];
   T psiR = Wj[PSI];
   T entrR = Wj[ENTR];
@@ -420,10 +416,16 @@ __device__ __forceinline__ void compute_flux(const T gamma_gas, const T *wij, co
   flux[ENER] = F[ENER];
   flux[MOMX] = iAxx * F[MOMX] + iAxy * F[MOMY] + iAxz * F[MOMZ];
   flux[MOMY] = iAyx * F[MOMX] + iAyy * F[MOMY] + iAyz * F[MOMZ];
-  flux[MOMZ] = iAzx * F[MOMX] + iAzy * F[MOMY] + iAzz * F[MOMZ];
   flux[WBX] = iAxx * F[WBX] + iAxy * F[WBY] + iAxz * F[WBZ];
   flux[WBY] = iAyx * F[WBX] + iAyy * F[WBY] + iAyz * F[WBZ];
-  flux[WBZ] = iAzx * F[WBX] + iAzy * F[WBY] + iAzz * F[WBZ];
+  if constexpr (std::is_same<T, float>::value) {
+    // FP32: iAzy = 0, as above
+    flux[MOMZ] = iAzx * F[MOMX] + iAzz * F[MOMZ];
+    flux[WBZ] = iAzx * F[WBX] + iAzz * F[WBZ];
+  } else {
+    flux[MOMZ] = iAzx * F[MOMX] + iAzy * F[MOMY] + iAzz * F[MOMZ];
+    flux[WBZ] = iAzx * F[WBX] + iAzy * F[WBY] + iAzz * F[WBZ];
+  }
   flux[MPSI] = F[MASS] * (F[MASS] > c0 ? psiL : psiR);
   flux[MENTR] = F[MASS] * (F[MASS] > c0 ? entrL : entrR);
 }
