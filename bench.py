@@ -244,6 +244,27 @@ def substep_bench(nsub, n=128, dtmax=1.0 / 64, host_driver=True, cpu=True):
            "active_face_fraction": [round(f / m.nface, 4) for f in faces],
            "rungs": int(D.nrung), "dtmax": dtmax, "diagnostics": D.ctx.diagnostics(),
            "workload": f"thin disk, {m.ncell} jittered lattice generators"}
+    # the same substeps with the FP32 face arithmetic (fvm_set_face_precision 32):
+    # its rate, and its state against the FP64 run's
+    D32 = DeviceSubstepper(m, W0, dtmax=dtmax, limiter=G.LIMITER_SMOOTH, face_bits=32)
+    D32.init_rungs()
+    D32.ctx.reset_diagnostics()
+    t0 = time.perf_counter()
+    for k in range(nsub):
+        D32.substep()
+    wall32 = time.perf_counter() - t0
+    f32 = [h[2] for h in D32.history]
+    l1 = lambda a, b: float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
+    U64, U32 = D.state("U").reshape(m.ncell, 10), D32.state("U").reshape(m.ncell, 10)
+    W64, W32 = D.state("W").reshape(m.ncell, 10), D32.state("W").reshape(m.ncell, 10)
+    scale = np.max(np.abs(W64), axis=0) + 1e-300
+    out["face32"] = {"value": sum(f32) / wall32, "wall_s": round(wall32, 3),
+                     "same_substeps_as_fp64": D32.history == D.history,
+                     "l1_rho_vs_fp64": l1(W32[:, 0], W64[:, 0]), "l1_E_vs_fp64": l1(U32[:, 4], U64[:, 4]),
+                     "max_rel_W_vs_fp64": float(np.max(np.abs(W32 - W64) / scale)),
+                     "diagnostics": D32.ctx.diagnostics()}
+    D32.ctx.close()
+    log(f"[substeps] FP32 faces: {out['face32']}")
     out["roofline"] = substep_roofline(m, W0, dtmax, nsub, D.history)
     if host_driver:
         S = Substepper(m, W0, dtmax=dtmax, limiter=G.LIMITER_SMOOTH)

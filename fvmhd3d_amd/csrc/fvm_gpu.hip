@@ -550,15 +550,19 @@ constexpr int NREC_F = NF + 4;
 
 // computeFluidUpdate.cpp:99-209: the inputs of computeFlux for active face a
 // (boundary swap, space-time extrapolation of both sides, face velocity)
-struct FaceIn {
+// F: the face arithmetic (double: the reference's; float: the FP32 face
+// states, formed from FP64 geometry, dt and dti as the fused FP32 path forms
+// them)
+template <typename F> struct FaceIn {
   int i, j, jbnd;
   bool sw;
-  double dt, area, normal[3], wij[3], Wi[NF], Wj[NF];
+  double dt, area, normal[3], wij[3];
+  F Wi[NF], Wj[NF];
 };
 
-template <typename S>
+template <typename S, typename F>
 __device__ __forceinline__ void face_inputs(const Cells<S> &C, const MeshView &M, const Phys &P, int a,
-                                            unsigned long long *diag, FaceIn &q) {
+                                            unsigned long long *diag, FaceIn<F> &q) {
   const int np = C.np;
   const int f = M.fact[a];
   int i = M.s1[f], j = M.s2[f];
@@ -588,21 +592,38 @@ __device__ __forceinline__ void face_inputs(const Cells<S> &C, const MeshView &M
   if (diag && !(dsl > 0.0)) atomicAdd(&diag[FVM_DIAG_DSL], 1ULL);
   const double drj[3] = {dri[0] - normal[0] * dsl, dri[1] - normal[1] * dsl, dri[2] - normal[2] * dsl};
 
-  double *Wi = q.Wi, *Wj = q.Wj;
+  F *Wi = q.Wi, *Wj = q.Wj;
   for (int k = 0; k < NF; k++) {
-    Wi[k] = FLD(C.w, k, i, np);
-    Wj[k] = FLD(C.w, k, j, np);
+    Wi[k] = (F)FLD(C.w, k, i, np);
+    Wj[k] = (F)FLD(C.w, k, j, np);
   }
   const int jbnd = C.bnd[j];
   if (jbnd == NO_BOUNDARY) {
-    for (int k = 0; k < NF; k++) {
-      Wi[k] += (double)FLD(C.x, k, i, np) * dri[0] + (double)FLD(C.y, k, i, np) * dri[1] +
-               (double)FLD(C.z, k, i, np) * dri[2] + (double)FLD(C.t, k, i, np) * dti;
-      Wj[k] += (double)FLD(C.x, k, j, np) * drj[0] + (double)FLD(C.y, k, j, np) * drj[1] +
-               (double)FLD(C.z, k, j, np) * drj[2] + (double)FLD(C.t, k, j, np) * dtj;
+    if constexpr (std::is_same<F, double>::value) {
+      for (int k = 0; k < NF; k++) {
+        Wi[k] += (double)FLD(C.x, k, i, np) * dri[0] + (double)FLD(C.y, k, i, np) * dri[1] +
+                 (double)FLD(C.z, k, i, np) * dri[2] + (double)FLD(C.t, k, i, np) * dti;
+        Wj[k] += (double)FLD(C.x, k, j, np) * drj[0] + (double)FLD(C.y, k, j, np) * drj[1] +
+                 (double)FLD(C.z, k, j, np) * drj[2] + (double)FLD(C.t, k, j, np) * dtj;
+      }
+    } else {
+      // FP32: the half-step state W + Wt dti, then the gradient terms, in FMAs
+      const float ri[3] = {(float)dri[0], (float)dri[1], (float)dri[2]};
+      const float rj[3] = {(float)drj[0], (float)drj[1], (float)drj[2]};
+      const float ti = (float)dti, tj = (float)dtj;
+      for (int k = 0; k < NF; k++) {
+        Wi[k] = __builtin_fmaf((float)FLD(C.z, k, i, np), ri[2],
+                               __builtin_fmaf((float)FLD(C.y, k, i, np), ri[1],
+                                              __builtin_fmaf((float)FLD(C.x, k, i, np), ri[0],
+                                                             __builtin_fmaf((float)FLD(C.t, k, i, np), ti, Wi[k]))));
+        Wj[k] = __builtin_fmaf((float)FLD(C.z, k, j, np), rj[2],
+                               __builtin_fmaf((float)FLD(C.y, k, j, np), rj[1],
+                                              __builtin_fmaf((float)FLD(C.x, k, j, np), rj[0],
+                                                             __builtin_fmaf((float)FLD(C.t, k, j, np), tj, Wj[k]))));
+      }
     }
   }
-  if (diag && (!(Wi[DENS] > 0.0) || !(Wi[ETHM] > 0.0) || !(Wj[DENS] > 0.0) || !(Wj[ETHM] > 0.0)))
+  if (diag && (!(Wi[DENS] > F(0.0)) || !(Wi[ETHM] > F(0.0)) || !(Wj[DENS] > F(0.0)) || !(Wj[ETHM] > F(0.0))))
     atomicAdd(&diag[FVM_DIAG_POSITIVITY], 1ULL);
 
   const double vi[3] = {(double)FLD(C.vel, 0, i, np), (double)FLD(C.vel, 1, i, np), (double)FLD(C.vel, 2, i, np)};
@@ -625,20 +646,27 @@ __device__ __forceinline__ void face_inputs(const Cells<S> &C, const MeshView &M
   q.area = area;
 }
 
-template <typename S>
+// F = float: the FP32 face arithmetic (face states and HLLD); the records
+// times dt and area, and every sum over them, stay FP64
+template <typename S, typename F>
 __global__ void __launch_bounds__(256) k_face_flux(Cells<S> C, MeshView M, Phys P, double *rec, int *swp,
                                    unsigned long long *diag) {
   const int a = xgid();
   if (a >= M.nfa) return;
-  FaceIn q;
-  face_inputs(C, M, P, a, diag, q);
-  double psi, Bn, flux[NF];
-  compute_flux<double>(P.gamma, q.wij, q.Wi, q.Wj, q.jbnd, q.normal, psi, Bn, flux);
+  FaceIn<F> q;
+  face_inputs<S, F>(C, M, P, a, diag, q);
+  F psi, Bn, flux[NF];
+  F wij[3], nrm[3];
+  for (int d = 0; d < 3; d++) {
+    wij[d] = (F)q.wij[d];
+    nrm[d] = (F)q.normal[d];
+  }
+  compute_flux<F>((F)P.gamma, wij, q.Wi, q.Wj, q.jbnd, nrm, psi, Bn, flux);
 
   const double tarea = q.dt * q.area;
-  for (int k = 0; k < NF; k++) REC(rec, k, a, NREC_F) = flux[k] * tarea;
-  for (int d = 0; d < 3; d++) REC(rec, NF + d, a, NREC_F) = q.normal[d] * (psi * tarea);
-  REC(rec, NF + 3, a, NREC_F) = Bn * tarea;
+  for (int k = 0; k < NF; k++) REC(rec, k, a, NREC_F) = (double)flux[k] * tarea;
+  for (int d = 0; d < 3; d++) REC(rec, NF + d, a, NREC_F) = q.normal[d] * ((double)psi * tarea);
+  REC(rec, NF + 3, a, NREC_F) = (double)Bn * tarea;
   swp[a] = q.sw ? 1 : 0;
 }
 
@@ -649,8 +677,8 @@ __global__ void __launch_bounds__(256) k_face_states(Cells<S> C, MeshView M, Phy
                                                      double *Wi, double *Wj, int *jbnd, double *normal) {
   const int q0 = gid();
   if (q0 >= count) return;
-  FaceIn q;
-  face_inputs(C, M, P, first + q0, nullptr, q);
+  FaceIn<double> q;
+  face_inputs<S, double>(C, M, P, first + q0, nullptr, q);
   for (int d = 0; d < 3; d++) {
     wij[3 * (size_t)q0 + d] = q.wij[d];
     normal[3 * (size_t)q0 + d] = q.normal[d];
@@ -898,7 +926,8 @@ __global__ void __launch_bounds__(256) k_timestep(Cells<S> C, MeshView M, Phys P
 // ---------------------------------------------------------------------------
 constexpr int NREC_P = 15;
 
-template <typename S>
+// F = float: the FP32 HLLD of the first-order states; the records stay FP64
+template <typename S, typename F>
 __global__ void __launch_bounds__(256) k_predictor_face(Cells<S> C, MeshView M, Phys P, double *rec, int *swp,
                                         unsigned long long *diag) {
   const int a = xgid();
@@ -942,11 +971,22 @@ __global__ void __launch_bounds__(256) k_predictor_face(Cells<S> C, MeshView M, 
   const double r = 1.0 / sqr(dsl);
   const double wij[3] = {vij[0] * 0.5 - rij[0] * fdv * r, vij[1] * 0.5 - rij[1] * fdv * r,
                          vij[2] * 0.5 - rij[2] * fdv * r};
-  double psi, Bn, flux[NF];
-  compute_flux<double>(P.gamma, wij, Wi, Wj, jbnd, normal, psi, Bn, flux);
-  for (int k = 0; k < NF; k++) REC(rec, k, a, NREC_P) = flux[k] * area;
-  for (int d = 0; d < 3; d++) REC(rec, NF + d, a, NREC_P) = normal[d] * (psi * area);
-  REC(rec, NF + 3, a, NREC_P) = Bn * area;
+  F psi, Bn, flux[NF];
+  {
+    F wf[3], nf[3], Wif[NF], Wjf[NF];
+    for (int d = 0; d < 3; d++) {
+      wf[d] = (F)wij[d];
+      nf[d] = (F)normal[d];
+    }
+    for (int k = 0; k < NF; k++) {
+      Wif[k] = (F)Wi[k];
+      Wjf[k] = (F)Wj[k];
+    }
+    compute_flux<F>((F)P.gamma, wf, Wif, Wjf, jbnd, nf, psi, Bn, flux);
+  }
+  for (int k = 0; k < NF; k++) REC(rec, k, a, NREC_P) = (double)flux[k] * area;
+  for (int d = 0; d < 3; d++) REC(rec, NF + d, a, NREC_P) = normal[d] * ((double)psi * area);
+  REC(rec, NF + 3, a, NREC_P) = (double)Bn * area;
   REC(rec, NF + 4, a, NREC_P) = area * (rij[0] * wij[0] + rij[1] * wij[1] + rij[2] * wij[2]) / dsl;
   swp[a] = sw ? 1 : 0;
 }
@@ -1926,8 +1966,12 @@ template <typename S> static int do_fluid_update(fvm_ctx *c) {
     return rc;
   unsigned long long before[FVM_NDIAG], after[FVM_NDIAG];
   if (!c->defer_sync && (rc = read_diag(c, before))) return rc;
-  LAUNCH_T(c, FVM_KT_FACE_FLUX, fvm::k_face_flux<S>, nfa, cells<S>(c), meshview(c), phys(c), (double *)c->rec.p, (int *)c->swp.p,
-         (unsigned long long *)c->diag.p);
+  if (c->face32)
+    LAUNCH_T(c, FVM_KT_FACE_FLUX, (fvm::k_face_flux<S, float>), nfa, cells<S>(c), meshview(c), phys(c), (double *)c->rec.p,
+             (int *)c->swp.p, (unsigned long long *)c->diag.p);
+  else
+    LAUNCH_T(c, FVM_KT_FACE_FLUX, (fvm::k_face_flux<S, double>), nfa, cells<S>(c), meshview(c), phys(c), (double *)c->rec.p,
+             (int *)c->swp.p, (unsigned long long *)c->diag.p);
   if (c->np >= fvm::ONE_THREAD_MIN_CELLS)
     LAUNCH_T(c, FVM_KT_SCATTER, fvm::k_scatter<S>, c->np, cells<S>(c), meshview(c), (const double *)c->rec.p,
              (const int *)c->swp.p);
@@ -1938,6 +1982,12 @@ template <typename S> static int do_fluid_update(fvm_ctx *c) {
   if ((rc = read_diag(c, after))) return rc;
   if (after[FVM_DIAG_POSITIVITY] != before[FVM_DIAG_POSITIVITY])
     return fail(c, FVM_EPOS, "computeFluidUpdate: non-positive density or thermal energy at a face");
+  return FVM_OK;
+}
+int fvm_set_face_precision(fvm_ctx *c, int bits) {
+  if (!c) return FVM_EARG;
+  if (bits != 32 && bits != 64) return fail(c, FVM_EARG, "fvm_set_face_precision: bits must be 32 or 64");
+  c->face32 = bits == 32;
   return FVM_OK;
 }
 int fvm_compute_fluid_update(fvm_ctx *c) {
@@ -2051,8 +2101,12 @@ template <typename S> static int do_predictor(fvm_ctx *c, double dtmax, int min_
   int rc;
   if ((rc = ensure(c, c->rec, sizeof(double) * fvm::NREC_P * (size_t)nfa)) || (rc = ensure(c, c->swp, sizeof(int) * (size_t)nfa)))
     return rc;
-  LAUNCH_T(c, FVM_KT_PREDICTOR_FACE, fvm::k_predictor_face<S>, nfa, cells<S>(c), meshview(c), phys(c), (double *)c->rec.p, (int *)c->swp.p,
-         (unsigned long long *)c->diag.p);
+  if (c->face32)
+    LAUNCH_T(c, FVM_KT_PREDICTOR_FACE, (fvm::k_predictor_face<S, float>), nfa, cells<S>(c), meshview(c), phys(c),
+             (double *)c->rec.p, (int *)c->swp.p, (unsigned long long *)c->diag.p);
+  else
+    LAUNCH_T(c, FVM_KT_PREDICTOR_FACE, (fvm::k_predictor_face<S, double>), nfa, cells<S>(c), meshview(c), phys(c),
+             (double *)c->rec.p, (int *)c->swp.p, (unsigned long long *)c->diag.p);
   if (c->nactive >= fvm::ONE_THREAD_MIN_CELLS)
     LAUNCH_T(c, FVM_KT_PREDICTOR_CELL, fvm::k_predictor_cell<S>, c->nactive, cells<S>(c), meshview(c), phys(c),
              (const double *)c->rec.p, (const int *)c->swp.p, dtmax, min_rung, (unsigned long long *)c->diag.p);

@@ -110,6 +110,9 @@ struct fvm_ctx {
   // synchronisation (and the fluid update its positivity read-back) to the
   // substep, which synchronises once at its end
   bool defer_sync = false;
+  // FP32 face arithmetic in the face kernels of the reference-order calls
+  // (fvm_set_face_precision)
+  bool face32 = false;
   DBuf field[FVM_NFIELDS];
   DBuf m_s1, m_s2, m_c, m_n, m_coff, m_cface, m_cvol, m_fact, m_soff, m_sent;
   DBuf m_cn;  // per-face geometry record (centroid, n) of the same faces as m_c / m_n

@@ -82,6 +82,7 @@ SIGNATURES = {
     "fvm_fused_timing": [_P, _I],
     "fvm_fused_kernel_ms": [_P, _P, _P, _I],
     "fvm_kernel_timing": [_P, _I],
+    "fvm_set_face_precision": [_P, _I],
     "fvm_kernel_ms": [_P, _P, _P, _I],
     "fvm_measure_copy_bandwidth": [_P, ctypes.c_longlong, _I, ctypes.POINTER(ctypes.c_double)],
     "fvm_measure_runtime_copy_bandwidth": [_P, ctypes.c_longlong, _I, ctypes.POINTER(ctypes.c_double)],
@@ -349,6 +350,11 @@ class Context:
         nl = np.zeros(4, dtype=np.int64)
         self._chk(self.L.fvm_fused_kernel_ms(self.h, _ptr(ms), _ptr(nl), 4))
         return ms, nl
+
+    def set_face_precision(self, bits):
+        """Face arithmetic of the fluid update and predictor calls: 64 (the
+        reference's) or 32 (FP32 face states and HLLD, FP64 sums)."""
+        self._chk(self.L.fvm_set_face_precision(self.h, int(bits)))
 
     def kernel_timing(self, enable=True):
         """HIP-event timing of the reference-order kernels (fvm_kernel_timing);

@@ -325,14 +325,17 @@ class DeviceSubstepper:
     The same substeps (Iterate.cpp:92-202) as ``Substepper`` with ``GpuBackend``,
     but the layout, the gathers, the new rungs and the write-back run on the
     device (``fvm_substep_run``); the host keeps only the rung clock, fed by
-    the device's rung histogram.  The mesh must be periodic (no boundary cells)."""
+    the device's rung histogram.  The mesh must be periodic (no boundary cells).
+    face_bits = 32 runs the face states and HLLD of the fluid update and the
+    predictor in FP32 (fvm_set_face_precision); 64 is the reference's FP64."""
 
-    def __init__(self, mesh, W, gamma=5.0 / 3.0, courant=0.4, dtmax=1.0 / 64, limiter=0, nrung=6):
+    def __init__(self, mesh, W, gamma=5.0 / 3.0, courant=0.4, dtmax=1.0 / 64, limiter=0, nrung=6, face_bits=64):
         from . import gpu as G
         self.mesh = mesh
         self.gamma, self.courant, self.limiter, self.nrung = gamma, courant, limiter, nrung
         self.clock = RungClock(dtmax)
         self.ctx = G.Context(gamma=gamma, courant=courant, box=tuple(float(b) for b in mesh.box), limiter=limiter)
+        self.ctx.set_face_precision(face_bits)
         self.ctx.substep_setup(mesh.faces, mesh.cell_off, mesh.cell_face, mesh.volume)
         W = np.array(W, dtype=np.float64)
         self.ctx.substep_upload("w", W)

@@ -249,6 +249,16 @@ int fvm_fused_kernel_ms(fvm_ctx *ctx, double *ms, long long *launches, int n);
 #define FVM_KT_PREDICTOR_FACE 6  /* k_predictor_face (computePredictor, face part) */
 #define FVM_KT_PREDICTOR_CELL 7  /* k_predictor_cell (computePredictor, cell part) */
 #define FVM_NKT 8
+/* Face arithmetic of fvm_compute_fluid_update and fvm_compute_predictor (and so
+ * of the substeps, fvm_substep_run): bits = 64 (default) is the reference's
+ * FP64; bits = 32 forms the face states (W + Wt dti + grad . dr, with dt, dti
+ * and the geometry in FP64) and runs computeFlux's HLLD in FP32, while the
+ * records times dt and area, their per-cell sums, pressure recovery and the
+ * CFL step stay FP64.  No counterpart in the reference: it selects the FP32
+ * face arithmetic of the mixed-precision plan for the individual-timestep
+ * calls.  FVM_EARG for any other bits. */
+int fvm_set_face_precision(fvm_ctx *ctx, int bits);
+
 int fvm_kernel_timing(fvm_ctx *ctx, int enable);
 int fvm_kernel_ms(fvm_ctx *ctx, double *ms, long long *launches, int n);
 

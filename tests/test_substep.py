@@ -330,6 +330,50 @@ def test_sparse_sweep_long_cascades_match_oracle(frac):
 
 
 @pytest.mark.gpu
+def test_device_substeps_face32_against_oracle():
+    """The substeps with the FP32 face arithmetic (fvm_set_face_precision 32: face
+    states and HLLD of the fluid update and the predictor in FP32; geometry, dt,
+    dti, the per-cell sums, pressure recovery and the CFL step in FP64) against
+    the FP64 oracle behind the host driver: the thin disk on 16^3 lattice sites,
+    dtmax 1/8, 10 substeps.  The substeps (active cells and faces) are the
+    oracle's; the state is within FP32 rounding of it (largest difference 1e-5
+    of each column's largest magnitude, relative L1 of density and total energy
+    1e-6); mass, applied plus pending, is conserved to FP64 roundoff.  Any face
+    precision but 32 or 64 is refused."""
+    from fvmhd3d_amd import gpu as G
+    from fvmhd3d_amd.substep import DeviceSubstepper, ic_thin_disk
+    m = workload_mesh("lattice", 16)
+    W, dtmax = ic_thin_disk(m.pos), 1.0 / 8
+    So = Substepper(m, W, dtmax=dtmax, limiter=2)
+    Bo = OracleBackend(limiter=2)
+    So.init_rungs(Bo)
+    D = DeviceSubstepper(m, W, dtmax=dtmax, limiter=2, face_bits=32)
+    with pytest.raises(G.FvmError):
+        D.ctx.set_face_precision(16)
+    D.init_rungs()
+    M0 = float(np.sum(D.state("U").reshape(m.ncell, 10)[:, 0]))
+    for k in range(10):
+        a = So.substep(Bo)
+        b = D.substep()
+        assert a == b, (k, a, b)
+    for name in ("W", "U"):
+        ref = np.asarray(getattr(So, name))
+        got = D.state(name).reshape(ref.shape)
+        scale = np.max(np.abs(ref), axis=0) + 1e-300
+        err = float(np.max(np.abs(got - ref) / scale))
+        print(f"{name}: largest difference / column scale {err:.2e}")
+        assert err <= 1e-5, (name, err)
+    l1 = lambda a, b: float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
+    Wd, Ud = D.state("W").reshape(m.ncell, 10), D.state("U").reshape(m.ncell, 10)
+    l1_rho, l1_E = l1(Wd[:, 0], So.W[:, 0]), l1(Ud[:, 4], So.U[:, 4])
+    print(f"relative L1: rho {l1_rho:.2e}, E {l1_E:.2e}")
+    assert l1_rho <= 1e-6 and l1_E <= 1e-6, (l1_rho, l1_E)
+    mass = float(np.sum(Ud[:, 0] + D.state("dU").reshape(m.ncell, 10)[:, 0]))
+    assert abs(mass - M0) <= 1e-12 * M0, (mass, M0)
+    D.ctx.close()
+
+
+@pytest.mark.gpu
 def test_device_substeps_conserve_mass_minmod():
     """Device substeps with the minmod limiter over two rung cycles: every substep's
     count matches the host driver's, and mass (applied U plus the neighbours' pending
