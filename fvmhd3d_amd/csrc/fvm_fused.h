@@ -623,7 +623,8 @@ k_cfl_predict_recon(FusedArgs<T> A) {
       const double l_min = A.lmin[c];
       Uc[MPSI] *= exp(-cr * ch_sig / l_min * dt);
       double W1[NF];
-      to_primitive(Uc, V * exp(pdV * dt / V), W1);
+      // exp(0) = 1 exactly: a static mesh skips the exponential
+      to_primitive(Uc, pdV == 0.0 ? V : V * exp(pdV * dt / V), W1);
       const bool ef = W1[ETHM] <= 0.0;
       if (ef) W1[ETHM] = ethm_from_entropy_d(g64, W1);
       else W1[ENTR] = entropy_from_ethm_d(g64, W1);
@@ -703,11 +704,17 @@ k_cfl_predict_recon(FusedArgs<T> A) {
     }
     A.dt_new[c] = dtn;
   }
-  // global minimum of dt_new (dt > 0: IEEE bit patterns order like the values)
+  // global minimum of dt_new (dt > 0: IEEE bit patterns order like the values),
+  // one atomic per block
   double v = dtn;
   for (int o = 32; o > 0; o >>= 1) v = smin(v, __shfl_xor(v, o));
-  if (__lane_id() == 0 && v < REF_HUGE && v >= 0.0)
-    atomicMin(A.dtmin_bits, (unsigned long long)__double_as_longlong(v));
+  __shared__ double wmin[FB / 64];
+  if (__lane_id() == 0) wmin[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < FB / 64; w++) v = smin(v, wmin[w]);
+    if (v < REF_HUGE && v >= 0.0) atomicMin(A.dtmin_bits, (unsigned long long)__double_as_longlong(v));
+  }
 }
 
 // ---------------------------------------------------------------------------
