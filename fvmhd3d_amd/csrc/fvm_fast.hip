@@ -10,7 +10,7 @@
 
 void fvm_fast_release(fvm_ctx *c) {
   FusedState &z = c->fz;
-  for (DBuf *b : {&z.perm, &z.inv, &z.deg, &z.gbase, &z.fe, &z.fdr, &z.lmin, &z.U, &z.W64, &z.cvol, &z.mvel, &z.rec[0], &z.rec[1], &z.wc[0], &z.wc[1], &z.tv, &z.mm,
+  for (DBuf *b : {&z.perm, &z.inv, &z.deg, &z.gbase, &z.fe, &z.fdr, &z.fnb, &z.lmin, &z.U, &z.W64, &z.cvol, &z.mvel, &z.rec[0], &z.rec[1], &z.wc[0], &z.wc[1], &z.tv, &z.mm,
                   &z.dt_new, &z.dtmin, &z.tdev, &z.hist, &z.sg_faceL, &z.sg_face, &z.sg_fid, &z.sg_flux})
     fvm_release(*b);
   z.sg = nullptr;

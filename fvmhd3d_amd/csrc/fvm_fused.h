@@ -117,6 +117,7 @@ template <typename T> struct FusedArgs {
   const int *gbase;         // [n/64] first slot of each 64-cell group in fe
   const FaceEndRec<T> *fe;  // SELL: face-end k of cell c at gbase[c/64] + 64*k + c%64
   const float *fdr;         // FP32: own-side centroid offset of each face-end slot, [slot][3]
+  const int *fnb;           // FP32: the neighbour field of each face-end slot (fe[slot].nbr), compact
   const double *cvol;       // [n]
   const double *lmin;       // [n] pow(V / (4 pi / 3), 1 / 3), formed once at setup
   const double *mvel;       // [3][n] generator velocity (moving mesh only)
@@ -1072,8 +1073,10 @@ __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
                                              0x00020000);
   const int e0 = live ? A.gbase[c >> 6] + (c & 63) : 0;
   FeRaw rcur = fe_raw(A.fe, deg > 0 ? e0 : 0);
-  const int j1 = fe_nbr(A.fe[deg > 1 ? e0 + 64 : 0].nbr);
-  int jn = A.fe[deg > 2 ? e0 + 128 : 0].nbr;  // raw, decoded where it is used
+  // the neighbours of later steps come from the compact neighbour array: 256
+  // contiguous bytes per wavefront and step, against 2 KiB of records
+  const int j1 = fe_nbr(A.fnb[deg > 1 ? e0 + 64 : 0]);
+  int jn = A.fnb[deg > 2 ? e0 + 128 : 0];  // raw, decoded where it is used
   // own state at the half step, extrapolated per face-end along drL / drR
   T Wc[NF], gc[3 * NF];
   if (live) {
@@ -1117,7 +1120,7 @@ __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
     __builtin_amdgcn_s_waitcnt(WAIT_LGKM);  // every lane's reads of the buffer are done
     wave_lds_sync();
     rcur = fe_raw(A.fe, q + 1 < deg ? e + 64 : 0);
-    const int jnext = A.fe[q + 3 < deg ? e + 192 : 0].nbr;
+    const int jnext = A.fnb[q + 3 < deg ? e + 192 : 0];
     gather_async<OFF32>(G, q + 2 < deg ? fe_nbr(jn) : 0, (q & 1) ? buf1 : buf0, lane);
     jn = jnext;
     if (act) {
@@ -1196,12 +1199,14 @@ __global__ void k_build_faceends(int n, const int *perm, const int *inv, const i
 }
 
 // FP32: own-side centroid offset of every face-end slot (drL or drR as the
-// kernels decode them); unused slots of short cells stay unwritten
-__global__ void k_build_dr(int n, const int *deg, const int *gbase, const FaceEndRec<float> *fe, float *fdr) {
+// kernels decode them) and its neighbour field; unused slots of short cells
+// keep their offsets unwritten and neighbour 0
+__global__ void k_build_dr(int n, const int *deg, const int *gbase, const FaceEndRec<float> *fe, float *fdr, int *fnb) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int q = 0; q < deg[i]; q++) {
     const int slot = gbase[i >> 6] + 64 * q + (i & 63);
+    fnb[slot] = fe[slot].nbr;
     FE<float> g;
     decode_fe(fe[slot], g);
     const float *dr = g.isL ? g.drL : g.drR;
@@ -1269,6 +1274,7 @@ template <typename T> FusedArgs<T> fused_args(fvm_ctx *c, double dtmax) {
   A.gbase = (const int *)z.gbase.p;
   A.fe = (const FaceEndRec<T> *)z.fe.p;
   A.fdr = (const float *)z.fdr.p;
+  A.fnb = (const int *)z.fnb.p;
   A.cvol = (const double *)z.cvol.p;
   A.lmin = (const double *)z.lmin.p;
   A.mvel = (const double *)z.mvel.p;
@@ -1355,9 +1361,11 @@ template <typename T> int fused_setup_t(fvm_ctx *c) {
                      c->prm.domain_size[2], (const int *)z.gbase.p, (FaceEndRec<T> *)z.fe.p);
   FVM_HIPCHK(c, hipGetLastError());
   if constexpr (std::is_same<T, float>::value) {
-    if ((rc = fvm_ensure(c, z.fdr, 3 * sizeof(float) * z.slots))) return rc;
+    if ((rc = fvm_ensure(c, z.fdr, 3 * sizeof(float) * z.slots)) || (rc = fvm_ensure(c, z.fnb, sizeof(int) * z.slots)))
+      return rc;
+    FVM_HIPCHK(c, hipMemsetAsync(z.fnb.p, 0, sizeof(int) * z.slots, c->stream));
     hipLaunchKernelGGL(k_build_dr, dim3(nb), dim3(256), 0, c->stream, n, (const int *)z.deg.p,
-                       (const int *)z.gbase.p, (const FaceEndRec<float> *)z.fe.p, (float *)z.fdr.p);
+                       (const int *)z.gbase.p, (const FaceEndRec<float> *)z.fe.p, (float *)z.fdr.p, (int *)z.fnb.p);
     FVM_HIPCHK(c, hipGetLastError());
   }
   struct {

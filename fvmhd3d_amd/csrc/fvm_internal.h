@@ -47,6 +47,7 @@ struct FusedState {
   // internal (window-sorted) cell order: perm[i] = mesh cell of internal cell i
   DBuf perm, inv, deg, gbase, fe;
   DBuf fdr;  // FP32 path: own-side centroid offset per face-end slot (3 floats)
+  DBuf fnb;  // FP32 path: the face-end records' neighbour field, compact (one int per slot)
   DBuf U, W64, cvol, mvel;  // FP64 state and geometry in internal order
   DBuf lmin;                // (3V/4pi)^(1/3) per cell (the volume is fixed on this path)
   DBuf rec[2], wc[2], tv, mm, dt_new, dtmin, tdev, hist;
