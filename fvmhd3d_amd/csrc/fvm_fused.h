@@ -1077,14 +1077,43 @@ __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
   // contiguous bytes per wavefront and step, against 2 KiB of records
   const int j1 = fe_nbr(A.fnb[deg > 1 ? e0 + 64 : 0]);
   int jn = A.fnb[deg > 2 ? e0 + 128 : 0];  // raw, decoded where it is used
-  // own state at the half step, extrapolated per face-end along drL / drR
+  // own state at the half step, extrapolated per face-end along drL / drR.
+  // The wavefront's 64 cells are consecutive, so their records are one
+  // contiguous 10 KiB block: read through LDS with 10 loads of 1 KiB (lane l of
+  // load k: byte 1024k + 16l of the block), then each lane takes its record
+  // from 160 l.  Past the last cell the buffer form reads zeros and the global
+  // form re-reads the first record.
   T Wc[NF], gc[3 * NF];
-  if (live) {
-    const T *rc = A.rec + (size_t)c * REC32;
-    ldv<NF>(rc, Wc);
-    ldv<3 * NF>(rc + R32_G, gc);
+  {
+    const int cw = c - lane;
+#pragma unroll
+    for (int k = 0; k < GQ; k++) {
+      const size_t off = (size_t)cw * (REC32 * sizeof(float)) + 1024u * k + 16u * (unsigned)lane;
+      if constexpr (OFF32) {  // below 2^24 cells the offset fits 32 bits
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(G.rs, (lds_void_t *)(buf1 + 64 * k), 16, (int)(unsigned)off, 0, 0, 0);
+      } else {
+        const bool in = off < (size_t)A.n * (REC32 * sizeof(float));
+        const float *src = reinterpret_cast<const float *>(reinterpret_cast<const char *>(A.rec) + (in ? off : 0));
+        __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)(buf1 + 64 * k), 16, 0, 0);
+      }
+    }
   }
   gather_async<OFF32>(G, deg > 0 ? fe_nbr(fe_rec(rcur).nbr) : 0, buf0, lane);
+  __builtin_amdgcn_s_waitcnt(WAIT_VMGQ);  // the own records are in LDS (step 0's gather may not be)
+  wave_lds_sync();
+  {
+    const unsigned ob = myb + (unsigned)(GBUF * sizeof(float4));
+    T x[4 * GQ];
+#pragma unroll
+    for (int k = 0; k < GQ; k++) {
+      const f4v_t w = *(lds_cf4_t *)(size_t)(ob + 16u * (unsigned)k);
+      x[4 * k] = w.x; x[4 * k + 1] = w.y; x[4 * k + 2] = w.z; x[4 * k + 3] = w.w;
+    }
+    for (int k = 0; k < NF; k++) Wc[k] = x[k];
+    for (int k = 0; k < 3 * NF; k++) gc[k] = x[R32_G + k];
+  }
+  __builtin_amdgcn_s_waitcnt(WAIT_LGKM);  // every lane's reads of the second buffer are done
+  wave_lds_sync();
   gather_async<OFF32>(G, deg > 1 ? j1 : 0, buf1, lane);
 
   const StepTimes st = step_times(A);
