@@ -97,11 +97,16 @@ def test_fused_conservation_and_determinism():
         assert abs(tot1[k] - tot0[k]) <= 1e-12 * mag[k] + 1e-15, (k, tot1[k] - tot0[k])
 
 
-def test_fused_fp32_addressing_variants_agree(monkeypatch):
+@pytest.mark.parametrize("n", [12, 13])
+def test_fused_fp32_addressing_variants_agree(monkeypatch, n):
     """The FP32 flux kernel addresses the neighbour records with 32-bit offsets
     up to 2^24 cells and with 64-bit addresses beyond; FVM_FUSED_ADDR64 (read
-    at every launch) forces the latter.  Both must give the same bits."""
-    m, st, lim = _case("blast", 12)
+    at every launch) forces the latter.  Both must give the same bits: with
+    12^3 = 1728 cells (27 whole wavefronts) and 13^3 = 2197 (the last
+    wavefront partly past the last cell, whose block read of its own records
+    runs past the array: zeros in the 32-bit form, the first record in the
+    64-bit form, neither used)."""
+    m, st, lim = _case("blast", n)
     monkeypatch.delenv("FVM_FUSED_ADDR64", raising=False)
     a = _run_gpu(st.copy(), lim, G.FUSED_FP32, 4, 1.0)
     monkeypatch.setenv("FVM_FUSED_ADDR64", "1")
