@@ -1127,7 +1127,10 @@ __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
   // are read no more.
   for (int q = 0, e = e0; q < maxdeg; q++, e += 64) {
     // step q's records are in LDS: all but the last GQ loads (step q+1's
-    // gather, issued after step q's) have completed
+    // gather, issued after step q's) have completed.  From this wait until its
+    // next gather is issued the wavefront runs at raised priority, so that the
+    // other wavefront's face arithmetic does not delay its loads.
+    __builtin_amdgcn_s_setprio(2);
     __builtin_amdgcn_s_waitcnt(WAIT_VMGQ);
     wave_lds_sync();
     const bool act = q < deg;
@@ -1151,6 +1154,7 @@ __global__ void __launch_bounds__(FB) k_flux_apply_coop(FusedArgs<float> A) {
     rcur = fe_raw(A.fe, q + 1 < deg ? e + 64 : 0);
     const int jnext = A.fnb[q + 3 < deg ? e + 192 : 0];
     gather_async<OFF32>(G, q + 2 < deg ? fe_nbr(jn) : 0, (q & 1) ? buf1 : buf0, lane);
+    __builtin_amdgcn_s_setprio(0);
     jn = jnext;
     if (act) {
       T Wo[NF];
